@@ -1,0 +1,129 @@
+/*
+ * spined.h -- entry points of libspined beyond the ls_* ABI.
+ *
+ * SpinED's ground-state solve calls PRIMME's eigh with a Haskell callback
+ * that crosses into ls_operator_matmat with host vectors on every iteration
+ * (src/SpinED.hs:383-404, src/SpinED/Internal.hs:411-429, :457-458).  The
+ * device-resident replacement is spined_eigsh: the whole Lanczos solve runs
+ * on the GPU and only eigenvalues, residuals and (optionally) eigenvectors
+ * come back.  It is the one line SpinED's `diagonalize` would change
+ * (SpinED.hs:404).  The rest of this header is the device-pointer matvec
+ * used by tests and benchmarks, the batched solver and the learned start.
+ */
+#ifndef SPINED_H
+#define SPINED_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "lattice_symmetries.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Storage / arithmetic mode of the eigensolver. */
+typedef enum {
+  SPINED_PRECISION_FP64 = 0,  /* vectors stored and accumulated in FP64 */
+  SPINED_PRECISION_MIXED = 1, /* FP32 vector storage, FP64 row accumulation and
+                                 FP64 recurrence / dots / projected solve, then
+                                 an FP64 refinement phase to the tolerance */
+} spined_precision;
+
+typedef enum {
+  SPINED_START_RANDOM = 0,  /* v0[r] = U(-1, 1) from a counter hash of (seed, r) */
+  SPINED_START_VECTOR = 1,  /* host vector in start_vector (length = dimension) */
+  SPINED_START_LEARNED = 2, /* amplitudes from the periodic CNN in `model` */
+} spined_start;
+
+typedef struct spined_eigsh_options {
+  int32_t precision;        /* spined_precision */
+  int32_t start;            /* spined_start */
+  uint64_t seed;
+  int32_t max_basis_size;   /* Lanczos basis m (max_primme_basis_size); 0 = auto */
+  int32_t min_restart_size; /* vectors kept on restart (min_primme_restart_size); 0 = auto */
+  int32_t max_block_size;   /* max_primme_block_size; accepted, block solver not used */
+  int32_t verbose;          /* 1 = one log line per restart on stderr */
+  int64_t max_matvecs;      /* 0 = auto */
+  const double *start_vector;
+  const void *model;        /* spined_cnn* for SPINED_START_LEARNED */
+} spined_eigsh_options;
+
+typedef struct spined_monitor_info {
+  int64_t matvecs;
+  int32_t restarts;
+  int32_t phase;            /* 0 = FP64, 1 = FP32-storage phase */
+  int32_t nconverged;
+  int32_t k;
+  const double *ritz_values;   /* k values */
+  const double *residuals;     /* k residual estimates */
+} spined_monitor_info;
+
+/* Called once per restart cycle (mirrors PrimmeMonitor, SpinED.hs:379-382).
+ * Return non-zero to stop. */
+typedef int (*spined_monitor_fn)(const spined_monitor_info *info, void *ctx);
+
+typedef struct spined_stats {
+  int64_t matvecs;          /* total H*v applications */
+  int64_t matvecs_mixed;    /* of which in the FP32-storage phase */
+  int32_t restarts;
+  double seconds_total;
+  double seconds_matvec;    /* wall time inside matvec kernels (events) */
+  double seconds_start;     /* start-vector construction */
+} spined_stats;
+
+void spined_eigsh_options_init(spined_eigsh_options *opts);
+
+/* Lowest k eigenpairs of the operator to residual ||H v - lambda v|| <= tol
+ * (absolute, unit v).  evecs: host, column-major size x k with leading
+ * dimension ld (may be NULL); dtype LS_FLOAT64 or LS_FLOAT32 selects its
+ * element type (datatype of SpinED.hs:142-156).  Replaces eigh at
+ * SpinED.hs:404. */
+int spined_eigsh(const ls_operator *op, int dtype, uint64_t k, double tol,
+                 const spined_eigsh_options *opts, double *evals, void *evecs, uint64_t ld,
+                 double *rnorms, spined_monitor_fn monitor, void *monitor_ctx);
+/* Statistics of the last spined_eigsh on this operator. */
+int spined_get_last_stats(const ls_operator *op, spined_stats *out);
+
+/* ---- device-resident helpers ------------------------------------------- */
+/* The library's HIP stream for the current device (hipStream_t). */
+void *spined_stream(void);
+int spined_synchronize(void);
+/* Device pointer to the basis' representatives (ascending) and stabiliser
+ * sizes; valid while the basis lives and is built. */
+int spined_basis_device_states(const ls_spin_basis *basis, const uint64_t **reps,
+                               const uint32_t **stabilizers, uint64_t *size);
+/* y = H x with DEVICE pointers, enqueued on spined_stream(); dtype as in
+ * ls_operator_matmat plus mixed mode: x float32 / y float64 when
+ * dtype = LS_FLOAT32 and y_is_fp64 = 1. */
+int spined_operator_matmat_device(const ls_operator *op, int dtype, uint64_t size,
+                                  uint64_t block_size, const void *x, uint64_t x_stride,
+                                  void *y, uint64_t y_stride, int y_is_fp64);
+/* Number of off-diagonal terms (sigma states) H generates from the basis:
+ * the per-matvec unit count behind the algorithmic byte figure. */
+int spined_operator_count_offdiag(const ls_operator *op, uint64_t *out);
+/* Orbit method used by the matvec: 0 = generic per-element programs,
+ * 1 = ring (rotation/reflection run pruning). */
+int spined_operator_orbit_method(const ls_operator *op);
+/* Force an orbit method (for parity tests): -1 = automatic. */
+int spined_operator_set_orbit_method(ls_operator *op, int method);
+/* Tunables picked by the autotuner: bucket bits of the index table. */
+int spined_basis_set_bucket_bits(ls_spin_basis *basis, int bits);
+int spined_basis_get_bucket_bits(const ls_spin_basis *basis);
+/* Exact sector dimension by character counting (before ls_build). */
+int spined_basis_sector_dimension(const ls_spin_basis *basis, double *out);
+
+/* ---- test hooks (no counterpart in the reference) ---------------------- */
+/* Runs the device orbit code (the same inline functions the kernels use) on
+ * the host for `count` states: method 0 = generic programs, 1 = ring.  For
+ * CPU-side verification against the oracle; no compute path uses it. */
+int spined_debug_orbit(const ls_spin_basis *basis, int method, uint64_t count,
+                       const uint64_t *states, uint64_t *reps, int32_t *phases, int32_t *counts,
+                       uint8_t *valid);
+/* 1 if the basis' group is a ring (rotation/reflection) group. */
+int spined_debug_ring_available(const ls_spin_basis *basis);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,556 @@
+// api.cpp -- the ls_* C ABI (include/lattice_symmetries.h) and the spined_*
+// entry points (include/spined.h).  Each function cites the reference
+// binding it serves in src/SpinED/Internal.hs.
+#include <cstdarg>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "engine.hpp"
+
+using namespace spined;
+
+struct ls_symmetry {
+  std::shared_ptr<const Symmetry> p;
+};
+struct ls_group {
+  std::shared_ptr<const Group> p;
+};
+struct ls_spin_basis {
+  std::shared_ptr<Basis> p;
+};
+struct ls_states {
+  std::shared_ptr<std::vector<uint64_t>> data;
+};
+struct ls_interaction {
+  std::shared_ptr<const Interaction> p;
+};
+struct ls_operator {
+  std::shared_ptr<Operator> p;
+};
+
+namespace spined {
+
+static std::atomic<bool> g_logging{false};
+
+bool logging_enabled() { return g_logging.load(std::memory_order_relaxed); }
+
+void log_line(const char *fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  fprintf(stderr, "[spined] %s\n", buf);
+  fflush(stderr);
+}
+
+[[noreturn]] void throw_hip(hipError_t e, const char *expr, const char *file, int line) {
+  (void)hipGetLastError();
+  throw Error(LS_DEVICE_ERROR, std::string(hipGetErrorString(e)) + " in " + expr + " at " +
+                                   file + ":" + std::to_string(line));
+}
+
+int current_device() {
+  int d = 0;
+  SPINED_HIP(hipGetDevice(&d));
+  return d;
+}
+
+hipStream_t stream() {
+  static std::mutex mu;
+  static std::vector<hipStream_t> streams;
+  const int d = current_device();
+  std::lock_guard<std::mutex> lk(mu);
+  if (static_cast<int>(streams.size()) <= d) streams.resize(d + 1, nullptr);
+  if (!streams[d]) SPINED_HIP(hipStreamCreateWithFlags(&streams[d], hipStreamNonBlocking));
+  return streams[d];
+}
+
+} // namespace spined
+
+namespace {
+
+const char *message_of(int status) {
+  switch (status) {
+  case LS_SUCCESS: return "no error";
+  case LS_INVALID_PERMUTATION: return "argument is not a valid permutation";
+  case LS_INVALID_SECTOR: return "invalid symmetry sector: must be smaller than the periodicity";
+  case LS_INCOMPATIBLE_SYMMETRIES: return "symmetries are incompatible";
+  case LS_INVALID_NUMBER_SPINS: return "invalid number of spins";
+  case LS_INVALID_HAMMING_WEIGHT: return "invalid Hamming weight";
+  case LS_INVALID_SPIN_INVERSION: return "invalid spin inversion";
+  case LS_OUT_OF_MEMORY: return "failed to allocate memory";
+  case LS_CACHE_NOT_BUILT: return "basis has not been built; call ls_build first";
+  case LS_INVALID_ARGUMENT: return "invalid argument";
+  case LS_NOT_CONVERGED: return "eigensolver did not converge";
+  case LS_INVALID_DATATYPE: return "invalid or unsupported datatype";
+  case LS_DIMENSION_MISMATCH: return "dimensions do not match";
+  case LS_DEVICE_ERROR: return "GPU (HIP) error";
+  case LS_NOT_IMPLEMENTED: return "not implemented";
+  case LS_INTERNAL_ERROR: return "internal error";
+  default: return "unknown error";
+  }
+}
+
+// Make sure the basis' group tables are on the device.
+void ensure_basis_device(Basis &b) { ensure_device(b.tables); }
+
+void require_built(const Basis &b) {
+  if (!b.built) throw Error(LS_CACHE_NOT_BUILT, "basis has not been built");
+}
+
+VecType vec_type(int dtype) {
+  switch (dtype) {
+  case LS_FLOAT32: return VecType::F32;
+  case LS_FLOAT64: return VecType::F64;
+  case LS_COMPLEX64: return VecType::C64;
+  case LS_COMPLEX128: return VecType::C128;
+  default: throw Error(LS_INVALID_DATATYPE, "datatype must be 0..3");
+  }
+}
+
+size_t elem_size(int dtype) {
+  switch (dtype) {
+  case LS_FLOAT32: return 4;
+  case LS_FLOAT64: return 8;
+  case LS_COMPLEX64: return 8;
+  default: return 16;
+  }
+}
+
+// Locks the operator's basis for device work: builds are serialised with
+// matvecs, and device tables are uploaded on first use.
+Operator &prepare_operator(const ls_operator *op) {
+  if (!op || !op->p) throw Error(LS_INVALID_ARGUMENT, "operator is NULL");
+  Operator &o = *op->p;
+  Basis &b = *o.basis;
+  require_built(b);
+  ensure_basis_device(b);
+  ensure_device(o.tables);
+  return o;
+}
+
+int create_interaction(ls_interaction **out, const void *matrix, unsigned number_sites,
+                       const uint16_t *sites, int n) {
+  return guarded([&] {
+    if (!out || (!matrix) || (number_sites && !sites))
+      throw Error(LS_INVALID_ARGUMENT, "NULL argument");
+    auto it = std::make_shared<Interaction>();
+    it->n = n;
+    const int dim = 1 << n;
+    const double *m = static_cast<const double *>(matrix);
+    it->matrix.resize(dim * dim);
+    for (int i = 0; i < dim * dim; ++i) {
+      it->matrix[i] = cplx(m[2 * i], m[2 * i + 1]);
+      if (m[2 * i + 1] != 0.0) it->is_real = false;
+    }
+    for (unsigned t = 0; t < number_sites; ++t) {
+      std::array<uint16_t, 4> s{0, 0, 0, 0};
+      for (int k = 0; k < n; ++k) {
+        s[k] = sites[t * n + k];
+        if (s[k] >= 64) throw Error(LS_INVALID_ARGUMENT, "site index must be < 64");
+        for (int q = 0; q < k; ++q)
+          if (s[q] == s[k]) throw Error(LS_INVALID_ARGUMENT, "repeated site in an interaction");
+      }
+      it->sites.push_back(s);
+    }
+    *out = new ls_interaction{it};
+  });
+}
+
+} // namespace
+
+extern "C" {
+
+// ---- errors & logging (Internal.hs:39-54) ----------------------------------
+const char *ls_error_to_string(int status) {
+  const char *m = message_of(status);
+  char *s = static_cast<char *>(std::malloc(std::strlen(m) + 1));
+  if (s) std::strcpy(s, m);
+  return s;
+}
+
+void ls_destroy_string(const char *message) { std::free(const_cast<char *>(message)); }
+void ls_enable_logging(void) { g_logging.store(true); }
+void ls_disable_logging(void) { g_logging.store(false); }
+
+// ---- symmetry (Internal.hs:69-116) -----------------------------------------
+int ls_create_symmetry(ls_symmetry **out, unsigned length, const unsigned permutation[],
+                       unsigned sector) {
+  return guarded([&] {
+    if (!out || (length && !permutation)) throw Error(LS_INVALID_ARGUMENT, "NULL argument");
+    auto s = std::make_shared<Symmetry>();
+    s->perm.assign(permutation, permutation + length);
+    s->periodicity = periodicity_of(s->perm);
+    if (s->periodicity == 0) throw Error(LS_INVALID_PERMUTATION, "not a permutation");
+    if (sector >= s->periodicity) throw Error(LS_INVALID_SECTOR, "sector out of range");
+    s->sector = sector;
+    *out = new ls_symmetry{s};
+  });
+}
+
+void ls_destroy_symmetry(ls_symmetry *symmetry) { delete symmetry; }
+unsigned ls_get_sector(const ls_symmetry *s) { return s ? s->p->sector : 0; }
+double ls_get_phase(const ls_symmetry *s) {
+  return s ? static_cast<double>(s->p->sector) / static_cast<double>(s->p->periodicity) : 0.0;
+}
+unsigned ls_get_periodicity(const ls_symmetry *s) { return s ? s->p->periodicity : 0; }
+
+// ---- group (Internal.hs:120-155) ------------------------------------------
+int ls_create_group(ls_group **out, unsigned size, const ls_symmetry *generators[]) {
+  return guarded([&] {
+    if (!out || (size && !generators)) throw Error(LS_INVALID_ARGUMENT, "NULL argument");
+    std::vector<Symmetry> gens;
+    for (unsigned i = 0; i < size; ++i) {
+      if (!generators[i]) throw Error(LS_INVALID_ARGUMENT, "NULL generator");
+      gens.push_back(*generators[i]->p);
+    }
+    *out = new ls_group{close_group(gens)};
+  });
+}
+
+void ls_destroy_group(ls_group *group) { delete group; }
+unsigned ls_get_group_size(const ls_group *g) {
+  return g ? static_cast<unsigned>(g->p->perms.size()) : 0;
+}
+
+// ---- basis (Internal.hs:172-254) -------------------------------------------
+int ls_create_spin_basis(ls_spin_basis **out, const ls_group *group, unsigned number_spins,
+                         int hamming_weight, int spin_inversion) {
+  return guarded([&] {
+    if (!out || !group) throw Error(LS_INVALID_ARGUMENT, "NULL argument");
+    const Group &g = *group->p;
+    if (number_spins == 0 || number_spins > 64)
+      throw Error(LS_INVALID_NUMBER_SPINS, "number of spins must be in [1, 64]");
+    if (g.n != 0 && static_cast<unsigned>(g.n) != number_spins)
+      throw Error(LS_INVALID_NUMBER_SPINS, "number of spins differs from the symmetries' size");
+    if (hamming_weight < -1 || hamming_weight > static_cast<int>(number_spins))
+      throw Error(LS_INVALID_HAMMING_WEIGHT, "Hamming weight must be -1 or in [0, n]");
+    if (spin_inversion < -1 || spin_inversion > 1)
+      throw Error(LS_INVALID_SPIN_INVERSION, "spin inversion must be -1, 0 or +1");
+    if (spin_inversion != 0 && hamming_weight >= 0 &&
+        2 * hamming_weight != static_cast<int>(number_spins))
+      throw Error(LS_INVALID_SPIN_INVERSION, "spin inversion needs hamming_weight = n/2");
+    auto b = std::make_shared<Basis>();
+    b->group = group->p;
+    b->n = static_cast<int>(number_spins);
+    b->w = hamming_weight;
+    b->z = spin_inversion;
+    build_orbit_tables(g, b->n, spin_inversion, b->tables);
+    *out = new ls_spin_basis{b};
+  });
+}
+
+void ls_destroy_spin_basis(ls_spin_basis *basis) { delete basis; }
+
+int ls_build(ls_spin_basis *basis) {
+  return guarded([&] {
+    if (!basis) throw Error(LS_INVALID_ARGUMENT, "NULL basis");
+    Basis &b = *basis->p;
+    std::lock_guard<std::mutex> lk(b.mu);
+    if (b.built) return;
+    ensure_basis_device(b);
+    build_basis(b);
+  });
+}
+
+int ls_build_unsafe(ls_spin_basis *basis, uint64_t size, const uint64_t representatives[]) {
+  return guarded([&] {
+    if (!basis || (size && !representatives)) throw Error(LS_INVALID_ARGUMENT, "NULL argument");
+    Basis &b = *basis->p;
+    std::lock_guard<std::mutex> lk(b.mu);
+    ensure_basis_device(b);
+    adopt_basis(b, size, representatives);
+  });
+}
+
+int ls_get_number_states(const ls_spin_basis *basis, uint64_t *out) {
+  return guarded([&] {
+    if (!basis || !out) throw Error(LS_INVALID_ARGUMENT, "NULL argument");
+    const Basis &b = *basis->p;
+    require_built(b);
+    *out = b.D;
+  });
+}
+
+int ls_get_states(ls_states **out, const ls_spin_basis *basis) {
+  return guarded([&] {
+    if (!basis || !out) throw Error(LS_INVALID_ARGUMENT, "NULL argument");
+    Basis &b = *basis->p;
+    std::lock_guard<std::mutex> lk(b.mu);
+    require_built(b);
+    if (!b.host_reps) {
+      auto h = std::make_shared<std::vector<uint64_t>>(b.D);
+      if (b.D)
+        SPINED_HIP(hipMemcpy(h->data(), b.reps.ptr, b.D * 8, hipMemcpyDeviceToHost));
+      b.host_reps = h;
+    }
+    *out = new ls_states{b.host_reps};
+  });
+}
+
+const uint64_t *ls_states_get_data(const ls_states *states) {
+  return states && states->data ? states->data->data() : nullptr;
+}
+uint64_t ls_states_get_size(const ls_states *states) {
+  return states && states->data ? states->data->size() : 0;
+}
+void ls_destroy_states(ls_states *states) { delete states; }
+
+// ---- interactions (Internal.hs:256-367) ------------------------------------
+int ls_create_interaction1(ls_interaction **out, const void *matrix, unsigned number_sites,
+                           const uint16_t *sites) {
+  return create_interaction(out, matrix, number_sites, sites, 1);
+}
+int ls_create_interaction2(ls_interaction **out, const void *matrix, unsigned number_sites,
+                           const uint16_t *sites) {
+  return create_interaction(out, matrix, number_sites, sites, 2);
+}
+int ls_create_interaction3(ls_interaction **out, const void *matrix, unsigned number_sites,
+                           const uint16_t *sites) {
+  return create_interaction(out, matrix, number_sites, sites, 3);
+}
+int ls_create_interaction4(ls_interaction **out, const void *matrix, unsigned number_sites,
+                           const uint16_t *sites) {
+  return create_interaction(out, matrix, number_sites, sites, 4);
+}
+bool ls_interaction_is_real(const ls_interaction *it) { return it && it->p->is_real; }
+void ls_destroy_interaction(ls_interaction *it) { delete it; }
+
+// ---- operator (Internal.hs:369-458) ----------------------------------------
+int ls_create_operator(ls_operator **out, const ls_spin_basis *basis, unsigned number_terms,
+                       const ls_interaction *const terms[]) {
+  return guarded([&] {
+    if (!out || !basis || (number_terms && !terms))
+      throw Error(LS_INVALID_ARGUMENT, "NULL argument");
+    auto op = std::make_shared<Operator>();
+    op->basis = basis->p;
+    for (unsigned i = 0; i < number_terms; ++i) {
+      if (!terms[i]) throw Error(LS_INVALID_ARGUMENT, "NULL interaction");
+      op->terms.push_back(terms[i]->p);
+    }
+    compile_operator(*op);
+    *out = new ls_operator{op};
+  });
+}
+
+void ls_destroy_operator(ls_operator *op) { delete op; }
+
+bool ls_operator_is_real(const ls_operator *op) { return op && op->p->tables.is_real; }
+
+int ls_operator_matmat(const ls_operator *op, int dtype, uint64_t size, uint64_t block_size,
+                       const void *x, uint64_t x_stride, void *y, uint64_t y_stride) {
+  return guarded([&] {
+    Operator &o = prepare_operator(op);
+    const VecType vt = vec_type(dtype);
+    if (size != o.basis->D) throw Error(LS_DIMENSION_MISMATCH, "size differs from dimension");
+    if (block_size == 0) return;
+    if (!x || !y) throw Error(LS_INVALID_ARGUMENT, "NULL vector");
+    if (x_stride < size || y_stride < size)
+      throw Error(LS_INVALID_ARGUMENT, "stride smaller than size");
+    const size_t es = elem_size(dtype);
+    hipStream_t s = stream();
+    DeviceBuffer<char> dx(size * es), dy(size * es);
+    for (uint64_t c = 0; c < block_size; ++c) {
+      const char *xc = static_cast<const char *>(x) + c * x_stride * es;
+      char *yc = static_cast<char *>(y) + c * y_stride * es;
+      SPINED_HIP(hipMemcpyAsync(dx.ptr, xc, size * es, hipMemcpyHostToDevice, s));
+      launch_matvec(MatvecCall{&o, vt, vt, dx.ptr, dy.ptr, size, s});
+      SPINED_HIP(hipMemcpyAsync(yc, dy.ptr, size * es, hipMemcpyDeviceToHost, s));
+    }
+    SPINED_HIP(hipStreamSynchronize(s));
+  });
+}
+
+int ls_operator_expectation(const ls_operator *op, int dtype, uint64_t size, uint64_t block_size,
+                            const void *x, uint64_t x_stride, void *out) {
+  return guarded([&] {
+    Operator &o = prepare_operator(op);
+    (void)vec_type(dtype);
+    if (size != o.basis->D) throw Error(LS_DIMENSION_MISMATCH, "size differs from dimension");
+    if (block_size && (!x || !out)) throw Error(LS_INVALID_ARGUMENT, "NULL argument");
+    const size_t es = elem_size(dtype);
+    std::vector<char> y(size * es);
+    double *res = static_cast<double *>(out);
+    for (uint64_t c = 0; c < block_size; ++c) {
+      const char *xc = static_cast<const char *>(x) + c * x_stride * es;
+      const int st = ls_operator_matmat(op, dtype, size, 1, xc, size, y.data(), size);
+      if (st) throw Error(st, "matvec failed");
+      double re = 0.0, im = 0.0;
+      for (uint64_t i = 0; i < size; ++i) {
+        switch (dtype) {
+        case LS_FLOAT32:
+          re += static_cast<double>(reinterpret_cast<const float *>(xc)[i]) *
+                reinterpret_cast<const float *>(y.data())[i];
+          break;
+        case LS_FLOAT64:
+          re += reinterpret_cast<const double *>(xc)[i] * reinterpret_cast<const double *>(y.data())[i];
+          break;
+        case LS_COMPLEX64: {
+          const float *a = reinterpret_cast<const float *>(xc) + 2 * i;
+          const float *b = reinterpret_cast<const float *>(y.data()) + 2 * i;
+          re += static_cast<double>(a[0]) * b[0] + static_cast<double>(a[1]) * b[1];
+          im += static_cast<double>(a[0]) * b[1] - static_cast<double>(a[1]) * b[0];
+          break;
+        }
+        default: {
+          const double *a = reinterpret_cast<const double *>(xc) + 2 * i;
+          const double *b = reinterpret_cast<const double *>(y.data()) + 2 * i;
+          re += a[0] * b[0] + a[1] * b[1];
+          im += a[0] * b[1] - a[1] * b[0];
+        }
+        }
+      }
+      res[2 * c] = re;
+      res[2 * c + 1] = im;
+    }
+  });
+}
+
+// ---- spined_* ---------------------------------------------------------------
+void spined_eigsh_options_init(spined_eigsh_options *o) {
+  if (!o) return;
+  std::memset(o, 0, sizeof(*o));
+  o->precision = SPINED_PRECISION_FP64;
+  o->start = SPINED_START_RANDOM;
+  o->seed = 0x5eed;
+}
+
+int spined_eigsh(const ls_operator *op, int dtype, uint64_t k, double tol,
+                 const spined_eigsh_options *opts, double *evals, void *evecs, uint64_t ld,
+                 double *rnorms, spined_monitor_fn monitor, void *monitor_ctx) {
+  int status = LS_SUCCESS;
+  const int g = guarded([&] {
+    Operator &o = prepare_operator(op);
+    if (!evals) throw Error(LS_INVALID_ARGUMENT, "evals is NULL");
+    spined_eigsh_options d;
+    spined_eigsh_options_init(&d);
+    status = eigsh(o, dtype, k, tol, opts ? *opts : d, evals, evecs, ld, rnorms, monitor,
+                   monitor_ctx);
+  });
+  return g ? g : status;
+}
+
+int spined_get_last_stats(const ls_operator *op, spined_stats *out) {
+  return guarded([&] {
+    if (!op || !out) throw Error(LS_INVALID_ARGUMENT, "NULL argument");
+    std::lock_guard<std::mutex> lk(op->p->mu);
+    *out = op->p->last_stats;
+  });
+}
+
+void *spined_stream(void) {
+  try {
+    return static_cast<void *>(stream());
+  } catch (...) {
+    return nullptr;
+  }
+}
+
+int spined_synchronize(void) {
+  return guarded([] { SPINED_HIP(hipStreamSynchronize(stream())); });
+}
+
+int spined_basis_device_states(const ls_spin_basis *basis, const uint64_t **reps,
+                               const uint32_t **stabilizers, uint64_t *size) {
+  return guarded([&] {
+    if (!basis) throw Error(LS_INVALID_ARGUMENT, "NULL basis");
+    const Basis &b = *basis->p;
+    require_built(b);
+    if (reps) *reps = b.reps.ptr;
+    if (stabilizers) *stabilizers = b.stab.ptr;
+    if (size) *size = b.D;
+  });
+}
+
+int spined_operator_matmat_device(const ls_operator *op, int dtype, uint64_t size,
+                                  uint64_t block_size, const void *x, uint64_t x_stride,
+                                  void *y, uint64_t y_stride, int y_is_fp64) {
+  return guarded([&] {
+    Operator &o = prepare_operator(op);
+    const VecType xt = vec_type(dtype);
+    VecType yt = xt;
+    if (y_is_fp64) {
+      if (xt != VecType::F32) throw Error(LS_INVALID_DATATYPE, "y_is_fp64 needs float32 x");
+      yt = VecType::F64;
+    }
+    if (size != o.basis->D) throw Error(LS_DIMENSION_MISMATCH, "size differs from dimension");
+    if (x_stride < size || y_stride < size)
+      throw Error(LS_INVALID_ARGUMENT, "stride smaller than size");
+    const size_t xs = elem_size(dtype), ys = y_is_fp64 ? 8 : elem_size(dtype);
+    for (uint64_t c = 0; c < block_size; ++c)
+      launch_matvec(MatvecCall{&o, xt, yt, static_cast<const char *>(x) + c * x_stride * xs,
+                               static_cast<char *>(y) + c * y_stride * ys, size, stream()});
+  });
+}
+
+int spined_operator_count_offdiag(const ls_operator *op, uint64_t *out) {
+  return guarded([&] {
+    Operator &o = prepare_operator(op);
+    if (!out) throw Error(LS_INVALID_ARGUMENT, "NULL argument");
+    *out = count_offdiag_device(o);
+  });
+}
+
+int spined_operator_orbit_method(const ls_operator *op) {
+  return op ? op->p->orbit_method() : -1;
+}
+
+int spined_operator_set_orbit_method(ls_operator *op, int method) {
+  return guarded([&] {
+    if (!op || method < -1 || method > 1) throw Error(LS_INVALID_ARGUMENT, "bad argument");
+    op->p->orbit_method_override = method;
+  });
+}
+
+int spined_basis_set_bucket_bits(ls_spin_basis *basis, int bits) {
+  return guarded([&] {
+    if (!basis || bits < -1 || bits > 30) throw Error(LS_INVALID_ARGUMENT, "bad argument");
+    Basis &b = *basis->p;
+    std::lock_guard<std::mutex> lk(b.mu);
+    b.bucket_bits_request = bits;
+    if (b.built) build_index(b);
+  });
+}
+
+int spined_basis_get_bucket_bits(const ls_spin_basis *basis) {
+  return basis ? basis->p->bucket_bits : -1;
+}
+
+int spined_basis_sector_dimension(const ls_spin_basis *basis, double *out) {
+  return guarded([&] {
+    if (!basis || !out) throw Error(LS_INVALID_ARGUMENT, "NULL argument");
+    double im = 0.0;
+    *out = sector_dimension(*basis->p, &im);
+  });
+}
+
+// Host execution of the device orbit code (same inline functions) so that
+// the generic and ring algorithms can be checked against the oracle without
+// a GPU.  Not used by any compute path.
+int spined_debug_orbit(const ls_spin_basis *basis, int method, uint64_t count,
+                       const uint64_t *states, uint64_t *reps, int32_t *phases, int32_t *counts,
+                       uint8_t *valid) {
+  return guarded([&] {
+    if (!basis || (count && (!states || !reps))) throw Error(LS_INVALID_ARGUMENT, "NULL");
+    const Basis &b = *basis->p;
+    if (method == 1 && !b.tables.ring_available)
+      throw Error(LS_INVALID_ARGUMENT, "ring method unavailable for this group");
+    const auto ga = b.tables.generic_args(false);
+    const auto ra = b.tables.ring_args(false);
+    for (uint64_t i = 0; i < count; ++i) {
+      const OrbitResult r = method == 1 ? orbit_ring(ra, states[i]) : orbit_generic(ga, states[i]);
+      reps[i] = r.rep;
+      if (phases) phases[i] = r.phase;
+      if (counts) counts[i] = r.count;
+      if (valid) valid[i] = r.valid ? 1 : 0;
+    }
+  });
+}
+
+int spined_debug_ring_available(const ls_spin_basis *basis) {
+  return basis && basis->p->tables.ring_available ? 1 : 0;
+}
+
+} // extern "C"

@@ -1,0 +1,331 @@
+// basis.hip -- symmetry-adapted basis construction on the GPU (ls_build,
+// src/SpinED/Internal.hs:178-182, :230-236; caller SpinED.hs:315-337).
+//
+// Pipeline, per segment of the candidate range:
+//   1. enumerate: thread t owns 64 consecutive candidates (colex rank order,
+//      which is ascending integer order; Gosper's successor steps within the
+//      word), tests each for "orbit minimum with a symmetric state" and
+//      writes a 64-bit hit mask; each block writes its hit count;
+//   2. scan: one block turns the block counts into exclusive offsets;
+//   3. emit: each thread re-walks its word and writes its hits at the block
+//      offset plus its in-block prefix.
+// Because enumeration is monotone and the compaction is order preserving,
+// the representatives come out globally sorted: no sort pass is needed.
+// Stabiliser sizes and the bucket index follow.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <vector>
+
+#include "engine.hpp"
+
+namespace spined {
+
+namespace {
+
+constexpr int kBlock = 256;
+
+struct EnumArgs {
+  const uint64_t *__restrict__ binom; // 65 x 65
+  int n;
+  int w;              // -1: all 2^n states
+  uint64_t total;     // number of candidates
+};
+
+__device__ inline uint64_t unrank_colex(const EnumArgs &e, uint64_t rank) {
+  if (e.w < 0) return rank;
+  uint64_t x = 0;
+  int c = e.n - 1;
+  for (int k = e.w; k >= 1; --k) {
+    while (e.binom[c * 65 + k] > rank) --c;
+    x |= 1ull << c;
+    rank -= e.binom[c * 65 + k];
+    --c;
+  }
+  return x;
+}
+
+__device__ inline uint64_t next_state(const EnumArgs &e, uint64_t v) {
+  if (e.w < 0) return v + 1;
+  // Gosper's hack: next larger integer with the same popcount
+  const uint64_t t = v | (v - 1);
+  return (t + 1) | (((~t & (0 - ~t)) - 1) >> (__builtin_ctzll(v) + 1));
+}
+
+template <int METHOD>
+__device__ inline bool is_basis_state(const GenericOrbitArgs &ga, const RingOrbitArgs &ra,
+                                      uint64_t x) {
+  if constexpr (METHOD == 1) {
+    const OrbitResult r = orbit_ring(ra, x);
+    return r.rep == x && r.valid;
+  } else {
+    return rep_test_generic(ga, x) != 0;
+  }
+}
+
+__device__ inline uint32_t block_sum(uint32_t v, uint32_t *lds) {
+  // wave reduction then across the 4 waves of the block
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) lds[wid] = v;
+  __syncthreads();
+  uint32_t s = 0;
+  if (threadIdx.x == 0)
+    for (int i = 0; i < kBlock / 64; ++i) s += lds[i];
+  return s;
+}
+
+template <int METHOD>
+__global__ void __launch_bounds__(kBlock)
+    enumerate_kernel(EnumArgs e, GenericOrbitArgs ga, RingOrbitArgs ra, uint64_t word0,
+                     uint64_t nwords, uint64_t *__restrict__ masks,
+                     uint32_t *__restrict__ block_counts) {
+  __shared__ uint32_t lds[kBlock / 64];
+  const uint64_t t = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
+  uint64_t bits = 0;
+  if (t < nwords) {
+    const uint64_t r0 = (word0 + t) * 64;
+    const uint64_t rem = e.total - r0;
+    const int cnt = rem < 64 ? static_cast<int>(rem) : 64;
+    uint64_t x = unrank_colex(e, r0);
+    for (int i = 0; i < cnt; ++i) {
+      if (is_basis_state<METHOD>(ga, ra, x)) bits |= 1ull << i;
+      if (i + 1 < cnt) x = next_state(e, x);
+    }
+    masks[t] = bits;
+  }
+  const uint32_t s = block_sum(static_cast<uint32_t>(__popcll(bits)), lds);
+  if (threadIdx.x == 0) block_counts[blockIdx.x] = s;
+}
+
+// Exclusive scan of n block counts into offsets (single block of 1024).
+__global__ void __launch_bounds__(1024)
+    scan_counts_kernel(const uint32_t *__restrict__ counts, uint64_t n,
+                       uint64_t *__restrict__ offsets, uint64_t *__restrict__ total) {
+  __shared__ uint64_t part[1024];
+  const uint64_t per = (n + 1023) / 1024;
+  const uint64_t b = threadIdx.x * per;
+  const uint64_t e = b + per < n ? b + per : n;
+  uint64_t s = 0;
+  for (uint64_t i = b; i < e; ++i) s += counts[i];
+  part[threadIdx.x] = s;
+  __syncthreads();
+  // Hillis-Steele inclusive scan over 1024 partial sums
+  for (int off = 1; off < 1024; off <<= 1) {
+    const uint64_t v = threadIdx.x >= static_cast<unsigned>(off) ? part[threadIdx.x - off] : 0;
+    __syncthreads();
+    part[threadIdx.x] += v;
+    __syncthreads();
+  }
+  uint64_t run = threadIdx.x ? part[threadIdx.x - 1] : 0;
+  for (uint64_t i = b; i < e; ++i) {
+    offsets[i] = run;
+    run += counts[i];
+  }
+  if (threadIdx.x == 1023) *total = part[1023];
+}
+
+__global__ void __launch_bounds__(kBlock)
+    emit_kernel(EnumArgs e, uint64_t word0, uint64_t nwords, const uint64_t *__restrict__ masks,
+                const uint64_t *__restrict__ block_offsets, uint64_t base,
+                uint64_t *__restrict__ reps) {
+  __shared__ uint32_t lds[kBlock];
+  const uint64_t t = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
+  const uint64_t bits = t < nwords ? masks[t] : 0;
+  const uint32_t c = static_cast<uint32_t>(__popcll(bits));
+  lds[threadIdx.x] = c;
+  __syncthreads();
+  for (int off = 1; off < kBlock; off <<= 1) {
+    const uint32_t v = threadIdx.x >= static_cast<unsigned>(off) ? lds[threadIdx.x - off] : 0;
+    __syncthreads();
+    lds[threadIdx.x] += v;
+    __syncthreads();
+  }
+  if (bits == 0) return;
+  uint64_t pos = base + block_offsets[blockIdx.x] + lds[threadIdx.x] - c;
+  const uint64_t r0 = (word0 + t) * 64;
+  const int last = 63 - __clzll(bits);
+  uint64_t x = unrank_colex(e, r0);
+  for (int i = 0; i <= last; ++i) {
+    if ((bits >> i) & 1) reps[pos++] = x;
+    if (i < last) x = next_state(e, x);
+  }
+}
+
+template <int METHOD>
+__global__ void __launch_bounds__(kBlock)
+    stabilizer_kernel(GenericOrbitArgs ga, RingOrbitArgs ra, const uint64_t *__restrict__ reps,
+                      uint64_t D, uint32_t *__restrict__ stab) {
+  const uint64_t i = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
+  if (i >= D) return;
+  OrbitResult r;
+  if constexpr (METHOD == 1)
+    r = orbit_ring(ra, reps[i]);
+  else
+    r = orbit_generic(ga, reps[i]);
+  stab[i] = static_cast<uint32_t>(r.count);
+}
+
+__global__ void __launch_bounds__(kBlock)
+    bucket_kernel(const uint64_t *__restrict__ reps, uint64_t D, int shift,
+                  uint32_t *__restrict__ buckets) {
+  const uint64_t i = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
+  if (i >= D) return;
+  const int64_t bi = static_cast<int64_t>(reps[i] >> shift);
+  const int64_t bp = i ? static_cast<int64_t>(reps[i - 1] >> shift) : -1;
+  for (int64_t b = bp + 1; b <= bi; ++b) buckets[b] = static_cast<uint32_t>(i);
+}
+
+__global__ void __launch_bounds__(kBlock)
+    bucket_tail_kernel(uint64_t first, uint64_t last_inclusive, uint32_t value,
+                       uint32_t *__restrict__ buckets) {
+  const uint64_t b = first + static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
+  if (b <= last_inclusive) buckets[b] = value;
+}
+
+uint64_t binomial(int n, int k) {
+  if (k < 0 || k > n) return 0;
+  static uint64_t table[65][65];
+  static bool ready = false;
+  if (!ready) {
+    for (int a = 0; a <= 64; ++a) {
+      table[a][0] = 1;
+      for (int b = 1; b <= a; ++b) table[a][b] = table[a - 1][b - 1] + (b < a ? table[a - 1][b] : 0);
+    }
+    ready = true;
+  }
+  return table[n][k];
+}
+
+unsigned grid_for(uint64_t n) { return static_cast<unsigned>((n + kBlock - 1) / kBlock); }
+
+void compute_stabilizers(Basis &b, hipStream_t s) {
+  if (b.D == 0) return;
+  const auto ga = b.tables.generic_args(true);
+  const auto ra = b.tables.ring_args(true);
+  if (b.tables.ring_available)
+    stabilizer_kernel<1><<<grid_for(b.D), kBlock, 0, s>>>(ga, ra, b.reps.ptr, b.D, b.stab.ptr);
+  else
+    stabilizer_kernel<0><<<grid_for(b.D), kBlock, 0, s>>>(ga, ra, b.reps.ptr, b.D, b.stab.ptr);
+  SPINED_CHECK_LAUNCH();
+}
+
+} // namespace
+
+void build_index(Basis &b) {
+  hipStream_t s = stream();
+  if (b.D >= 0xffffffffull)
+    throw Error(LS_NOT_IMPLEMENTED, "dimensions of 2^32 - 1 or more are not supported");
+  int bits = b.bucket_bits_request;
+  if (bits < 0) {
+    int lg = 0;
+    while ((1ull << lg) < b.D) ++lg;
+    bits = std::max(1, std::min(30, lg - 1));
+  }
+  bits = std::max(1, std::min(30, bits));
+  uint64_t last = 0;
+  if (b.D) SPINED_HIP(hipMemcpy(&last, b.reps.ptr + (b.D - 1), 8, hipMemcpyDeviceToHost));
+  const int bitlen = last ? 64 - __builtin_clzll(last) : 0;
+  b.shift = std::max(0, bitlen - bits);
+  b.bucket_bits = bits;
+  b.nbuckets = (last >> b.shift) + 1;
+  b.buckets.allocate(b.nbuckets + 1);
+  if (b.D) {
+    bucket_kernel<<<grid_for(b.D), kBlock, 0, s>>>(b.reps.ptr, b.D, b.shift, b.buckets.ptr);
+    SPINED_CHECK_LAUNCH();
+  }
+  const uint64_t first_tail = b.D ? (last >> b.shift) + 1 : 0;
+  const uint64_t ntail = b.nbuckets + 1 - first_tail;
+  bucket_tail_kernel<<<grid_for(ntail), kBlock, 0, s>>>(first_tail, b.nbuckets,
+                                                        static_cast<uint32_t>(b.D), b.buckets.ptr);
+  SPINED_CHECK_LAUNCH();
+  SPINED_HIP(hipStreamSynchronize(s));
+}
+
+void build_basis(Basis &b) {
+  const auto t0 = std::chrono::steady_clock::now();
+  hipStream_t s = stream();
+  const int n = b.n;
+  uint64_t total;
+  if (b.w < 0) {
+    if (n > 48) throw Error(LS_NOT_IMPLEMENTED, "no Hamming weight restriction above 48 spins");
+    total = 1ull << n;
+  } else {
+    total = binomial(n, b.w);
+  }
+  double imag = 0.0;
+  const double expect_d = sector_dimension(b, &imag);
+  const uint64_t capacity = static_cast<uint64_t>(std::llround(std::max(expect_d, 0.0)));
+  if (capacity >= 0xffffffffull)
+    throw Error(LS_NOT_IMPLEMENTED, "dimensions of 2^32 - 1 or more are not supported");
+
+  std::vector<uint64_t> hb(65 * 65);
+  for (int a = 0; a <= 64; ++a)
+    for (int c = 0; c <= 64; ++c) hb[a * 65 + c] = binomial(a, c);
+  DeviceBuffer<uint64_t> binom(hb.size());
+  SPINED_HIP(hipMemcpyAsync(binom.ptr, hb.data(), hb.size() * 8, hipMemcpyHostToDevice, s));
+
+  b.reps.allocate(std::max<uint64_t>(capacity, 1));
+  const uint64_t words_total = (total + 63) / 64;
+  const uint64_t seg_words = std::min<uint64_t>(words_total, 1ull << 22);
+  const uint64_t seg_blocks = (seg_words + kBlock - 1) / kBlock;
+  DeviceBuffer<uint64_t> masks(seg_words);
+  DeviceBuffer<uint32_t> counts(seg_blocks);
+  DeviceBuffer<uint64_t> offsets(seg_blocks + 1);
+  DeviceBuffer<uint64_t> seg_total(1);
+  EnumArgs e{binom.ptr, n, b.w, total};
+  const auto ga = b.tables.generic_args(true);
+  const auto ra = b.tables.ring_args(true);
+  uint64_t emitted = 0;
+  for (uint64_t w0 = 0; w0 < words_total; w0 += seg_words) {
+    const uint64_t nw = std::min(seg_words, words_total - w0);
+    const unsigned nb = grid_for(nw);
+    if (b.tables.ring_available)
+      enumerate_kernel<1><<<nb, kBlock, 0, s>>>(e, ga, ra, w0, nw, masks.ptr, counts.ptr);
+    else
+      enumerate_kernel<0><<<nb, kBlock, 0, s>>>(e, ga, ra, w0, nw, masks.ptr, counts.ptr);
+    SPINED_CHECK_LAUNCH();
+    scan_counts_kernel<<<1, 1024, 0, s>>>(counts.ptr, nb, offsets.ptr, seg_total.ptr);
+    SPINED_CHECK_LAUNCH();
+    uint64_t segc = 0;
+    SPINED_HIP(hipMemcpyAsync(&segc, seg_total.ptr, 8, hipMemcpyDeviceToHost, s));
+    SPINED_HIP(hipStreamSynchronize(s));
+    if (emitted + segc > capacity)
+      throw Error(LS_INTERNAL_ERROR, "basis build found more states than the sector dimension");
+    if (segc) {
+      emit_kernel<<<nb, kBlock, 0, s>>>(e, w0, nw, masks.ptr, offsets.ptr, emitted, b.reps.ptr);
+      SPINED_CHECK_LAUNCH();
+    }
+    emitted += segc;
+  }
+  if (emitted != capacity)
+    throw Error(LS_INTERNAL_ERROR, "basis build found " + std::to_string(emitted) +
+                                       " states, character formula gives " +
+                                       std::to_string(capacity));
+  b.D = emitted;
+  b.stab.allocate(std::max<uint64_t>(b.D, 1));
+  compute_stabilizers(b, s);
+  SPINED_HIP(hipStreamSynchronize(s));
+  build_index(b);
+  b.host_reps.reset();
+  b.built = true;
+  b.seconds_build =
+      std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+void adopt_basis(Basis &b, uint64_t n, const uint64_t *reps) {
+  hipStream_t s = stream();
+  if (n >= 0xffffffffull)
+    throw Error(LS_NOT_IMPLEMENTED, "dimensions of 2^32 - 1 or more are not supported");
+  b.reps.allocate(std::max<uint64_t>(n, 1));
+  if (n) SPINED_HIP(hipMemcpy(b.reps.ptr, reps, n * 8, hipMemcpyHostToDevice));
+  b.D = n;
+  b.stab.allocate(std::max<uint64_t>(n, 1));
+  compute_stabilizers(b, s);
+  SPINED_HIP(hipStreamSynchronize(s));
+  build_index(b);
+  b.host_reps = std::make_shared<std::vector<uint64_t>>(reps, reps + n);
+  b.built = true;
+}
+
+} // namespace spined

@@ -1,0 +1,167 @@
+// engine.hpp -- host-side objects behind the ls_* handles and the kernel
+// launchers they call.  Ownership follows the reference's ForeignPtr model
+// (src/SpinED/Internal.hs:116, :150, :228, :362, :402): every handle holds a
+// shared_ptr, dependants keep their dependencies alive, and the last release
+// (from any thread) frees host and device memory.
+#pragma once
+
+#include <array>
+#include <complex>
+#include <cstdint>
+#include <memory>
+#include <mutex>
+#include <vector>
+
+#include "../../include/spined.h"
+#include "common.hpp"
+#include "orbit.hpp"
+
+namespace spined {
+
+using cplx = std::complex<double>;
+
+struct Symmetry {
+  std::vector<unsigned> perm;
+  unsigned sector = 0;
+  unsigned periodicity = 1;
+};
+
+// Closure of the generators with exact phases (integers modulo L).
+struct Group {
+  int n = 0; // number of sites; 0 when built from no generators
+  std::vector<Symmetry> generators;
+  std::vector<std::vector<uint8_t>> perms; // perms[0] is the identity
+  std::vector<int64_t> phases;
+  int64_t L = 2;
+};
+
+unsigned periodicity_of(const std::vector<unsigned> &perm); // 0 if not a permutation
+std::shared_ptr<Group> close_group(const std::vector<Symmetry> &gens);
+
+// Device tables describing the full symmetry group (permutations x optional
+// spin inversion) of a basis.
+struct OrbitTables {
+  int n = 0;
+  int nperm = 0;
+  int has_flip = 0;
+  int32_t flip_phase = 0;
+  int32_t L = 2;
+  bool trivial_chars = true; // every phase is 0 (incl. the flip)
+  bool real_chars = true;    // every phase is 0 or L/2
+  bool ring_available = false;
+  DeviceBuffer<uint64_t> masks;
+  DeviceBuffer<int32_t> shifts;
+  DeviceBuffer<uint32_t> offsets;
+  DeviceBuffer<uint32_t> flags;
+  DeviceBuffer<int32_t> phases;
+  DeviceBuffer<int32_t> ring_phases;
+  // host mirrors (used by spined_debug_orbit)
+  std::vector<uint64_t> h_masks;
+  std::vector<int32_t> h_shifts;
+  std::vector<uint32_t> h_offsets, h_flags;
+  std::vector<int32_t> h_phases, h_ring_phases;
+  int ring_has_rev = 0;
+  uint32_t total_terms = 0;
+  bool on_device = false;
+
+  GenericOrbitArgs generic_args(bool device) const;
+  RingOrbitArgs ring_args(bool device) const;
+};
+
+// Host tables at object creation (no GPU needed); uploaded on first use.
+void build_orbit_tables(const Group &g, int n, int spin_inversion, OrbitTables &t);
+void ensure_device(OrbitTables &t);
+
+struct Basis {
+  std::shared_ptr<const Group> group;
+  int n = 0, w = -1, z = 0;
+  int device = 0;
+  OrbitTables tables;
+  std::mutex mu;
+  bool built = false;
+  uint64_t D = 0;
+  DeviceBuffer<uint64_t> reps;
+  DeviceBuffer<uint32_t> stab;
+  DeviceBuffer<uint32_t> buckets;
+  int bucket_bits_request = -1; // -1: automatic
+  int bucket_bits = 0;
+  int shift = 0;
+  uint64_t nbuckets = 0;
+  std::shared_ptr<std::vector<uint64_t>> host_reps; // lazily downloaded
+  double seconds_build = 0.0;
+
+  IndexArgs index_args() const {
+    return IndexArgs{reps.ptr, buckets.ptr, nbuckets, shift};
+  }
+};
+
+double sector_dimension(const Basis &b, double *imag);
+void build_basis(Basis &b);                                  // basis.hip
+void adopt_basis(Basis &b, uint64_t n, const uint64_t *reps); // basis.hip (ls_build_unsafe)
+void build_index(Basis &b);                                   // basis.hip
+
+struct Interaction {
+  int n = 0;
+  std::vector<cplx> matrix; // 2^n x 2^n row-major
+  std::vector<std::array<uint16_t, 4>> sites;
+  bool is_real = true;
+};
+
+// Compiled operator tables.  Off-diagonal templates are ordered by (tuple,
+// local state a, target state b); the row gather visits the active ones in
+// that order, which is also the oracle's summation order.
+struct OperatorTables {
+  int ntuples = 0;
+  int ntemplates = 0;
+  int max_local = 0; // max 2^n over tuples
+  bool is_real = true;
+  // per tuple: packed sites (4 x 8 bits), n, offset into diag values and
+  // into the (a -> template range) table
+  std::vector<uint32_t> h_tuple_sites, h_tuple_n, h_tuple_off;
+  std::vector<double> h_diag_re, h_diag_im;   // per tuple, 2^n entries
+  std::vector<uint32_t> h_range;              // per tuple, 2^n + 1 template offsets
+  std::vector<uint64_t> h_flip;               // per template: XOR mask
+  std::vector<double> h_coef_re, h_coef_im;   // per template: conj(M[b][a])
+  DeviceBuffer<uint32_t> tuple_sites, tuple_n, tuple_off, range;
+  DeviceBuffer<double> diag_re, diag_im, coef_re, coef_im;
+  DeviceBuffer<uint64_t> flip;
+  bool on_device = false;
+};
+
+void ensure_device(OperatorTables &t);
+
+struct Operator {
+  std::shared_ptr<Basis> basis;
+  std::vector<std::shared_ptr<const Interaction>> terms;
+  OperatorTables tables;
+  int orbit_method_override = -1;
+  std::mutex mu;
+  spined_stats last_stats{};
+  int orbit_method() const; // 0 generic, 1 ring
+};
+
+void compile_operator(Operator &op);
+
+// ---- kernel launchers (matvec.hip) ----------------------------------------
+enum class VecType { F32, F64, C64, C128 };
+
+struct MatvecCall {
+  const Operator *op;
+  VecType xt, yt;
+  const void *x;
+  void *y;
+  uint64_t size;
+  hipStream_t stream;
+};
+void launch_matvec(const MatvecCall &c);
+uint64_t count_offdiag_device(const Operator &op);
+
+// ---- eigensolver (solver.hip) ---------------------------------------------
+int eigsh(Operator &op, int dtype, uint64_t k, double tol, const spined_eigsh_options &opts,
+          double *evals, void *evecs, uint64_t ld, double *rnorms, spined_monitor_fn monitor,
+          void *ctx);
+
+// ---- misc kernels (solver.hip) ----------------------------------------------
+void launch_random_fill(double *v, uint64_t n, uint64_t seed, hipStream_t s);
+
+} // namespace spined

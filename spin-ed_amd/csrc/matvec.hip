@@ -1,0 +1,428 @@
+// matvec.hip -- matrix-free H*x row gather (ls_operator_matmat,
+// src/SpinED/Internal.hs:377-378, :411-429).
+//
+// For representative r (row i) with |Stab r| = s_r:
+//   y_i = D(r) x_i + sum_sigma conj(c_sigma) chi(h) sqrt(|Stab r'| / s_r) x_{idx(r')}
+// where H|r> = sum_sigma c_sigma |sigma>, r' = h.sigma = min orbit(sigma),
+// terms whose orbit has inconsistent phases (no symmetric state) or whose
+// representative is not in the basis (another magnetisation sector) are
+// dropped.  One lane owns one row and writes it once: no atomics.
+//
+// Per row, pass A walks the interaction tuples once: it accumulates the
+// diagonal and marks the off-diagonal templates whose local state matches
+// (a 64-bit word per 64 templates).  Pass B pops the marked templates in
+// index order -- (tuple, a, b) order, the oracle's summation order -- and
+// for each does orbit minimum, bucket + binary search and one fma gather.
+// Lanes of a wave run pass B for max(#active) iterations; the orbit method
+// (generic programs or ring run pruning) is a template parameter.
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "engine.hpp"
+
+namespace spined {
+
+namespace {
+
+constexpr int kBlock = 256;
+
+struct OpArgs {
+  const uint32_t *__restrict__ tuple_sites;
+  const uint32_t *__restrict__ tuple_n;
+  const uint32_t *__restrict__ tuple_off;
+  const uint32_t *__restrict__ range;
+  const double *__restrict__ diag_re;
+  const double *__restrict__ diag_im;
+  const uint64_t *__restrict__ flip;
+  const double *__restrict__ coef_re;
+  const double *__restrict__ coef_im;
+  const double *__restrict__ phase_cos; // complex sectors: chi(phase)
+  const double *__restrict__ phase_sin;
+  int32_t ntuples;
+  int32_t ntemplates;
+};
+
+template <class T> struct is_cplx : std::false_type {};
+template <> struct is_cplx<float2> : std::true_type {};
+template <> struct is_cplx<double2> : std::true_type {};
+
+__device__ inline double ld_re(const float *x, uint64_t i) { return static_cast<double>(x[i]); }
+__device__ inline double ld_re(const double *x, uint64_t i) { return x[i]; }
+__device__ inline double2 ld_c(const float2 *x, uint64_t i) {
+  const float2 v = x[i];
+  return make_double2(v.x, v.y);
+}
+__device__ inline double2 ld_c(const double2 *x, uint64_t i) { return x[i]; }
+__device__ inline void st_re(float *y, uint64_t i, double v) { y[i] = static_cast<float>(v); }
+__device__ inline void st_re(double *y, uint64_t i, double v) { y[i] = v; }
+__device__ inline void st_c(float2 *y, uint64_t i, double2 v) {
+  y[i] = make_float2(static_cast<float>(v.x), static_cast<float>(v.y));
+}
+__device__ inline void st_c(double2 *y, uint64_t i, double2 v) { y[i] = v; }
+
+// Bijective block remap: each of the 8 XCDs gets a contiguous range of rows,
+// so neighbouring rows (which share much of their gather targets) meet in
+// the same L2.
+__device__ inline uint64_t xcd_block(uint64_t b, uint64_t nblocks) {
+  const uint64_t xcd = b & 7, idx = b >> 3;
+  const uint64_t q = nblocks >> 3, r = nblocks & 7;
+  return xcd * q + (xcd < r ? xcd : r) + idx;
+}
+
+template <int METHOD>
+__device__ inline OrbitResult do_orbit(const GenericOrbitArgs &ga, const RingOrbitArgs &ra,
+                                       uint64_t s) {
+  if constexpr (METHOD == 1)
+    return orbit_ring(ra, s);
+  else
+    return orbit_generic(ga, s);
+}
+
+__device__ inline int local_state(uint32_t sites, int n, uint64_t r) {
+  int a = 0;
+  for (int k = 0; k < n; ++k) a = (a << 1) | static_cast<int>((r >> ((sites >> (8 * k)) & 0xff)) & 1);
+  return a;
+}
+
+template <int NW>
+__device__ inline void mark_range(uint64_t (&act)[NW], uint32_t b, uint32_t e) {
+#pragma unroll
+  for (int w = 0; w < NW; ++w) {
+    const uint32_t lo = 64u * w, hi = lo + 64u;
+    const uint32_t s = b > lo ? b : lo, t = e < hi ? e : hi;
+    if (s < t) {
+      const uint32_t len = t - s;
+      const uint64_t m = len == 64 ? ~0ull : (((1ull << len) - 1) << (s - lo));
+      act[w] |= m;
+    }
+  }
+}
+
+template <int NW> __device__ inline int pop_first(uint64_t (&act)[NW]) {
+  int j = -1;
+#pragma unroll
+  for (int w = 0; w < NW; ++w) {
+    if (j < 0 && act[w]) {
+      j = 64 * w + __builtin_ctzll(act[w]);
+      act[w] &= act[w] - 1;
+    }
+  }
+  return j;
+}
+
+template <int NW> __device__ inline bool any_set(const uint64_t (&act)[NW]) {
+  uint64_t o = 0;
+#pragma unroll
+  for (int w = 0; w < NW; ++w) o |= act[w];
+  return o != 0;
+}
+
+// ---- real rows -------------------------------------------------------------
+template <int METHOD, int NW, class XT, class YT>
+__global__ void __launch_bounds__(kBlock)
+    matvec_real_kernel(OpArgs o, GenericOrbitArgs ga, RingOrbitArgs ra, IndexArgs ix,
+                       const uint32_t *__restrict__ stab, uint64_t D,
+                       const XT *__restrict__ x, YT *__restrict__ y) {
+  const uint64_t lb = xcd_block(blockIdx.x, gridDim.x);
+  const uint64_t i = lb * kBlock + threadIdx.x;
+  if (i >= D) return;
+  const uint64_t r = ix.reps[i];
+  uint64_t act[NW];
+#pragma unroll
+  for (int w = 0; w < NW; ++w) act[w] = 0;
+  double diag = 0.0;
+  for (int t = 0; t < o.ntuples; ++t) {
+    const int n = static_cast<int>(o.tuple_n[t]);
+    const uint32_t off = o.tuple_off[t];
+    const int a = local_state(o.tuple_sites[t], n, r);
+    diag += o.diag_re[off + a];
+    const uint32_t* rg = o.range + off + t + a;
+    mark_range<NW>(act, rg[0], rg[1]);
+  }
+  double acc = diag * ld_re(x, i);
+  const double sr = static_cast<double>(stab[i]);
+  while (any_set<NW>(act)) {
+    const int j = pop_first<NW>(act);
+    const uint64_t sigma = r ^ o.flip[j];
+    const OrbitResult orb = do_orbit<METHOD>(ga, ra, sigma);
+    if (!orb.valid) continue;
+    const int64_t k = find_index(ix, orb.rep);
+    if (k < 0) continue;
+    double coef = o.coef_re[j] * sqrt(static_cast<double>(orb.count) / sr);
+    if (orb.phase != 0) coef = -coef;
+    acc = fma(coef, ld_re(x, static_cast<uint64_t>(k)), acc);
+  }
+  st_re(y, i, acc);
+}
+
+// Rows with more than 256 off-diagonal templates: per-tuple loop.
+template <int METHOD, class XT, class YT>
+__global__ void __launch_bounds__(kBlock)
+    matvec_real_direct_kernel(OpArgs o, GenericOrbitArgs ga, RingOrbitArgs ra, IndexArgs ix,
+                              const uint32_t *__restrict__ stab, uint64_t D,
+                              const XT *__restrict__ x, YT *__restrict__ y) {
+  const uint64_t lb = xcd_block(blockIdx.x, gridDim.x);
+  const uint64_t i = lb * kBlock + threadIdx.x;
+  if (i >= D) return;
+  const uint64_t r = ix.reps[i];
+  double diag = 0.0;
+  for (int t = 0; t < o.ntuples; ++t) {
+    const int a = local_state(o.tuple_sites[t], static_cast<int>(o.tuple_n[t]), r);
+    diag += o.diag_re[o.tuple_off[t] + a];
+  }
+  double acc = diag * ld_re(x, i);
+  const double sr = static_cast<double>(stab[i]);
+  for (int t = 0; t < o.ntuples; ++t) {
+    const uint32_t off = o.tuple_off[t];
+    const int a = local_state(o.tuple_sites[t], static_cast<int>(o.tuple_n[t]), r);
+    const uint32_t b = o.range[off + t + a], e = o.range[off + t + a + 1];
+    for (uint32_t j = b; j < e; ++j) {
+      const OrbitResult orb = do_orbit<METHOD>(ga, ra, r ^ o.flip[j]);
+      if (!orb.valid) continue;
+      const int64_t k = find_index(ix, orb.rep);
+      if (k < 0) continue;
+      double coef = o.coef_re[j] * sqrt(static_cast<double>(orb.count) / sr);
+      if (orb.phase != 0) coef = -coef;
+      acc = fma(coef, ld_re(x, static_cast<uint64_t>(k)), acc);
+    }
+  }
+  st_re(y, i, acc);
+}
+
+// ---- complex rows ------------------------------------------------------------
+template <int METHOD, class XT, class YT>
+__global__ void __launch_bounds__(kBlock)
+    matvec_complex_kernel(OpArgs o, GenericOrbitArgs ga, RingOrbitArgs ra, IndexArgs ix,
+                          const uint32_t *__restrict__ stab, uint64_t D,
+                          const XT *__restrict__ x, YT *__restrict__ y) {
+  const uint64_t lb = xcd_block(blockIdx.x, gridDim.x);
+  const uint64_t i = lb * kBlock + threadIdx.x;
+  if (i >= D) return;
+  const uint64_t r = ix.reps[i];
+  double dre = 0.0, dim = 0.0;
+  for (int t = 0; t < o.ntuples; ++t) {
+    const int a = local_state(o.tuple_sites[t], static_cast<int>(o.tuple_n[t]), r);
+    dre += o.diag_re[o.tuple_off[t] + a];
+    dim += o.diag_im[o.tuple_off[t] + a];
+  }
+  const double2 xi = ld_c(x, i);
+  double are = fma(dre, xi.x, -(dim * xi.y));
+  double aim = fma(dre, xi.y, dim * xi.x);
+  const double sr = static_cast<double>(stab[i]);
+  for (int t = 0; t < o.ntuples; ++t) {
+    const uint32_t off = o.tuple_off[t];
+    const int a = local_state(o.tuple_sites[t], static_cast<int>(o.tuple_n[t]), r);
+    const uint32_t b = o.range[off + t + a], e = o.range[off + t + a + 1];
+    for (uint32_t j = b; j < e; ++j) {
+      const OrbitResult orb = do_orbit<METHOD>(ga, ra, r ^ o.flip[j]);
+      if (!orb.valid) continue;
+      const int64_t k = find_index(ix, orb.rep);
+      if (k < 0) continue;
+      const double f = sqrt(static_cast<double>(orb.count) / sr);
+      const double cre = o.phase_cos[orb.phase], cim = o.phase_sin[orb.phase];
+      const double mre = o.coef_re[j], mim = o.coef_im[j];
+      const double kre = fma(mre, cre, -(mim * cim)) * f;
+      const double kim = fma(mre, cim, mim * cre) * f;
+      const double2 xk = ld_c(x, static_cast<uint64_t>(k));
+      are = fma(kre, xk.x, are);
+      are = fma(-kim, xk.y, are);
+      aim = fma(kre, xk.y, aim);
+      aim = fma(kim, xk.x, aim);
+    }
+  }
+  st_c(y, i, make_double2(are, aim));
+}
+
+// ---- counting (measurement) --------------------------------------------------
+template <int METHOD>
+__global__ void __launch_bounds__(kBlock)
+    count_offdiag_kernel(OpArgs o, GenericOrbitArgs ga, RingOrbitArgs ra, IndexArgs ix,
+                         uint64_t D, unsigned long long *__restrict__ partial) {
+  __shared__ unsigned long long lds[kBlock / 64];
+  const uint64_t i = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
+  unsigned long long c = 0;
+  if (i < D) {
+    const uint64_t r = ix.reps[i];
+    for (int t = 0; t < o.ntuples; ++t) {
+      const uint32_t off = o.tuple_off[t];
+      const int a = local_state(o.tuple_sites[t], static_cast<int>(o.tuple_n[t]), r);
+      const uint32_t b = o.range[off + t + a], e = o.range[off + t + a + 1];
+      for (uint32_t j = b; j < e; ++j) {
+        const OrbitResult orb = do_orbit<METHOD>(ga, ra, r ^ o.flip[j]);
+        if (orb.valid && find_index(ix, orb.rep) >= 0) ++c;
+      }
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) c += __shfl_down(c, off, 64);
+  if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long s = 0;
+    for (int w = 0; w < kBlock / 64; ++w) s += lds[w];
+    partial[blockIdx.x] = s;
+  }
+}
+
+struct PhaseTable {
+  DeviceBuffer<double> cos_t, sin_t;
+  int L = 0;
+};
+
+// chi(phase) = exp(2 pi i phase / L), exact at quarter turns (same values
+// as the oracle's phase_to_complex).
+void phase_values(int64_t p, int64_t L, double &c, double &s) {
+  p = ((p % L) + L) % L;
+  if (p == 0) { c = 1.0; s = 0.0; }
+  else if (2 * p == L) { c = -1.0; s = 0.0; }
+  else if (4 * p == L) { c = 0.0; s = 1.0; }
+  else if (4 * p == 3 * L) { c = 0.0; s = -1.0; }
+  else {
+    const double ang = 2.0 * M_PI * static_cast<double>(p) / static_cast<double>(L);
+    c = std::cos(ang);
+    s = std::sin(ang);
+  }
+}
+
+OpArgs op_args(const Operator &op, const double *pc, const double *ps) {
+  const auto &t = op.tables;
+  OpArgs a{};
+  a.tuple_sites = t.tuple_sites.ptr;
+  a.tuple_n = t.tuple_n.ptr;
+  a.tuple_off = t.tuple_off.ptr;
+  a.range = t.range.ptr;
+  a.diag_re = t.diag_re.ptr;
+  a.diag_im = t.diag_im.ptr;
+  a.flip = t.flip.ptr;
+  a.coef_re = t.coef_re.ptr;
+  a.coef_im = t.coef_im.ptr;
+  a.phase_cos = pc;
+  a.phase_sin = ps;
+  a.ntuples = t.ntuples;
+  a.ntemplates = t.ntemplates;
+  return a;
+}
+
+unsigned grid_for(uint64_t n) { return static_cast<unsigned>((n + kBlock - 1) / kBlock); }
+
+template <int METHOD, class XT, class YT>
+void launch_real(const OpArgs &o, const GenericOrbitArgs &ga, const RingOrbitArgs &ra,
+                 const IndexArgs &ix, const uint32_t *stab, uint64_t D, const XT *x, YT *y,
+                 hipStream_t s) {
+  const unsigned g = grid_for(D);
+  if (o.ntemplates <= 64)
+    matvec_real_kernel<METHOD, 1, XT, YT><<<g, kBlock, 0, s>>>(o, ga, ra, ix, stab, D, x, y);
+  else if (o.ntemplates <= 128)
+    matvec_real_kernel<METHOD, 2, XT, YT><<<g, kBlock, 0, s>>>(o, ga, ra, ix, stab, D, x, y);
+  else if (o.ntemplates <= 256)
+    matvec_real_kernel<METHOD, 4, XT, YT><<<g, kBlock, 0, s>>>(o, ga, ra, ix, stab, D, x, y);
+  else
+    matvec_real_direct_kernel<METHOD, XT, YT><<<g, kBlock, 0, s>>>(o, ga, ra, ix, stab, D, x, y);
+  SPINED_CHECK_LAUNCH();
+}
+
+template <int METHOD, class XT, class YT>
+void launch_complex(const OpArgs &o, const GenericOrbitArgs &ga, const RingOrbitArgs &ra,
+                    const IndexArgs &ix, const uint32_t *stab, uint64_t D, const XT *x, YT *y,
+                    hipStream_t s) {
+  matvec_complex_kernel<METHOD, XT, YT><<<grid_for(D), kBlock, 0, s>>>(o, ga, ra, ix, stab, D, x, y);
+  SPINED_CHECK_LAUNCH();
+}
+
+template <int METHOD>
+void dispatch(const MatvecCall &c, const OpArgs &o, const GenericOrbitArgs &ga,
+              const RingOrbitArgs &ra, const IndexArgs &ix, const uint32_t *stab, uint64_t D) {
+  hipStream_t s = c.stream;
+  using VT = VecType;
+  if (c.xt == VT::F64 && c.yt == VT::F64)
+    launch_real<METHOD>(o, ga, ra, ix, stab, D, static_cast<const double *>(c.x),
+                        static_cast<double *>(c.y), s);
+  else if (c.xt == VT::F32 && c.yt == VT::F32)
+    launch_real<METHOD>(o, ga, ra, ix, stab, D, static_cast<const float *>(c.x),
+                        static_cast<float *>(c.y), s);
+  else if (c.xt == VT::F32 && c.yt == VT::F64)
+    launch_real<METHOD>(o, ga, ra, ix, stab, D, static_cast<const float *>(c.x),
+                        static_cast<double *>(c.y), s);
+  else if (c.xt == VT::C128 && c.yt == VT::C128)
+    launch_complex<METHOD>(o, ga, ra, ix, stab, D, static_cast<const double2 *>(c.x),
+                           static_cast<double2 *>(c.y), s);
+  else if (c.xt == VT::C64 && c.yt == VT::C64)
+    launch_complex<METHOD>(o, ga, ra, ix, stab, D, static_cast<const float2 *>(c.x),
+                           static_cast<float2 *>(c.y), s);
+  else
+    throw Error(LS_INVALID_DATATYPE, "unsupported matvec type combination");
+}
+
+std::mutex g_phase_mu;
+
+const PhaseTable &phase_table(const Basis &b) {
+  // one table per (device, L); small and kept for the process lifetime
+  static std::vector<std::pair<std::pair<int, int>, std::unique_ptr<PhaseTable>>> cache;
+  std::lock_guard<std::mutex> lk(g_phase_mu);
+  const int dev = current_device();
+  for (auto &kv : cache)
+    if (kv.first.first == dev && kv.first.second == b.tables.L) return *kv.second;
+  auto t = std::make_unique<PhaseTable>();
+  const int L = b.tables.L;
+  if (L > (1 << 20)) throw Error(LS_NOT_IMPLEMENTED, "complex sectors with huge periodicity");
+  std::vector<double> c(L), s(L);
+  for (int p = 0; p < L; ++p) phase_values(p, L, c[p], s[p]);
+  t->cos_t.allocate(L);
+  t->sin_t.allocate(L);
+  SPINED_HIP(hipMemcpy(t->cos_t.ptr, c.data(), 8 * L, hipMemcpyHostToDevice));
+  SPINED_HIP(hipMemcpy(t->sin_t.ptr, s.data(), 8 * L, hipMemcpyHostToDevice));
+  t->L = L;
+  cache.emplace_back(std::make_pair(dev, L), std::move(t));
+  return *cache.back().second;
+}
+
+} // namespace
+
+void launch_matvec(const MatvecCall &c) {
+  const Operator &op = *c.op;
+  const Basis &b = *op.basis;
+  if (c.size != b.D) throw Error(LS_DIMENSION_MISMATCH, "vector length differs from the basis dimension");
+  if (b.D == 0) return;
+  const bool cplx = c.xt == VecType::C64 || c.xt == VecType::C128;
+  if (!cplx && !op.tables.is_real)
+    throw Error(LS_INVALID_DATATYPE, "operator is complex; use a complex datatype");
+  const double *pc = nullptr, *ps = nullptr;
+  if (cplx) {
+    const auto &pt = phase_table(b);
+    pc = pt.cos_t.ptr;
+    ps = pt.sin_t.ptr;
+  }
+  const OpArgs o = op_args(op, pc, ps);
+  const auto ga = b.tables.generic_args(true);
+  const auto ra = b.tables.ring_args(true);
+  const IndexArgs ix = b.index_args();
+  if (op.orbit_method() == 1)
+    dispatch<1>(c, o, ga, ra, ix, b.stab.ptr, b.D);
+  else
+    dispatch<0>(c, o, ga, ra, ix, b.stab.ptr, b.D);
+}
+
+uint64_t count_offdiag_device(const Operator &op) {
+  const Basis &b = *op.basis;
+  if (b.D == 0) return 0;
+  hipStream_t s = stream();
+  const OpArgs o = op_args(op, nullptr, nullptr);
+  const auto ga = b.tables.generic_args(true);
+  const auto ra = b.tables.ring_args(true);
+  const IndexArgs ix = b.index_args();
+  const unsigned g = grid_for(b.D);
+  DeviceBuffer<unsigned long long> part(g);
+  if (op.orbit_method() == 1)
+    count_offdiag_kernel<1><<<g, kBlock, 0, s>>>(o, ga, ra, ix, b.D, part.ptr);
+  else
+    count_offdiag_kernel<0><<<g, kBlock, 0, s>>>(o, ga, ra, ix, b.D, part.ptr);
+  SPINED_CHECK_LAUNCH();
+  std::vector<unsigned long long> h(g);
+  SPINED_HIP(hipMemcpyAsync(h.data(), part.ptr, 8ull * g, hipMemcpyDeviceToHost, s));
+  SPINED_HIP(hipStreamSynchronize(s));
+  uint64_t tot = 0;
+  for (auto v : h) tot += v;
+  return tot;
+}
+
+} // namespace spined

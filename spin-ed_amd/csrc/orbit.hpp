@@ -1,0 +1,289 @@
+// orbit.hpp -- orbit-minimum and index-search primitives shared by every
+// kernel.  All functions are __host__ __device__ so the same integer logic
+// can be exercised on the CPU (spined_debug_orbit) and compared with the
+// oracle without a GPU.
+//
+// State convention: bit i of a uint64 is site i; a permutation p acts as
+// (g.x)_i = x_{p[i]}; the representative of x is min_g g.x.  For each x the
+// scan returns
+//   rep   = min_g g.x,
+//   phase = phase (integer modulo L) of the first element reaching rep,
+//   count = #{g : g.x = rep}  (= |Stab(rep)|),
+//   valid = all elements reaching rep carry the same phase (the character is
+//           trivial on the stabiliser, i.e. rep spans a symmetric state).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace spined {
+
+struct OrbitResult {
+  uint64_t rep;
+  int32_t phase;
+  int32_t count;
+  bool valid;
+};
+
+__host__ __device__ inline uint64_t site_mask_of(int n) {
+  return n >= 64 ? ~0ull : ((1ull << n) - 1);
+}
+
+// Reverse the lowest n bits: site i -> site n-1-i.
+__host__ __device__ inline uint64_t reverse_n(uint64_t x, int n) {
+  const uint64_t r = __builtin_bitreverse64(x);
+  return n >= 64 ? r : (r >> (64 - n));
+}
+
+__host__ __device__ inline int ctz64(uint64_t v) { return __builtin_ctzll(v); }
+
+__host__ __device__ inline void orbit_update(uint64_t &best, int32_t &ph, int32_t &cnt, bool &ok,
+                                             uint64_t y, int32_t p) {
+  if (y < best) {
+    best = y;
+    ph = p;
+    cnt = 1;
+    ok = true;
+  } else if (y == best) {
+    cnt += 1;
+    ok = ok && (p == ph);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Generic groups: every permutation is a short program of (mask, shift)
+// terms, image = OR_t shift(src & mask_t, s_t), where src is x or its
+// n-bit reversal (whichever needs fewer distinct shifts; the reversal is
+// computed once per scan).  Spin inversion doubles every image: y and
+// y ^ site_mask.
+// ---------------------------------------------------------------------------
+struct GenericOrbitArgs {
+  const uint64_t *__restrict__ masks;   // all terms, element after element
+  const int32_t *__restrict__ shifts;   // > 0: left shift, < 0: right shift
+  const uint32_t *__restrict__ offsets; // n_perm + 1 term offsets
+  const uint32_t *__restrict__ flags;   // bit 0: program reads reverse_n(x)
+  const int32_t *__restrict__ phases;   // phase of each permutation (mod L)
+  int32_t n_perm;
+  int32_t n_sites;
+  uint64_t site_mask;
+  int32_t has_flip;
+  int32_t flip_phase;
+  int32_t L;
+};
+
+__host__ __device__ inline uint64_t apply_program(const GenericOrbitArgs &a, int g, uint64_t x,
+                                                  uint64_t rx) {
+  const uint64_t src = (a.flags[g] & 1u) ? rx : x;
+  uint64_t y = 0;
+  const uint32_t e = a.offsets[g + 1];
+  for (uint32_t t = a.offsets[g]; t < e; ++t) {
+    const uint64_t m = src & a.masks[t];
+    const int32_t s = a.shifts[t];
+    y |= s >= 0 ? (m << s) : (m >> (-s));
+  }
+  return y;
+}
+
+__host__ __device__ inline OrbitResult orbit_generic(const GenericOrbitArgs &a, uint64_t x) {
+  const uint64_t rx = reverse_n(x, a.n_sites);
+  uint64_t best = ~0ull;
+  int32_t ph = 0, cnt = 0;
+  bool ok = true;
+  for (int g = 0; g < a.n_perm; ++g) {
+    const uint64_t y = apply_program(a, g, x, rx);
+    const int32_t p = a.phases[g];
+    orbit_update(best, ph, cnt, ok, y, p);
+    if (a.has_flip) {
+      int32_t pf = p + a.flip_phase;
+      if (pf >= a.L) pf -= a.L;
+      orbit_update(best, ph, cnt, ok, y ^ a.site_mask, pf);
+    }
+  }
+  return OrbitResult{best, ph, cnt, ok};
+}
+
+// Early-exit representative test used by the basis builder: returns the
+// stabiliser size if x is a representative whose symmetric state exists,
+// else 0.
+__host__ __device__ inline uint32_t rep_test_generic(const GenericOrbitArgs &a, uint64_t x) {
+  const uint64_t rx = reverse_n(x, a.n_sites);
+  uint32_t cnt = 0;
+  for (int g = 0; g < a.n_perm; ++g) {
+    const uint64_t y = apply_program(a, g, x, rx);
+    const int32_t p = a.phases[g];
+    if (y < x) return 0;
+    if (y == x) {
+      if (p != 0) return 0;
+      ++cnt;
+    }
+    if (a.has_flip) {
+      const uint64_t yf = y ^ a.site_mask;
+      if (yf < x) return 0;
+      if (yf == x) {
+        int32_t pf = p + a.flip_phase;
+        if (pf >= a.L) pf -= a.L;
+        if (pf != 0) return 0;
+        ++cnt;
+      }
+    }
+  }
+  return cnt;
+}
+
+// ---------------------------------------------------------------------------
+// Rings: the group is the rotations of n sites in natural order, optionally
+// with the reflections and spin inversion.  Rotation by k maps u to
+// rotr(u, k) ((R_k u)_i = u_{(i+k) mod n}); reflection elements are
+// rotations of reverse_n(u).  The minimum image must start (at bit n-1) with
+// a longest cyclic run of zeros, so only rotations that align the top of a
+// longest zero run (longest one run for the flipped images) with bit n-1
+// are candidates: usually 1-4 instead of up to 4n.
+// ---------------------------------------------------------------------------
+struct RingOrbitArgs {
+  const int32_t *__restrict__ phase_tab; // [2][n]: phase of (reflected?, k)
+  int32_t n;
+  uint64_t mask;
+  int32_t has_rev;
+  int32_t has_flip;
+  int32_t flip_phase;
+  int32_t L;
+};
+
+__host__ __device__ inline uint64_t rotl_n(uint64_t z, int j, int n, uint64_t mask) {
+  return j == 0 ? z : (((z << j) | (z >> (n - j))) & mask);
+}
+
+__host__ __device__ inline uint64_t rotr_n(uint64_t u, int k, int n, uint64_t mask) {
+  return k == 0 ? u : (((u >> k) | (u << (n - k))) & mask);
+}
+
+// Positions q that are the top (highest cyclic position) of a longest
+// cyclic run of ones in z; len receives that length.  z == 0 yields every
+// position with len 0 (all rotations of an all-ones / all-zeros word tie).
+__host__ __device__ inline uint64_t longest_run_tops(uint64_t z, int n, uint64_t mask, int &len) {
+  if (z == 0) {
+    len = 0;
+    return mask;
+  }
+  if (z == mask) {
+    len = n;
+    return mask;
+  }
+  // a_L = tops of windows of L ones; a_{2L} = a_L & rotl(a_L, L).
+  uint64_t bs[6];
+  bs[0] = z;
+  uint64_t cur = z;
+  int l = 1;
+  bool grow = true;
+#pragma unroll
+  for (int q = 1; q < 6; ++q) {
+    bs[q] = 0;
+    if (grow && 2 * l <= n - 1) {
+      const uint64_t nb = bs[q - 1] & rotl_n(bs[q - 1], l, n, mask);
+      if (nb) {
+        bs[q] = nb;
+        cur = nb;
+        l *= 2;
+      } else {
+        grow = false;
+      }
+    } else {
+      grow = false;
+    }
+  }
+  // binary refinement: a_{l+s} = a_l & rotl(a_s, l)
+#pragma unroll
+  for (int q = 4; q >= 0; --q) {
+    const int s = 1 << q;
+    if (s < l && l + s <= n - 1) {
+      const uint64_t t = cur & rotl_n(bs[q], l, n, mask);
+      if (t) {
+        cur = t;
+        l += s;
+      }
+    }
+  }
+  len = l;
+  return cur;
+}
+
+__host__ __device__ inline void ring_candidates(const RingOrbitArgs &a, uint64_t u, uint64_t tops,
+                                                int len, int32_t fph, uint64_t &best, int32_t &ph,
+                                                int32_t &cnt, bool &ok) {
+  const int n = a.n;
+  const uint64_t ru = a.has_rev ? reverse_n(u, n) : 0;
+  while (tops) {
+    const int q = ctz64(tops);
+    tops &= tops - 1;
+    int k = q + 1;
+    if (k >= n) k -= n;
+    int32_t p = a.phase_tab[k] + fph;
+    if (p >= a.L) p -= a.L;
+    orbit_update(best, ph, cnt, ok, rotr_n(u, k, n, a.mask), p);
+    if (a.has_rev) {
+      int k2 = (n - q + len - 1) % n;
+      int32_t p2 = a.phase_tab[n + k2] + fph;
+      if (p2 >= a.L) p2 -= a.L;
+      orbit_update(best, ph, cnt, ok, rotr_n(ru, k2, n, a.mask), p2);
+    }
+  }
+}
+
+__host__ __device__ inline OrbitResult orbit_ring(const RingOrbitArgs &a, uint64_t x) {
+  const int n = a.n;
+  const uint64_t xn = ~x & a.mask;
+  int l0 = 0, l1 = -1;
+  const uint64_t t0 = longest_run_tops(xn, n, a.mask, l0); // zero runs of x
+  uint64_t t1 = 0;
+  if (a.has_flip) t1 = longest_run_tops(x, n, a.mask, l1); // one runs of x
+  const int lm = l0 > l1 ? l0 : l1;
+  uint64_t best = ~0ull;
+  int32_t ph = 0, cnt = 0;
+  bool ok = true;
+  if (l0 == lm) ring_candidates(a, x, t0, l0, 0, best, ph, cnt, ok);
+  if (a.has_flip && l1 == lm) ring_candidates(a, xn, t1, l1, a.flip_phase, best, ph, cnt, ok);
+  return OrbitResult{best, ph, cnt, ok};
+}
+
+// ---------------------------------------------------------------------------
+// Index of a representative: bucket table on the top bits plus a branchless
+// binary search inside the bucket.  buckets[b] = first i with
+// reps[i] >> shift >= b, for b in [0, nbuckets]; buckets[nbuckets] = D.
+// ---------------------------------------------------------------------------
+struct IndexArgs {
+  const uint64_t *__restrict__ reps;
+  const uint32_t *__restrict__ buckets;
+  uint64_t nbuckets;
+  int32_t shift;
+};
+
+__host__ __device__ inline int64_t find_index(const IndexArgs &t, uint64_t v) {
+  const uint64_t b = v >> t.shift;
+  if (b >= t.nbuckets) return -1;
+  uint32_t lo = t.buckets[b];
+  uint32_t n = t.buckets[b + 1] - lo;
+  if (n == 0) return -1;
+  while (n > 1) {
+    const uint32_t half = n >> 1;
+    lo = (t.reps[lo + half] <= v) ? lo + half : lo;
+    n -= half;
+  }
+  return t.reps[lo] == v ? static_cast<int64_t>(lo) : -1;
+}
+
+// Counter-hash start vector (SplitMix64 finaliser, Steele, Lea & Flood
+// 2014): v[r] = U(-1, 1) from (seed, r); identical on host and device.
+__host__ __device__ inline uint64_t mix64(uint64_t z) {
+  z += 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+__host__ __device__ inline double hash_uniform(uint64_t seed, uint64_t r) {
+  const uint64_t h = mix64(seed ^ mix64(r));
+  return 2.0 * (static_cast<double>(h >> 11) * 0x1.0p-53) - 1.0;
+}
+
+} // namespace spined

@@ -1,0 +1,696 @@
+// solver.hip -- device-resident eigensolver (replaces PRIMME's eigh at
+// src/SpinED.hs:383-404, which calls back into ls_operator_matmat through
+// inplaceApply, src/SpinED/Internal.hs:411-429, on every iteration).
+//
+// Thick-restart Lanczos (Wu & Simon, SIAM J. Matrix Anal. Appl. 22 (2000))
+// with full (twice classical Gram-Schmidt) reorthogonalisation:
+//   step j:  w = H v_j;  h = V_j^T w;  w -= V_j h;  (twice, FP64 dots)
+//            T[:j, j] = h;  beta_j = |w|;  v_{j+1} = w / beta_j
+//   cycle:   T = S diag(theta) S^T (one-workgroup Jacobi, FP64, on device);
+//            residual estimates |beta_{m-1} S[m-1, i]|;
+//            restart keeps the lowest Ritz vectors V S[:, :keep] (row-local,
+//            in place) and v_m.
+// Everything stays on the GPU; the host reads m + k scalars per cycle.
+//
+// Precision modes: FP64 stores V in double.  MIXED stores V in float while
+// the matvec accumulates rows in FP64 into a double w, and the dots, the
+// recurrence, the projected solve and the Ritz assembly are FP64; once the
+// FP32-storage phase stalls at its floor (~1e-6 |H|) the Ritz vector is
+// assembled in FP64 and an FP64 phase restarts from it to reach tol.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "engine.hpp"
+
+namespace spined {
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kMaxBasis = 64;
+constexpr int kDotChunk = 8;
+
+unsigned grid_for(uint64_t n, uint64_t cap = 8192) {
+  uint64_t g = (n + kBlock - 1) / kBlock;
+  return static_cast<unsigned>(std::max<uint64_t>(1, std::min(g, cap)));
+}
+
+__device__ inline double to_d(float v) { return static_cast<double>(v); }
+__device__ inline double to_d(double v) { return v; }
+
+__global__ void random_fill_kernel(double *v, uint64_t n, uint64_t seed) {
+  for (uint64_t r = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; r < n;
+       r += static_cast<uint64_t>(gridDim.x) * blockDim.x)
+    v[r] = hash_uniform(seed, r);
+}
+
+// partial[c*G + b] for vectors i in [8c, 8c+8): sum over this block's rows of
+// V_i[r] * w[r] (FP64).  V vectors are given by a device array of pointers.
+template <class VT>
+__global__ void __launch_bounds__(kBlock)
+    multidot_kernel(const VT *const *__restrict__ V, int nvec, const double *__restrict__ w,
+                    uint64_t n, double *__restrict__ partial) {
+  __shared__ double lds[kDotChunk][kBlock / 64];
+  const int c = blockIdx.y;
+  const int i0 = c * kDotChunk;
+  double acc[kDotChunk];
+#pragma unroll
+  for (int q = 0; q < kDotChunk; ++q) acc[q] = 0.0;
+  const VT *vp[kDotChunk];
+#pragma unroll
+  for (int q = 0; q < kDotChunk; ++q) vp[q] = (i0 + q < nvec) ? V[i0 + q] : nullptr;
+  for (uint64_t r = blockIdx.x * static_cast<uint64_t>(kBlock) + threadIdx.x; r < n;
+       r += static_cast<uint64_t>(gridDim.x) * kBlock) {
+    const double wr = w[r];
+#pragma unroll
+    for (int q = 0; q < kDotChunk; ++q)
+      if (i0 + q < nvec) acc[q] = fma(to_d(vp[q][r]), wr, acc[q]);
+  }
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int q = 0; q < kDotChunk; ++q) {
+    double v = acc[q];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    if (lane == 0) lds[q][wid] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < kDotChunk && i0 + static_cast<int>(threadIdx.x) < nvec) {
+    double s = 0.0;
+    for (int k = 0; k < kBlock / 64; ++k) s += lds[threadIdx.x][k];
+    partial[static_cast<uint64_t>(i0 + threadIdx.x) * gridDim.x + blockIdx.x] = s;
+  }
+}
+
+// h[i] (+)= sum_b partial[i*G + b], fixed order (deterministic).
+__global__ void reduce_partials_kernel(const double *__restrict__ partial, int nvec, int G,
+                                       double *__restrict__ h, int accumulate) {
+  const int i = blockIdx.x;
+  __shared__ double lds[kBlock];
+  double s = 0.0;
+  for (int b = threadIdx.x; b < G; b += kBlock) s += partial[static_cast<uint64_t>(i) * G + b];
+  lds[threadIdx.x] = s;
+  __syncthreads();
+  for (int off = kBlock / 2; off > 0; off >>= 1) {
+    if (threadIdx.x < static_cast<unsigned>(off)) lds[threadIdx.x] += lds[threadIdx.x + off];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0 && i < nvec) h[i] = accumulate ? h[i] + lds[0] : lds[0];
+}
+
+// w[r] -= sum_{i < nvec} h[i] V_i[r]
+template <class VT>
+__global__ void __launch_bounds__(kBlock)
+    gemv_sub_kernel(const VT *const *__restrict__ V, int nvec, const double *__restrict__ h,
+                    double *__restrict__ w, uint64_t n) {
+  for (uint64_t r = blockIdx.x * static_cast<uint64_t>(kBlock) + threadIdx.x; r < n;
+       r += static_cast<uint64_t>(gridDim.x) * kBlock) {
+    double acc = 0.0;
+    for (int i = 0; i < nvec; ++i) acc = fma(h[i], to_d(V[i][r]), acc);
+    w[r] -= acc;
+  }
+}
+
+// v[r] = w[r] / (*beta) (beta == 0 -> zero vector)
+template <class VT>
+__global__ void __launch_bounds__(kBlock)
+    scale_store_kernel(const double *__restrict__ w, const double *__restrict__ beta,
+                       VT *__restrict__ v, uint64_t n) {
+  const double b = *beta;
+  const double inv = b > 0.0 ? 1.0 / b : 0.0;
+  for (uint64_t r = blockIdx.x * static_cast<uint64_t>(kBlock) + threadIdx.x; r < n;
+       r += static_cast<uint64_t>(gridDim.x) * kBlock)
+    v[r] = static_cast<VT>(w[r] * inv);
+}
+
+__global__ void sqrt_inplace_kernel(double *v) { v[0] = sqrt(v[0]); }
+
+// T[i*ldt + j] = T[j*ldt + i] = h[i], i <= j; beta store
+__global__ void store_column_kernel(const double *__restrict__ h, int j, double *__restrict__ T,
+                                    int ldt) {
+  const int i = threadIdx.x;
+  if (i <= j) {
+    T[i * ldt + j] = h[i];
+    T[j * ldt + i] = h[i];
+  }
+}
+
+// Parallel cyclic Jacobi (round-robin pairing) for the symmetric m x m
+// projected matrix, m <= 64, in LDS.  evals ascending; S column-major.
+__global__ void __launch_bounds__(256)
+    jacobi_eigh_kernel(const double *__restrict__ T, int ldt, int m, double *__restrict__ evals,
+                       double *__restrict__ S, int max_sweeps) {
+  __shared__ double A[kMaxBasis][kMaxBasis + 1];
+  __shared__ double V[kMaxBasis][kMaxBasis + 1];
+  __shared__ double cs[kMaxBasis / 2], sn[kMaxBasis / 2];
+  __shared__ int pp[kMaxBasis / 2], qq[kMaxBasis / 2];
+  __shared__ double red[256];
+  __shared__ int done;
+  const int tid = threadIdx.x;
+  const int mp = m + (m & 1);
+  for (int e = tid; e < mp * mp; e += blockDim.x) {
+    const int i = e / mp, j = e % mp;
+    double a = 0.0;
+    if (i < m && j < m) a = 0.5 * (T[i * ldt + j] + T[j * ldt + i]);
+    A[i][j] = a;
+    V[i][j] = i == j ? 1.0 : 0.0;
+  }
+  __syncthreads();
+  for (int sweep = 0; sweep < max_sweeps; ++sweep) {
+    // convergence: off-diagonal mass vs diagonal mass
+    double off = 0.0, dia = 0.0;
+    for (int e = tid; e < mp * mp; e += blockDim.x) {
+      const int i = e / mp, j = e % mp;
+      const double a = A[i][j];
+      if (i == j)
+        dia += a * a;
+      else
+        off += a * a;
+    }
+    red[tid] = off;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+      if (tid < s) red[tid] += red[tid + s];
+      __syncthreads();
+    }
+    const double off_tot = red[0];
+    __syncthreads();
+    red[tid] = dia;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+      if (tid < s) red[tid] += red[tid + s];
+      __syncthreads();
+    }
+    const double dia_tot = red[0];
+    if (tid == 0) done = off_tot <= 1e-30 * (dia_tot + off_tot) || off_tot == 0.0;
+    __syncthreads();
+    if (done) break;
+    for (int step = 0; step < mp - 1; ++step) {
+      const int half = mp / 2;
+      if (tid < half) {
+        // round robin: position 0 holds player 0, others rotate
+        auto player = [&](int pos) { return pos == 0 ? 0 : 1 + ((pos - 1 + step) % (mp - 1)); };
+        int p = player(tid), q = player(mp - 1 - tid);
+        if (p > q) {
+          const int t = p;
+          p = q;
+          q = t;
+        }
+        pp[tid] = p;
+        qq[tid] = q;
+        const double apq = A[p][q];
+        double c = 1.0, s = 0.0;
+        if (apq != 0.0) {
+          const double app = A[p][p], aqq = A[q][q];
+          const double tau = (aqq - app) / (2.0 * apq);
+          const double t = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
+          c = 1.0 / sqrt(1.0 + t * t);
+          s = t * c;
+        }
+        cs[tid] = c;
+        sn[tid] = s;
+      }
+      __syncthreads();
+      // A <- A J (columns p, q) and V <- V J
+      for (int e = tid; e < half * mp; e += blockDim.x) {
+        const int k = e / mp, i = e % mp;
+        const int p = pp[k], q = qq[k];
+        const double c = cs[k], s = sn[k];
+        const double aip = A[i][p], aiq = A[i][q];
+        A[i][p] = c * aip - s * aiq;
+        A[i][q] = s * aip + c * aiq;
+        const double vip = V[i][p], viq = V[i][q];
+        V[i][p] = c * vip - s * viq;
+        V[i][q] = s * vip + c * viq;
+      }
+      __syncthreads();
+      // A <- J^T A (rows p, q)
+      for (int e = tid; e < half * mp; e += blockDim.x) {
+        const int k = e / mp, j = e % mp;
+        const int p = pp[k], q = qq[k];
+        const double c = cs[k], s = sn[k];
+        const double apj = A[p][j], aqj = A[q][j];
+        A[p][j] = c * apj - s * aqj;
+        A[q][j] = s * apj + c * aqj;
+      }
+      __syncthreads();
+    }
+  }
+  __syncthreads();
+  if (tid == 0) {
+    // selection sort of the m real eigenpairs (the padding index is dropped)
+    int order[kMaxBasis];
+    for (int i = 0; i < m; ++i) order[i] = i;
+    for (int i = 0; i < m; ++i) {
+      int best = i;
+      for (int j = i + 1; j < m; ++j)
+        if (A[order[j]][order[j]] < A[order[best]][order[best]]) best = j;
+      const int t = order[i];
+      order[i] = order[best];
+      order[best] = t;
+    }
+    for (int i = 0; i < m; ++i) {
+      const int c = order[i];
+      evals[i] = A[c][c];
+      for (int r = 0; r < m; ++r) S[i * m + r] = V[r][c];
+    }
+  }
+}
+
+// In place, row-local: V_i[r] <- sum_q S[i*m + q] V_q[r] for i < keep.
+template <class VT>
+__global__ void __launch_bounds__(kBlock)
+    restart_kernel(VT *const *__restrict__ V, int m, int keep, const double *__restrict__ S,
+                   uint64_t n) {
+  for (uint64_t r = blockIdx.x * static_cast<uint64_t>(kBlock) + threadIdx.x; r < n;
+       r += static_cast<uint64_t>(gridDim.x) * kBlock) {
+    double row[kMaxBasis];
+    for (int q = 0; q < m; ++q) row[q] = to_d(V[q][r]);
+    for (int i = 0; i < keep; ++i) {
+      double acc = 0.0;
+      for (int q = 0; q < m; ++q) acc = fma(S[i * m + q], row[q], acc);
+      V[i][r] = static_cast<VT>(acc);
+    }
+  }
+}
+
+// out[r] = sum_q S[col*m + q] V_q[r] (FP64 Ritz vector)
+template <class VT>
+__global__ void __launch_bounds__(kBlock)
+    ritz_kernel(const VT *const *__restrict__ V, int m, const double *__restrict__ s,
+                double *__restrict__ out, uint64_t n) {
+  for (uint64_t r = blockIdx.x * static_cast<uint64_t>(kBlock) + threadIdx.x; r < n;
+       r += static_cast<uint64_t>(gridDim.x) * kBlock) {
+    double acc = 0.0;
+    for (int q = 0; q < m; ++q) acc = fma(s[q], to_d(V[q][r]), acc);
+    out[r] = acc;
+  }
+}
+
+// w[r] = hy[r] - theta * y[r]
+__global__ void __launch_bounds__(kBlock)
+    residual_kernel(const double *__restrict__ hy, const double *__restrict__ y, double theta,
+                    double *__restrict__ w, uint64_t n) {
+  for (uint64_t r = blockIdx.x * static_cast<uint64_t>(kBlock) + threadIdx.x; r < n;
+       r += static_cast<uint64_t>(gridDim.x) * kBlock)
+    w[r] = fma(-theta, y[r], hy[r]);
+}
+
+template <class A, class B>
+__global__ void __launch_bounds__(kBlock) convert_kernel(const A *__restrict__ a, B *__restrict__ b,
+                                                         uint64_t n) {
+  for (uint64_t r = blockIdx.x * static_cast<uint64_t>(kBlock) + threadIdx.x; r < n;
+       r += static_cast<uint64_t>(gridDim.x) * kBlock)
+    b[r] = static_cast<B>(a[r]);
+}
+
+
+__global__ void store_column2_kernel(const double *__restrict__ h1, const double *__restrict__ h2,
+                                     int j, double *__restrict__ T, int ldt) {
+  const int i = threadIdx.x;
+  if (i <= j) {
+    const double v = h1[i] + h2[i];
+    T[i * ldt + j] = v;
+    T[j * ldt + i] = v;
+  }
+}
+
+// T <- diag(theta[0..keep)), zero elsewhere
+__global__ void reset_projected_kernel(double *__restrict__ T, int ldt,
+                                       const double *__restrict__ theta, int keep) {
+  for (int e = threadIdx.x; e < ldt * ldt; e += blockDim.x) {
+    const int i = e / ldt, j = e % ldt;
+    T[e] = (i == j && i < keep) ? theta[i] : 0.0;
+  }
+}
+
+double seconds_since(std::chrono::steady_clock::time_point t0) {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+struct PhaseResult {
+  std::vector<double> theta;  // lowest k Ritz values
+  std::vector<double> est;    // residual estimates
+  bool converged = false;
+  int cycles = 0;
+};
+
+// One thick-restart Lanczos phase with vector storage type VT.
+template <class VT> struct Phase {
+  Operator &op;
+  uint64_t n;
+  int m, k, keep;
+  hipStream_t s;
+  std::vector<DeviceBuffer<VT>> vecs;
+  DeviceBuffer<VT *> vptr;
+  DeviceBuffer<const double *> wptr;
+  DeviceBuffer<double> w, partial, h, T, theta, S, beta;
+  std::vector<hipEvent_t> ev;
+  int64_t matvecs = 0;
+  double matvec_seconds = 0.0;
+  unsigned G;
+  bool small;
+
+  Phase(Operator &o, uint64_t n_, int m_, int k_, int keep_, hipStream_t s_)
+      : op(o), n(n_), m(m_), k(k_), keep(keep_), s(s_) {
+    vecs.resize(m + 1);
+    for (auto &v : vecs) v.allocate(n);
+    std::vector<VT *> hp(m + 1);
+    for (int i = 0; i <= m; ++i) hp[i] = vecs[i].ptr;
+    vptr.allocate(m + 1);
+    SPINED_HIP(hipMemcpy(vptr.ptr, hp.data(), sizeof(VT *) * (m + 1), hipMemcpyHostToDevice));
+    w.allocate(n);
+    const double *wp = w.ptr;
+    wptr.allocate(1);
+    SPINED_HIP(hipMemcpy(wptr.ptr, &wp, sizeof(wp), hipMemcpyHostToDevice));
+    G = grid_for(n, 2048);
+    const int chunks = (m + 1 + kDotChunk - 1) / kDotChunk;
+    partial.allocate(static_cast<size_t>(chunks) * kDotChunk * G);
+    h.allocate(2 * kMaxBasis + 2);
+    T.allocate(kMaxBasis * kMaxBasis);
+    theta.allocate(kMaxBasis);
+    S.allocate(kMaxBasis * kMaxBasis);
+    beta.allocate(kMaxBasis + 1);
+    reset_projected_kernel<<<1, 256, 0, s>>>(T.ptr, kMaxBasis, theta.ptr, 0);
+    SPINED_CHECK_LAUNCH();
+    small = n <= (1u << 20);
+  }
+  ~Phase() {
+    for (auto e : ev) (void)hipEventDestroy(e);
+  }
+
+  void matvec(const void *x, VecType xt, double *y) {
+    hipEvent_t a, b;
+    SPINED_HIP(hipEventCreate(&a));
+    SPINED_HIP(hipEventCreate(&b));
+    SPINED_HIP(hipEventRecord(a, s));
+    MatvecCall c{&op, xt, VecType::F64, x, y, n, s};
+    launch_matvec(c);
+    SPINED_HIP(hipEventRecord(b, s));
+    ev.push_back(a);
+    ev.push_back(b);
+    ++matvecs;
+  }
+
+  void harvest_events() {
+    SPINED_HIP(hipStreamSynchronize(s));
+    for (size_t i = 0; i + 1 < ev.size(); i += 2) {
+      float ms = 0.f;
+      if (hipEventElapsedTime(&ms, ev[i], ev[i + 1]) == hipSuccess) matvec_seconds += ms * 1e-3;
+      (void)hipEventDestroy(ev[i]);
+      (void)hipEventDestroy(ev[i + 1]);
+    }
+    ev.clear();
+  }
+
+  template <class PT> void multidot(const PT *const *V, int nvec, const double *wv, double *hout) {
+    const int chunks = (nvec + kDotChunk - 1) / kDotChunk;
+    multidot_kernel<PT><<<dim3(G, chunks), kBlock, 0, s>>>(V, nvec, wv, n, partial.ptr);
+    SPINED_CHECK_LAUNCH();
+    reduce_partials_kernel<<<nvec, kBlock, 0, s>>>(partial.ptr, nvec, G, hout, 0);
+    SPINED_CHECK_LAUNCH();
+  }
+
+  void norm_w(double *out) {
+    multidot<double>(wptr.ptr, 1, w.ptr, out);
+    sqrt_inplace_kernel<<<1, 1, 0, s>>>(out);
+    SPINED_CHECK_LAUNCH();
+  }
+
+  // orthogonalise w against V[0..nvec) twice; h1 + h2 are the coefficients
+  void reorth(int nvec, double *h1, double *h2) {
+    multidot<VT>(vptr.ptr, nvec, w.ptr, h1);
+    gemv_sub_kernel<VT><<<G, kBlock, 0, s>>>(vptr.ptr, nvec, h1, w.ptr, n);
+    SPINED_CHECK_LAUNCH();
+    multidot<VT>(vptr.ptr, nvec, w.ptr, h2);
+    gemv_sub_kernel<VT><<<G, kBlock, 0, s>>>(vptr.ptr, nvec, h2, w.ptr, n);
+    SPINED_CHECK_LAUNCH();
+  }
+
+  // v_0 from an FP64 start vector in w (normalised here)
+  void set_start_from_w() {
+    norm_w(beta.ptr);
+    scale_store_kernel<VT><<<G, kBlock, 0, s>>>(w.ptr, beta.ptr, vecs[0].ptr, n);
+    SPINED_CHECK_LAUNCH();
+  }
+
+  void step(int j, uint64_t seed_breakdown) {
+    matvec(vecs[j].ptr, std::is_same<VT, float>::value ? VecType::F32 : VecType::F64, w.ptr);
+    double *h1 = h.ptr, *h2 = h.ptr + kMaxBasis + 1;
+    reorth(j + 1, h1, h2);
+    store_column2_kernel<<<1, kMaxBasis, 0, s>>>(h1, h2, j, T.ptr, kMaxBasis);
+    SPINED_CHECK_LAUNCH();
+    norm_w(beta.ptr + j);
+    if (small) {
+      // breakdown guard (invariant subspace): continue with a random vector
+      double bj = 0.0, tj = 0.0;
+      SPINED_HIP(hipMemcpyAsync(&bj, beta.ptr + j, 8, hipMemcpyDeviceToHost, s));
+      SPINED_HIP(hipMemcpyAsync(&tj, T.ptr + j * kMaxBasis + j, 8, hipMemcpyDeviceToHost, s));
+      SPINED_HIP(hipStreamSynchronize(s));
+      if (bj <= 1e-12 * std::max(1.0, std::fabs(tj)) && j + 1 < static_cast<int>(n)) {
+        launch_random_fill(w.ptr, n, seed_breakdown + 7919u * (j + 1), s);
+        double *g1 = h.ptr, *g2 = h.ptr + kMaxBasis + 1; // scratch (column already stored)
+        reorth(j + 1, g1, g2);
+        double nb = 0.0;
+        norm_w(beta.ptr + kMaxBasis);
+        SPINED_HIP(hipMemcpyAsync(&nb, beta.ptr + kMaxBasis, 8, hipMemcpyDeviceToHost, s));
+        SPINED_HIP(hipStreamSynchronize(s));
+        scale_store_kernel<VT><<<G, kBlock, 0, s>>>(w.ptr, beta.ptr + kMaxBasis, vecs[j + 1].ptr, n);
+        SPINED_CHECK_LAUNCH();
+        const double zero = 0.0;
+        SPINED_HIP(hipMemcpyAsync(beta.ptr + j, &zero, 8, hipMemcpyHostToDevice, s));
+        SPINED_HIP(hipStreamSynchronize(s));
+        return;
+      }
+    }
+    scale_store_kernel<VT><<<G, kBlock, 0, s>>>(w.ptr, beta.ptr + j, vecs[j + 1].ptr, n);
+    SPINED_CHECK_LAUNCH();
+  }
+
+  // Runs cycles until the k lowest estimates are <= tol, the matvec budget is
+  // spent, or (stall_detect) the estimates stop improving.
+  PhaseResult run(double tol, int64_t max_matvecs, bool stall_detect, int phase_id,
+                  spined_monitor_fn monitor, void *ctx, bool verbose, uint64_t seed) {
+    PhaseResult res;
+    int j0 = 0;
+    double best = 1e300;
+    int stalled = 0;
+    std::vector<double> th(m), Sh(static_cast<size_t>(m) * m);
+    for (;;) {
+      for (int j = j0; j < m; ++j) step(j, seed);
+      jacobi_eigh_kernel<<<1, 256, 0, s>>>(T.ptr, kMaxBasis, m, theta.ptr, S.ptr, 40);
+      SPINED_CHECK_LAUNCH();
+      double bm = 0.0;
+      SPINED_HIP(hipMemcpyAsync(th.data(), theta.ptr, 8 * m, hipMemcpyDeviceToHost, s));
+      SPINED_HIP(hipMemcpyAsync(Sh.data(), S.ptr, 8ull * m * m, hipMemcpyDeviceToHost, s));
+      SPINED_HIP(hipMemcpyAsync(&bm, beta.ptr + (m - 1), 8, hipMemcpyDeviceToHost, s));
+      harvest_events();
+      ++res.cycles;
+      res.theta.assign(th.begin(), th.begin() + k);
+      res.est.resize(k);
+      double worst = 0.0;
+      int nconv = 0;
+      for (int i = 0; i < k; ++i) {
+        res.est[i] = std::fabs(bm * Sh[static_cast<size_t>(i) * m + (m - 1)]);
+        worst = std::max(worst, res.est[i]);
+        if (res.est[i] <= tol) ++nconv;
+      }
+      if (verbose)
+        log_line("phase %d cycle %d matvecs %lld theta0 %.15g est %.3e", phase_id, res.cycles,
+                 static_cast<long long>(matvecs), th[0], worst);
+      int stop = 0;
+      if (monitor) {
+        spined_monitor_info info{matvecs, res.cycles, phase_id, nconv, k, res.theta.data(),
+                                 res.est.data()};
+        stop = monitor(&info, ctx);
+      }
+      if (nconv == k) {
+        res.converged = true;
+        return res;
+      }
+      if (stop || (max_matvecs > 0 && matvecs >= max_matvecs) || m >= static_cast<int>(n))
+        return res;
+      if (stall_detect) {
+        if (worst < 0.5 * best) {
+          best = worst;
+          stalled = 0;
+        } else if (++stalled >= 2) {
+          return res;
+        }
+      }
+      // thick restart: keep the lowest `keep` Ritz vectors and v_m
+      restart_kernel<VT><<<G, kBlock, 0, s>>>(vptr.ptr, m, keep, S.ptr, n);
+      SPINED_CHECK_LAUNCH();
+      SPINED_HIP(hipMemcpyAsync(vecs[keep].ptr, vecs[m].ptr, n * sizeof(VT),
+                                hipMemcpyDeviceToDevice, s));
+      reset_projected_kernel<<<1, 256, 0, s>>>(T.ptr, kMaxBasis, theta.ptr, keep);
+      SPINED_CHECK_LAUNCH();
+      j0 = keep;
+    }
+  }
+
+  // FP64 Ritz vector i into out
+  void ritz(int i, double *out) {
+    ritz_kernel<VT><<<G, kBlock, 0, s>>>(vptr.ptr, m, S.ptr + static_cast<size_t>(i) * m, out, n);
+    SPINED_CHECK_LAUNCH();
+  }
+};
+
+void pick_sizes(uint64_t n, uint64_t k, const spined_eigsh_options &o, int &m, int &keep) {
+  int mm = o.max_basis_size > 0 ? o.max_basis_size : 0;
+  if (mm <= 0) mm = static_cast<int>(std::max<uint64_t>(2 * k + 10, 24));
+  mm = std::max<int>(mm, static_cast<int>(k) + 2);
+  mm = std::min(mm, kMaxBasis);
+  if (static_cast<uint64_t>(mm) > n) mm = static_cast<int>(n);
+  m = mm;
+  int kp = o.min_restart_size > 0 ? o.min_restart_size : (m + static_cast<int>(k)) / 2;
+  kp = std::max(kp, static_cast<int>(k));
+  kp = std::min(kp, m - 1);
+  keep = std::max(kp, 0);
+}
+
+} // namespace
+
+void launch_random_fill(double *v, uint64_t n, uint64_t seed, hipStream_t s) {
+  random_fill_kernel<<<grid_for(n), kBlock, 0, s>>>(v, n, seed);
+  SPINED_CHECK_LAUNCH();
+}
+
+int eigsh(Operator &op, int dtype, uint64_t k, double tol, const spined_eigsh_options &opts,
+          double *evals, void *evecs, uint64_t ld, double *rnorms, spined_monitor_fn monitor,
+          void *ctx) {
+  const auto t0 = std::chrono::steady_clock::now();
+  Basis &b = *op.basis;
+  const uint64_t n = b.D;
+  if (k == 0 || k > n) throw Error(LS_INVALID_ARGUMENT, "number of eigenpairs must be in [1, dim]");
+  if (k > 16) throw Error(LS_NOT_IMPLEMENTED, "at most 16 eigenpairs per solve");
+  if (dtype != LS_FLOAT64 && dtype != LS_FLOAT32)
+    throw Error(LS_INVALID_DATATYPE, "spined_eigsh supports real float32/float64 output");
+  if (!op.tables.is_real) throw Error(LS_INVALID_DATATYPE, "spined_eigsh needs a real operator");
+  if (evecs && ld < n) throw Error(LS_INVALID_ARGUMENT, "leading dimension smaller than dim");
+  if (!(tol > 0.0)) tol = 1e-10; // PRIMME's eps = 0 means "its default"; ours is 1e-10
+  hipStream_t s = stream();
+  int m = 0, keep = 0;
+  pick_sizes(n, k, opts, m, keep);
+  spined_stats st{};
+
+  // ---- start vector (FP64, in a scratch buffer) --------------------------
+  DeviceBuffer<double> start(n);
+  const auto ts = std::chrono::steady_clock::now();
+  if (opts.start == SPINED_START_VECTOR) {
+    if (!opts.start_vector) throw Error(LS_INVALID_ARGUMENT, "start_vector is NULL");
+    SPINED_HIP(hipMemcpyAsync(start.ptr, opts.start_vector, n * 8, hipMemcpyHostToDevice, s));
+  } else if (opts.start == SPINED_START_LEARNED) {
+    throw Error(LS_NOT_IMPLEMENTED, "learned start vectors are evaluated by spined_cnn_start");
+  } else {
+    launch_random_fill(start.ptr, n, opts.seed, s);
+  }
+  SPINED_HIP(hipStreamSynchronize(s));
+  st.seconds_start = seconds_since(ts);
+
+  std::vector<double> theta(k), est(k);
+  bool converged = false;
+  DeviceBuffer<double> y(n), hy(n);
+
+  if (opts.precision == SPINED_PRECISION_MIXED && n > static_cast<uint64_t>(m)) {
+    // FP32-storage phase down to its floor, then hand the FP64 Ritz vector
+    // (sum of the k lowest) to the FP64 phase.
+    Phase<float> p(op, n, m, static_cast<int>(k), keep, s);
+    SPINED_HIP(hipMemcpyAsync(p.w.ptr, start.ptr, n * 8, hipMemcpyDeviceToDevice, s));
+    p.set_start_from_w();
+    const double tol32 = std::max(tol, 1e-5);
+    PhaseResult r = p.run(tol32, opts.max_matvecs, true, 1, monitor, ctx, opts.verbose, opts.seed);
+    double scale = 0.0;
+    for (double v : r.theta) scale = std::max(scale, std::fabs(v));
+    (void)scale;
+    // start = sum_i Ritz_i (FP64 assembly)
+    SPINED_HIP(hipMemsetAsync(start.ptr, 0, n * 8, s));
+    for (uint64_t i = 0; i < k; ++i) {
+      p.ritz(static_cast<int>(i), y.ptr);
+      residual_kernel<<<grid_for(n), kBlock, 0, s>>>(start.ptr, y.ptr, -1.0, start.ptr, n);
+      SPINED_CHECK_LAUNCH();
+    }
+    SPINED_HIP(hipStreamSynchronize(s));
+    st.matvecs += p.matvecs;
+    st.matvecs_mixed += p.matvecs;
+    st.seconds_matvec += p.matvec_seconds;
+    st.restarts += r.cycles;
+  }
+
+  Phase<double> p(op, n, m, static_cast<int>(k), keep, s);
+  SPINED_HIP(hipMemcpyAsync(p.w.ptr, start.ptr, n * 8, hipMemcpyDeviceToDevice, s));
+  p.set_start_from_w();
+  start.release();
+  int64_t budget = opts.max_matvecs > 0 ? std::max<int64_t>(1, opts.max_matvecs - st.matvecs) : 0;
+  for (int attempt = 0; attempt < 8; ++attempt) {
+    PhaseResult r = p.run(tol, budget, false, 0, monitor, ctx, opts.verbose, opts.seed + 1);
+    st.restarts += r.cycles;
+    // explicit residuals of the Ritz vectors (FP64 matvec)
+    bool ok = true;
+    for (uint64_t i = 0; i < k; ++i) {
+      p.ritz(static_cast<int>(i), y.ptr);
+      p.matvec(y.ptr, VecType::F64, hy.ptr);
+      // Rayleigh quotient and residual norm
+      const double *yp = y.ptr;
+      DeviceBuffer<const double *> yptr(1);
+      SPINED_HIP(hipMemcpyAsync(yptr.ptr, &yp, sizeof(yp), hipMemcpyHostToDevice, s));
+      DeviceBuffer<double> sc(4);
+      p.template multidot<double>(yptr.ptr, 1, hy.ptr, sc.ptr);     // y.Hy
+      p.template multidot<double>(yptr.ptr, 1, y.ptr, sc.ptr + 1);  // y.y
+      double hsc[2];
+      SPINED_HIP(hipMemcpyAsync(hsc, sc.ptr, 16, hipMemcpyDeviceToHost, s));
+      SPINED_HIP(hipStreamSynchronize(s));
+      const double lam = hsc[0] / hsc[1];
+      const double ynorm = std::sqrt(hsc[1]);
+      residual_kernel<<<grid_for(n), kBlock, 0, s>>>(hy.ptr, y.ptr, lam, hy.ptr, n);
+      SPINED_CHECK_LAUNCH();
+      const double *hp = hy.ptr;
+      SPINED_HIP(hipMemcpyAsync(yptr.ptr, &hp, sizeof(hp), hipMemcpyHostToDevice, s));
+      p.template multidot<double>(yptr.ptr, 1, hy.ptr, sc.ptr + 2);
+      double rr = 0.0;
+      SPINED_HIP(hipMemcpyAsync(&rr, sc.ptr + 2, 8, hipMemcpyDeviceToHost, s));
+      SPINED_HIP(hipStreamSynchronize(s));
+      theta[i] = lam;
+      est[i] = std::sqrt(std::max(rr, 0.0)) / ynorm;
+      if (est[i] > tol) ok = false;
+      if (evecs) {
+        // normalise and copy out in the requested element type
+        const double inv = 1.0 / ynorm;
+        std::vector<double> hv(n);
+        SPINED_HIP(hipMemcpyAsync(hv.data(), y.ptr, n * 8, hipMemcpyDeviceToHost, s));
+        SPINED_HIP(hipStreamSynchronize(s));
+        if (dtype == LS_FLOAT64) {
+          double *o = static_cast<double *>(evecs) + i * ld;
+          for (uint64_t q = 0; q < n; ++q) o[q] = hv[q] * inv;
+        } else {
+          float *o = static_cast<float *>(evecs) + i * ld;
+          for (uint64_t q = 0; q < n; ++q) o[q] = static_cast<float>(hv[q] * inv);
+        }
+      }
+    }
+    p.harvest_events();
+    if (ok || (budget > 0 && p.matvecs >= budget)) {
+      converged = ok;
+      break;
+    }
+    // estimates said converged but the explicit residual disagrees: keep
+    // iterating from the current basis with a tighter target
+    tol *= 0.5;
+  }
+  st.matvecs += p.matvecs;
+  st.seconds_matvec += p.matvec_seconds;
+  for (uint64_t i = 0; i < k; ++i) {
+    evals[i] = theta[i];
+    if (rnorms) rnorms[i] = est[i];
+  }
+  st.seconds_total = seconds_since(t0);
+  {
+    std::lock_guard<std::mutex> lk(op.mu);
+    op.last_stats = st;
+  }
+  return converged ? LS_SUCCESS : LS_NOT_CONVERGED;
+}
+
+} // namespace spined

@@ -1,0 +1,158 @@
+"""ctypes declarations of libspined (include/lattice_symmetries.h, include/spined.h).
+
+This plays the role of the `foreign import ccall` block of
+src/SpinED/Internal.hs:39-384: one Python declaration per C symbol, with the
+same argument meaning.  The library is built in-tree (spin-ed_amd/libspined.so)
+by ``__graft_entry__.build()``; importing this module never builds it, and a
+missing library is an error (there is no fallback path).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+PKG_ROOT = Path(__file__).resolve().parent.parent
+LIB_PATH = Path(os.environ.get("SPINED_LIBRARY", PKG_ROOT / "libspined.so"))
+
+LS_SYMBOLS = (
+    "ls_error_to_string", "ls_destroy_string", "ls_enable_logging", "ls_disable_logging",
+    "ls_create_symmetry", "ls_destroy_symmetry", "ls_get_sector", "ls_get_phase",
+    "ls_get_periodicity", "ls_create_group", "ls_destroy_group", "ls_get_group_size",
+    "ls_create_spin_basis", "ls_destroy_spin_basis", "ls_build", "ls_build_unsafe",
+    "ls_get_number_states", "ls_get_states", "ls_states_get_data", "ls_states_get_size",
+    "ls_destroy_states", "ls_create_interaction1", "ls_create_interaction2",
+    "ls_create_interaction3", "ls_create_interaction4", "ls_interaction_is_real",
+    "ls_destroy_interaction", "ls_create_operator", "ls_destroy_operator",
+    "ls_operator_matmat", "ls_operator_expectation", "ls_operator_is_real",
+)
+
+SPINED_SYMBOLS = (
+    "spined_eigsh_options_init", "spined_eigsh", "spined_get_last_stats", "spined_stream",
+    "spined_synchronize", "spined_basis_device_states", "spined_operator_matmat_device",
+    "spined_operator_count_offdiag", "spined_operator_orbit_method",
+    "spined_operator_set_orbit_method", "spined_basis_set_bucket_bits",
+    "spined_basis_get_bucket_bits", "spined_basis_sector_dimension", "spined_debug_orbit",
+    "spined_debug_ring_available",
+)
+
+
+class SpinedEigshOptions(C.Structure):
+    _fields_ = [
+        ("precision", C.c_int32),
+        ("start", C.c_int32),
+        ("seed", C.c_uint64),
+        ("max_basis_size", C.c_int32),
+        ("min_restart_size", C.c_int32),
+        ("max_block_size", C.c_int32),
+        ("verbose", C.c_int32),
+        ("max_matvecs", C.c_int64),
+        ("start_vector", C.c_void_p),
+        ("model", C.c_void_p),
+    ]
+
+
+class SpinedMonitorInfo(C.Structure):
+    _fields_ = [
+        ("matvecs", C.c_int64),
+        ("restarts", C.c_int32),
+        ("phase", C.c_int32),
+        ("nconverged", C.c_int32),
+        ("k", C.c_int32),
+        ("ritz_values", C.POINTER(C.c_double)),
+        ("residuals", C.POINTER(C.c_double)),
+    ]
+
+
+class SpinedStats(C.Structure):
+    _fields_ = [
+        ("matvecs", C.c_int64),
+        ("matvecs_mixed", C.c_int64),
+        ("restarts", C.c_int32),
+        ("seconds_total", C.c_double),
+        ("seconds_matvec", C.c_double),
+        ("seconds_start", C.c_double),
+    ]
+
+
+MONITOR_FN = C.CFUNCTYPE(C.c_int, C.POINTER(SpinedMonitorInfo), C.c_void_p)
+
+_lib = None
+
+
+def load():
+    """Load libspined; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise OSError(f"{LIB_PATH} is missing: run __graft_entry__.build() (make -C spin-ed_amd/csrc)")
+    L = C.CDLL(str(LIB_PATH))
+    P, I, U, U64, D = C.c_void_p, C.c_int, C.c_uint, C.c_uint64, C.c_double
+    PP = C.POINTER(C.c_void_p)
+    sig = {
+        "ls_error_to_string": (C.c_void_p, [I]),
+        "ls_destroy_string": (None, [C.c_void_p]),
+        "ls_enable_logging": (None, []),
+        "ls_disable_logging": (None, []),
+        "ls_create_symmetry": (I, [PP, U, P, U]),
+        "ls_destroy_symmetry": (None, [P]),
+        "ls_get_sector": (U, [P]),
+        "ls_get_phase": (D, [P]),
+        "ls_get_periodicity": (U, [P]),
+        "ls_create_group": (I, [PP, U, P]),
+        "ls_destroy_group": (None, [P]),
+        "ls_get_group_size": (U, [P]),
+        "ls_create_spin_basis": (I, [PP, P, U, I, I]),
+        "ls_destroy_spin_basis": (None, [P]),
+        "ls_build": (I, [P]),
+        "ls_build_unsafe": (I, [P, U64, P]),
+        "ls_get_number_states": (I, [P, C.POINTER(U64)]),
+        "ls_get_states": (I, [PP, P]),
+        "ls_states_get_data": (C.POINTER(U64), [P]),
+        "ls_states_get_size": (U64, [P]),
+        "ls_destroy_states": (None, [P]),
+        "ls_create_interaction1": (I, [PP, P, U, P]),
+        "ls_create_interaction2": (I, [PP, P, U, P]),
+        "ls_create_interaction3": (I, [PP, P, U, P]),
+        "ls_create_interaction4": (I, [PP, P, U, P]),
+        "ls_interaction_is_real": (C.c_bool, [P]),
+        "ls_destroy_interaction": (None, [P]),
+        "ls_create_operator": (I, [PP, P, U, P]),
+        "ls_destroy_operator": (None, [P]),
+        "ls_operator_matmat": (I, [P, I, U64, U64, P, U64, P, U64]),
+        "ls_operator_expectation": (I, [P, I, U64, U64, P, U64, P]),
+        "ls_operator_is_real": (C.c_bool, [P]),
+        "spined_eigsh_options_init": (None, [C.POINTER(SpinedEigshOptions)]),
+        "spined_eigsh": (I, [P, I, U64, D, C.POINTER(SpinedEigshOptions), P, P, U64, P,
+                             MONITOR_FN, P]),
+        "spined_get_last_stats": (I, [P, C.POINTER(SpinedStats)]),
+        "spined_stream": (P, []),
+        "spined_synchronize": (I, []),
+        "spined_basis_device_states": (I, [P, PP, PP, C.POINTER(U64)]),
+        "spined_operator_matmat_device": (I, [P, I, U64, U64, P, U64, P, U64, I]),
+        "spined_operator_count_offdiag": (I, [P, C.POINTER(U64)]),
+        "spined_operator_orbit_method": (I, [P]),
+        "spined_operator_set_orbit_method": (I, [P, I]),
+        "spined_basis_set_bucket_bits": (I, [P, I]),
+        "spined_basis_get_bucket_bits": (I, [P]),
+        "spined_basis_sector_dimension": (I, [P, C.POINTER(D)]),
+        "spined_debug_orbit": (I, [P, I, U64, P, P, P, P, P]),
+        "spined_debug_ring_available": (I, [P]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def error_message(code: int) -> str:
+    """getErrorMessage (Internal.hs:48-54): owned string, freed after reading."""
+    L = load()
+    p = L.ls_error_to_string(code)
+    try:
+        return C.cast(p, C.c_char_p).value.decode()
+    finally:
+        L.ls_destroy_string(p)

@@ -93,6 +93,8 @@ int spined_synchronize(void);
  * sizes; valid while the basis lives and is built. */
 int spined_basis_device_states(const ls_spin_basis *basis, const uint64_t **reps,
                                const uint32_t **stabilizers, uint64_t *size);
+/* Host copy of the stabiliser sizes |Stab r| (norm N_r = sqrt(|Stab r| / |G|)). */
+int spined_basis_get_stabilizers(const ls_spin_basis *basis, uint32_t *out);
 /* y = H x with DEVICE pointers, enqueued on spined_stream(); dtype as in
  * ls_operator_matmat plus mixed mode: x float32 / y float64 when
  * dtype = LS_FLOAT32 and y_is_fp64 = 1. */

@@ -55,6 +55,8 @@ def lib():
         L.orc_build_range.argtypes = [P, C.c_uint64, C.c_uint64]
         L.orc_set_basis.restype = C.c_int
         L.orc_set_basis.argtypes = [P, C.c_int64, u64p]
+        L.orc_set_basis_with_stab.restype = C.c_int
+        L.orc_set_basis_with_stab.argtypes = [P, C.c_int64, u64p, u32p]
         L.orc_dimension.restype = C.c_int64
         L.orc_dimension.argtypes = [P]
         L.orc_get_basis.argtypes = [P, u64p, u32p]
@@ -182,6 +184,13 @@ class System:
         st = lib().orc_set_basis(self._h, len(r), r)
         if st:
             raise OracleError(st, "set_basis")
+
+    def set_basis_with_stab(self, reps, stab):
+        r = np.ascontiguousarray(reps, dtype=np.uint64)
+        c = np.ascontiguousarray(stab, dtype=np.uint32)
+        st = lib().orc_set_basis_with_stab(self._h, len(r), r, c)
+        if st:
+            raise OracleError(st, "set_basis_with_stab")
 
     @property
     def dimension(self):

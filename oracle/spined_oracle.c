@@ -563,6 +563,20 @@ int orc_set_basis(orc_system *s, int64_t n, const uint64_t *reps) {
   return ORC_OK;
 }
 
+int orc_set_basis_with_stab(orc_system *s, int64_t n, const uint64_t *reps,
+                            const uint32_t *stab) {
+  free(s->reps);
+  free(s->stab);
+  s->reps = (uint64_t *)malloc(sizeof(uint64_t) * (n ? n : 1));
+  s->stab = (uint32_t *)malloc(sizeof(uint32_t) * (n ? n : 1));
+  if (!s->reps || !s->stab) return ORC_OUT_OF_MEMORY;
+  memcpy(s->reps, reps, sizeof(uint64_t) * n);
+  memcpy(s->stab, stab, sizeof(uint32_t) * n);
+  s->D = n;
+  s->built = 1;
+  return ORC_OK;
+}
+
 int64_t orc_dimension(const orc_system *s) { return s->built ? s->D : -1; }
 
 void orc_get_basis(const orc_system *s, uint64_t *reps, uint32_t *stab) {

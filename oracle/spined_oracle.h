@@ -89,6 +89,9 @@ double orc_sector_dimension(const orc_system *s, double *imag);
 int64_t orc_build(orc_system *s);
 int64_t orc_build_range(orc_system *s, uint64_t first, uint64_t last); /* enumeration restricted to [first,last] */
 int orc_set_basis(orc_system *s, int64_t n, const uint64_t *reps); /* like ls_build_unsafe */
+/* Adopt representatives together with their stabiliser sizes (no orbit
+ * recomputation; used to time the CPU matvec on a large basis). */
+int orc_set_basis_with_stab(orc_system *s, int64_t n, const uint64_t *reps, const uint32_t *stab);
 int64_t orc_dimension(const orc_system *s);
 void orc_get_basis(const orc_system *s, uint64_t *reps, uint32_t *stab);
 int64_t orc_index(const orc_system *s, uint64_t rep); /* -1 if absent */

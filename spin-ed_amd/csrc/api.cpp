@@ -260,6 +260,7 @@ int ls_build_unsafe(ls_spin_basis *basis, uint64_t size, const uint64_t represen
     if (!basis || (size && !representatives)) throw Error(LS_INVALID_ARGUMENT, "NULL argument");
     Basis &b = *basis->p;
     std::lock_guard<std::mutex> lk(b.mu);
+    validate_representatives(b, size, representatives);
     ensure_basis_device(b);
     adopt_basis(b, size, representatives);
   });
@@ -461,6 +462,15 @@ int spined_basis_device_states(const ls_spin_basis *basis, const uint64_t **reps
     if (reps) *reps = b.reps.ptr;
     if (stabilizers) *stabilizers = b.stab.ptr;
     if (size) *size = b.D;
+  });
+}
+
+int spined_basis_get_stabilizers(const ls_spin_basis *basis, uint32_t *out) {
+  return guarded([&] {
+    if (!basis || !out) throw Error(LS_INVALID_ARGUMENT, "NULL argument");
+    const Basis &b = *basis->p;
+    require_built(b);
+    if (b.D) SPINED_HIP(hipMemcpy(out, b.stab.ptr, b.D * 4, hipMemcpyDeviceToHost));
   });
 }
 

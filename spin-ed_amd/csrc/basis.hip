@@ -313,10 +313,23 @@ void build_basis(Basis &b) {
       std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 
+// The index (bucket table + binary search) needs strictly ascending,
+// in-range states; anything else is rejected before it reaches the GPU.
+void validate_representatives(const Basis &b, uint64_t n, const uint64_t *reps) {
+  const uint64_t mask = site_mask_of(b.n);
+  for (uint64_t i = 0; i < n; ++i) {
+    if ((reps[i] & ~mask) != 0)
+      throw Error(LS_INVALID_ARGUMENT, "representative has bits beyond number_spins");
+    if (i && reps[i] <= reps[i - 1])
+      throw Error(LS_INVALID_ARGUMENT, "representatives must be strictly ascending");
+  }
+}
+
 void adopt_basis(Basis &b, uint64_t n, const uint64_t *reps) {
   hipStream_t s = stream();
   if (n >= 0xffffffffull)
     throw Error(LS_NOT_IMPLEMENTED, "dimensions of 2^32 - 1 or more are not supported");
+  validate_representatives(b, n, reps);
   b.reps.allocate(std::max<uint64_t>(n, 1));
   if (n) SPINED_HIP(hipMemcpy(b.reps.ptr, reps, n * 8, hipMemcpyHostToDevice));
   b.D = n;

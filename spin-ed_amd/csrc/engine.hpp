@@ -98,6 +98,7 @@ struct Basis {
 double sector_dimension(const Basis &b, double *imag);
 void build_basis(Basis &b);                                  // basis.hip
 void adopt_basis(Basis &b, uint64_t n, const uint64_t *reps); // basis.hip (ls_build_unsafe)
+void validate_representatives(const Basis &b, uint64_t n, const uint64_t *reps);
 void build_index(Basis &b);                                   // basis.hip
 
 struct Interaction {

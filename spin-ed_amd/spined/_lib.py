@@ -29,7 +29,8 @@ LS_SYMBOLS = (
 
 SPINED_SYMBOLS = (
     "spined_eigsh_options_init", "spined_eigsh", "spined_get_last_stats", "spined_stream",
-    "spined_synchronize", "spined_basis_device_states", "spined_operator_matmat_device",
+    "spined_synchronize", "spined_basis_device_states", "spined_basis_get_stabilizers",
+    "spined_operator_matmat_device",
     "spined_operator_count_offdiag", "spined_operator_orbit_method",
     "spined_operator_set_orbit_method", "spined_basis_set_bucket_bits",
     "spined_basis_get_bucket_bits", "spined_basis_sector_dimension", "spined_debug_orbit",
@@ -130,6 +131,7 @@ def load():
         "spined_stream": (P, []),
         "spined_synchronize": (I, []),
         "spined_basis_device_states": (I, [P, PP, PP, C.POINTER(U64)]),
+        "spined_basis_get_stabilizers": (I, [P, P]),
         "spined_operator_matmat_device": (I, [P, I, U64, U64, P, U64, P, U64, I]),
         "spined_operator_count_offdiag": (I, [P, C.POINTER(U64)]),
         "spined_operator_orbit_method": (I, [P]),

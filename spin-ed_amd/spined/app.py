@@ -1,0 +1,207 @@
+"""The SpinED application flow -- mirror of src/SpinED.hs (orchestration) and
+app/Main.hs (runApp), on top of libspined.
+
+run_app(input.yaml) performs, in the reference's order (app/Main.hs:14-27):
+  readConfig -> toConfig -> prepareOutputFile -> buildRepresentatives ->
+  isOperatorReal -> withDatatype -> diagonalize -> computeExpectations.
+The one difference is inside `diagonalize`: the eigensolve runs on the GPU
+through spined_eigsh instead of PRIMME's eigh with a host callback
+(SpinED.hs:404).
+"""
+from __future__ import annotations
+
+import argparse
+import logging
+import sys
+import time
+from dataclasses import dataclass
+from typing import List, Optional
+
+import numpy as np
+
+from . import hdf5, internal as I
+from .config import ConfigSpec, OperatorSpec, read_config
+from .solver import eigsh
+
+log = logging.getLogger("spined")
+
+BASIS_PATH = "/basis"
+HAMILTONIAN_PATH = "/hamiltonian"
+OBSERVABLES_PATH = "/observables"
+WORKSPACE_PATH = "/_workspace"
+
+
+@dataclass
+class Operator:
+    """Operator (SpinED.hs:193-196): a name and the native operator."""
+    name: str
+    obj: I.Operator_
+
+
+@dataclass
+class UserConfig:
+    """UserConfig (SpinED.hs:202-213)."""
+    basis: I.SpinBasis
+    hamiltonian: Operator
+    observables: List[Operator]
+    output: str
+    num_evals: int
+    eps: float
+    datatype: str
+    max_basis_size: int
+    max_block_size: int
+    min_restart_size: int
+    spec: ConfigSpec
+
+
+def to_symmetry(s):
+    return I.mk_symmetry(s.permutation, s.sector)
+
+
+def to_basis(spec) -> I.SpinBasis:
+    """toBasis (SpinED.hs:178-188)."""
+    if spec.number_spins > 64:
+        raise I.SpinEDException(
+            f"invalid number_spins: {spec.number_spins}; exact diagonalization is not feasible "
+            "for systems larger than 64 spins")
+    log.debug("Building symmetry group...")
+    group = I.mk_group([to_symmetry(s) for s in spec.symmetries])
+    log.info("Symmetry group contains %d elements", group.size)
+    return I.mk_basis(group, spec.number_spins, spec.hamming_weight, spec.spin_inversion)
+
+
+def to_interaction(spec) -> I.Interaction:
+    return I.mk_interaction(spec.matrix, spec.sites)
+
+
+def to_operator(basis: I.SpinBasis, spec: OperatorSpec) -> Operator:
+    """toOperator (SpinED.hs:198-200)."""
+    return Operator(spec.name, I.mk_operator(basis, [to_interaction(t) for t in spec.terms]))
+
+
+def to_config(spec: ConfigSpec) -> UserConfig:
+    """toConfig (SpinED.hs:243-248)."""
+    basis = to_basis(spec.basis)
+    ham = to_operator(basis, spec.hamiltonian)
+    obs = [to_operator(basis, o) for o in spec.observables]
+    return UserConfig(basis, ham, obs, spec.output, spec.number_vectors, spec.precision,
+                      spec.datatype, spec.max_primme_basis_size, spec.max_primme_block_size,
+                      spec.min_primme_restart_size, spec)
+
+
+def prepare_output_file(cfg: UserConfig) -> None:
+    """prepareOutputFile (SpinED.hs:289-294)."""
+    with hdf5.File(cfg.output, "a") as f:
+        for g in (BASIS_PATH, HAMILTONIAN_PATH, OBSERVABLES_PATH, WORKSPACE_PATH):
+            f.create_group(g)
+
+
+def build_representatives(cfg: UserConfig) -> None:
+    """buildRepresentatives (SpinED.hs:315-337): reload /basis/representatives
+    when present (ls_build_unsafe), else build and store them."""
+    reps = None
+    with hdf5.File(cfg.output, "a") as f:
+        if f.exists(BASIS_PATH + "/representatives"):
+            log.info("Loading representatives from %s/representatives...", BASIS_PATH)
+            reps = f.read(BASIS_PATH + "/representatives", np.uint64)
+    if reps is None:
+        log.info("Building a list of representatives...")
+    I.build_basis(cfg.basis, reps)
+    dim = I.get_number_states(cfg.basis)
+    log.info("Hilbert space dimension is %d", dim)
+    if reps is None:
+        with hdf5.File(cfg.output, "a") as f:
+            f.write(BASIS_PATH + "/representatives", I.basis_get_states(cfg.basis))
+
+
+def with_datatype(is_real: bool, datatype: str):
+    """withDatatype (SpinED.hs:344-352)."""
+    if is_real:
+        return np.float32 if datatype == "float32" else np.float64
+    return np.complex64 if datatype == "float32" else np.complex128
+
+
+def diagonalize(cfg: UserConfig, dtype, model=None):
+    """diagonalize (SpinED.hs:370-411): lowest number_vectors eigenpairs,
+    written to /hamiltonian/{eigenvalues, eigenvectors, residuals}."""
+    dtype = np.dtype(dtype)
+    if dtype.kind == "c":
+        raise NotImplementedError("complex Hamiltonians are not yet supported by spined_eigsh")
+    log.debug("Using max_primme_basis_size=%d, max_primme_block_size=%d, "
+              "min_primme_restart_size=%d", cfg.max_basis_size, cfg.max_block_size,
+              cfg.min_restart_size)
+    log.info("Diagonalizing %s...", cfg.hamiltonian.name)
+
+    def monitor(info):
+        log.info("restart %d matvecs %d ritz %s residuals %s", info["restarts"], info["matvecs"],
+                 info["ritz_values"], info["residuals"])
+        return False
+
+    start_vector = None
+    if cfg.spec.start_vector == "learned":
+        from . import learned
+        start_vector = learned.start_vector_for(cfg, model)
+    res = eigsh(cfg.hamiltonian.obj, k=cfg.num_evals, tol=cfg.eps if cfg.eps > 0 else 1e-10,
+                precision=cfg.spec.solver_precision, seed=cfg.spec.seed,
+                start_vector=start_vector, max_basis_size=cfg.max_basis_size,
+                min_restart_size=cfg.min_restart_size, max_block_size=cfg.max_block_size,
+                dtype=dtype, monitor=monitor)
+    if res.status != 0:
+        log.warning("eigensolver stopped before reaching the tolerance: residuals %s", res.rnorms)
+    log.info("Obtained eigenvalues %s", res.evals)
+    real_t = np.float32 if dtype == np.float32 else np.float64
+    evals = res.evals.astype(real_t)
+    rnorms = res.rnorms.astype(real_t)
+    with hdf5.File(cfg.output, "a") as f:
+        f.write(HAMILTONIAN_PATH + "/eigenvalues", evals)
+        # KnownDataset (Block a): the column-major D x k block as a row-major (k, D) matrix
+        f.write(HAMILTONIAN_PATH + "/eigenvectors", np.ascontiguousarray(res.evecs.T))
+        f.write(HAMILTONIAN_PATH + "/residuals", rnorms)
+    return evals, res.evecs, rnorms
+
+
+def compute_expectations(cfg: UserConfig, evecs: np.ndarray) -> None:
+    """computeExpectations / computeExpectation (SpinED.hs:296-313)."""
+    for o in cfg.observables:
+        log.info("Computing expectation values of %s...", o.name)
+        m = I.expectation(o.obj, np.asfortranarray(evecs))
+        log.info("Obtained expectation values %s", m)
+        with hdf5.File(cfg.output, "a") as f:
+            f.write(OBSERVABLES_PATH + "/" + o.name, m.astype(np.complex128))
+
+
+def run_app(path: str, debug: bool = False) -> dict:
+    """runApp (app/Main.hs:14-27)."""
+    if debug:
+        I._lib.load().ls_enable_logging()
+    t0 = time.perf_counter()
+    cfg = to_config(read_config(path))
+    prepare_output_file(cfg)
+    build_representatives(cfg)
+    is_real = I.is_operator_real(cfg.hamiltonian.obj)
+    dtype = with_datatype(is_real, cfg.datatype)
+    evals, evecs, rnorms = diagonalize(cfg, dtype)
+    compute_expectations(cfg, evecs)
+    return dict(evals=evals, rnorms=rnorms, seconds=time.perf_counter() - t0,
+                dimension=I.get_number_states(cfg.basis), output=cfg.output)
+
+
+def main(argv=None) -> int:
+    """`spin-ed [--debug] input_file` (app/Main.hs:31-50)."""
+    p = argparse.ArgumentParser(
+        prog="spin-ed",
+        description="Exact diagonalization of spin systems on an AMD Instinct GPU. Takes a YAML "
+                    "file describing the basis, Hamiltonian and observables.")
+    p.add_argument("--debug", action="store_true", help="Enable debug output from libspined")
+    p.add_argument("-V", "--version", action="version", version="spined 0.1")
+    p.add_argument("input_file", help="Input yaml file")
+    a = p.parse_args(argv)
+    logging.basicConfig(level=logging.DEBUG if a.debug else logging.INFO,
+                        format="[%(asctime)s] %(levelname)s %(message)s")
+    out = run_app(a.input_file, a.debug)
+    log.info("done in %.3f s", out["seconds"])
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

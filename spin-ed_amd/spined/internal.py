@@ -188,6 +188,14 @@ def basis_get_states(basis: SpinBasis) -> np.ndarray:
     return out
 
 
+def basis_get_stabilizers(basis: SpinBasis) -> np.ndarray:
+    """|Stab r| per representative (N_r = sqrt(|Stab r| / |G|))."""
+    n = get_number_states(basis)
+    out = np.empty(n, dtype=np.uint32)
+    check_status(_lib.load().spined_basis_get_stabilizers(basis.ptr, out.ctypes.data))
+    return out
+
+
 def sector_dimension(basis: SpinBasis) -> float:
     out = C.c_double()
     check_status(_lib.load().spined_basis_sector_dimension(basis.ptr, C.byref(out)))

@@ -1,0 +1,170 @@
+"""C-ABI and host-logic checks that need no GPU: the library loads and
+exports every declared symbol, object creation and validation follow the
+reference's error behaviour (src/SpinED/Internal.hs, test/Spec.hs), and the
+device orbit code (run on the host through a test hook) matches the oracle."""
+import ctypes as C
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import pyoracle as O
+import spined
+from spined import _lib, lattice
+from spined.internal import LatticeSymmetriesException, SpinEDException
+
+ROOT = Path(__file__).resolve().parent.parent
+
+
+def declared_functions():
+    names = []
+    for h in (ROOT / "include").glob("*.h"):
+        text = re.sub(r"/\*.*?\*/", "", h.read_text(), flags=re.S)
+        for m in re.finditer(r"\b([a-z_][a-z0-9_]*)\s*\(", text):
+            name = m.group(1)
+            if name.startswith(("ls_", "spined_")) and not name.endswith("_fn"):
+                names.append(name)
+    return sorted(set(names))
+
+
+def test_library_exports_every_declared_symbol():
+    L = _lib.load()
+    names = declared_functions()
+    assert len([n for n in names if n.startswith("ls_")]) == 32
+    for n in names:
+        assert hasattr(L, n), n
+    assert set(_lib.LS_SYMBOLS) <= set(names)
+    assert set(_lib.SPINED_SYMBOLS) <= set(names)
+
+
+def test_error_strings_are_owned_and_freed():
+    for code in range(0, 16):
+        s = _lib.error_message(code)
+        assert isinstance(s, str) and s
+    assert _lib.error_message(0) == "no error"
+
+
+def test_symmetry_semantics_spec_hs():
+    s1 = spined.mk_symmetry([3, 2, 1, 0], 0)
+    assert spined.get_sector(s1) == 0 and spined.get_periodicity(s1) == 2
+    assert spined.get_phase(spined.mk_symmetry([1, 2, 3, 0], 1)) == 0.25
+    with pytest.raises(LatticeSymmetriesException) as e:
+        spined.mk_symmetry([4, 3, 4, 1], 0)
+    assert e.value.code == 1
+    with pytest.raises(LatticeSymmetriesException) as e:
+        spined.mk_symmetry([4, 3, 2, 1, 0], 3)
+    assert e.value.code == 2
+    with pytest.raises(SpinEDException):
+        spined.mk_symmetry([0, -1], 0)
+    with pytest.raises(SpinEDException):
+        spined.mk_symmetry([0, 1], -5)
+
+
+def test_group_semantics_spec_hs():
+    r = spined.mk_symmetry([3, 2, 1, 0], 0)
+    t = spined.mk_symmetry([1, 2, 3, 0], 0)
+    t1 = spined.mk_symmetry([1, 2, 3, 0], 1)
+    assert spined.get_group_size(spined.mk_group([r, t])) == 8
+    assert spined.get_group_size(spined.mk_group([])) == 1
+    with pytest.raises(LatticeSymmetriesException) as e:
+        spined.mk_group([r, t1])
+    assert e.value.code == 3
+
+
+def test_basis_validation():
+    g = spined.mk_group([spined.mk_symmetry([1, 2, 3, 0], 0), spined.mk_symmetry([3, 2, 1, 0], 0)])
+    b = spined.mk_basis(g, 4, 2)                         # test/Spec.hs:80-82
+    with pytest.raises(SpinEDException):
+        spined.mk_basis(g, -2)                           # test/Spec.hs:83-84
+    with pytest.raises(SpinEDException):
+        spined.mk_basis(g, 4, -1)
+    with pytest.raises(SpinEDException):
+        spined.mk_basis(g, 4, 2, 0)
+    for args, code in [((5, 2), 4), ((4, 5), 5), ((4, 1, 1), 6)]:
+        with pytest.raises(LatticeSymmetriesException) as e:
+            spined.mk_basis(g, *args)
+        assert e.value.code == code
+    with pytest.raises(LatticeSymmetriesException) as e:
+        spined.get_number_states(b)                      # not built
+    assert e.value.code == 8
+
+
+def test_interaction_validation_and_reality():
+    x = spined.mk_interaction([[1.0, 0.0], [0.0, -1.0]], [[0], [2]])   # test/Spec.hs:96-104
+    assert spined.is_real_interaction(x)
+    y = spined.mk_interaction([[0, complex(0, -1)], [complex(0, 1), 0]], [[0]])
+    assert not spined.is_real_interaction(y)
+    with pytest.raises(SpinEDException):
+        spined.mk_interaction(lattice.HEISENBERG, [])
+    with pytest.raises(SpinEDException):
+        spined.mk_interaction(lattice.HEISENBERG, [[0, 1, 2, 3, 4]])
+    with pytest.raises(SpinEDException):
+        spined.mk_interaction(lattice.HEISENBERG, [[0, 1], [2]])
+    with pytest.raises(SpinEDException):
+        spined.mk_interaction([[1, 2, 1], [4, 2, 1], [1, -2, 3]], [[0, 0, 0]])
+    with pytest.raises(SpinEDException):
+        spined.mk_interaction(lattice.HEISENBERG, [[0, -1]])
+    with pytest.raises(LatticeSymmetriesException):
+        spined.mk_interaction(lattice.HEISENBERG, [[1, 1]])
+
+
+def test_operator_reality_and_lazy_binding():
+    g = spined.mk_group(
+        [spined.mk_symmetry(p, s) for p, s in lattice.ring_symmetries(8, 0, 0)])
+    b = spined.mk_basis(g, 8, 4, 1)
+    op = spined.mk_operator(b, [spined.mk_interaction(lattice.HEISENBERG, lattice.ring_bonds(8))])
+    assert spined.is_operator_real(op)
+    gc = spined.mk_group([spined.mk_symmetry(lattice.ring_translation(8), 1)])
+    bc = spined.mk_basis(gc, 8)
+    opc = spined.mk_operator(bc, [spined.mk_interaction(lattice.HEISENBERG, lattice.ring_bonds(8))])
+    assert not spined.is_operator_real(opc)
+    # before ls_build a matvec is an error, not a crash
+    with pytest.raises(LatticeSymmetriesException) as e:
+        spined.apply(op, np.zeros(7))
+    assert e.value.code == 8
+
+
+def test_sector_dimension_api_matches_oracle():
+    cases = [(24, 12, 1, lattice.ring_symmetries(24, 0, 0)),
+             (36, 18, 1, lattice.square_symmetries(6)),
+             (40, 20, 1, lattice.ring_symmetries(40, 0, 0)),
+             (12, None, None, lattice.ring_symmetries(12, 5, None))]
+    for n, w, z, gens in cases:
+        b = spined.mk_basis(spined.mk_group([spined.mk_symmetry(p, s) for p, s in gens]), n, w, z)
+        assert spined.sector_dimension(b) == pytest.approx(O.System(n, w, z, gens).sector_dimension()[0], abs=1e-6)
+
+
+@pytest.mark.parametrize("n,w,z,gens", [
+    (40, 20, 1, lattice.ring_symmetries(40, 0, 0)),
+    (24, 12, -1, lattice.ring_symmetries(24, 12, 1)),
+    (17, None, None, lattice.ring_symmetries(17, 3, None)),
+    (36, 18, 1, lattice.square_symmetries(6)),
+    (26, 13, 1, []),
+])
+def test_device_orbit_code_matches_oracle(n, w, z, gens):
+    b = spined.mk_basis(spined.mk_group([spined.mk_symmetry(p, s) for p, s in gens]), n, w, z)
+    o = O.System(n, w, z, gens)
+    rng = np.random.default_rng(n)
+    if w is None:
+        st = rng.integers(0, 1 << n, 3000, dtype=np.uint64)
+    else:
+        st = np.array([sum(1 << int(i) for i in rng.choice(n, w, replace=False)) for _ in range(3000)],
+                      dtype=np.uint64)
+    ref = [o.orbit(int(x)) for x in st]
+    methods = [0] + ([1] if spined.ring_available(b) else [])
+    assert spined.ring_available(b) == (len(gens) > 0 and n in (40, 24, 17))
+    for m in methods:
+        reps, ph, cnt, val = spined.debug_orbit(b, st, m)
+        assert reps.tolist() == [r[0] for r in ref]
+        assert cnt.tolist() == [r[2] for r in ref]
+        assert val.tolist() == [r[3] for r in ref]
+        assert all(p == r[1] for p, r, v in zip(ph, ref, val) if v)
+
+
+def test_build_unsafe_rejects_unsorted_input():
+    g = spined.mk_group([])
+    b = spined.mk_basis(g, 6, 3)
+    with pytest.raises(LatticeSymmetriesException) as e:
+        spined.build_basis(b, np.array([7, 11, 13, 12], dtype=np.uint64))
+    assert e.value.code == 9

@@ -69,9 +69,9 @@ def main():
                                    "note only (rows by contiguous representative ranges + "
                                    "all-gather of x), not a measured path", "n_gpus": a.gpus}))
         return 2
-    import torch
     import spined
     from spined import lattice
+    from spined.device import DeviceArray, device_synchronize
 
     n, desc = CONFIGS[a.config]
     big = n >= 36
@@ -85,7 +85,7 @@ def main():
     op = spined.mk_operator(basis, [spined.mk_interaction(lattice.HEISENBERG,
                                                           lattice.ring_bonds(n))])
     spined.build_basis(basis)
-    torch.cuda.synchronize()
+    device_synchronize()
     t_build = time.perf_counter() - t0
     D = spined.get_number_states(basis)
     log(f"{a.config}: D={D} basis build {t_build:.3f} s")
@@ -93,12 +93,12 @@ def main():
     log(f"off-diagonal gathers per matvec: {e_off}")
 
     # one untimed matvec warms the kernels even when warmup = 0
-    x = torch.ones(D, dtype=torch.float64, device="cuda")
-    y = torch.empty_like(x)
-    torch.cuda.synchronize()
+    x = DeviceArray.from_numpy(np.ones(D))
+    y = DeviceArray(D, np.float64)
     spined.matvec_device(op, x, y, 1)
     spined.synchronize()
-    del x, y
+    x.free()
+    y.free()
 
     def solve():
         return spined.eigsh(op, k=1, tol=a.tol, precision=a.precision, seed=0x5EED,
@@ -110,16 +110,14 @@ def main():
         log(f"warmup {i}: E0={r.evals[0]:.15f} res={r.rnorms[0]:.2e} {r.stats}")
 
     results = []
-    torch.cuda.synchronize()
-    spined.synchronize()
+    device_synchronize()
     t_start = time.perf_counter()
     for i in range(steps):
         r = solve()
         results.append(r)
         log(f"step {i}: E0={r.evals[0]:.15f} res={r.rnorms[0]:.2e} matvecs={r.stats['matvecs']} "
             f"t={r.stats['seconds_total']:.3f}s matvec_t={r.stats['seconds_matvec']:.3f}s")
-    spined.synchronize()
-    torch.cuda.synchronize()
+    device_synchronize()
     elapsed = time.perf_counter() - t_start
     per_step = elapsed / steps
     last = results[-1]

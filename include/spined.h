@@ -47,6 +47,7 @@ typedef struct spined_eigsh_options {
   int64_t max_matvecs;      /* 0 = auto */
   const double *start_vector;
   const void *model;        /* spined_cnn* for SPINED_START_LEARNED */
+  uint64_t start_sign_mask; /* SPINED_START_LEARNED: Marshall sublattice mask */
 } spined_eigsh_options;
 
 typedef struct spined_monitor_info {
@@ -88,7 +89,17 @@ int spined_get_last_stats(const ls_operator *op, spined_stats *out);
 /* ---- device-resident helpers ------------------------------------------- */
 /* The library's HIP stream for the current device (hipStream_t). */
 void *spined_stream(void);
+/* Waits for the library stream. */
 int spined_synchronize(void);
+/* Waits for the whole device. */
+int spined_device_synchronize(void);
+/* Device memory owned by the caller (hipMalloc/hipFree of this library's
+ * runtime), and copies / fills on the library stream (synchronous). */
+int spined_device_malloc(void **out, uint64_t bytes);
+int spined_device_free(void *ptr);
+/* kind: 0 host->device, 1 device->host, 2 device->device */
+int spined_memcpy(void *dst, const void *src, uint64_t bytes, int kind);
+int spined_memset(void *dst, int value, uint64_t bytes);
 /* Device pointer to the basis' representatives (ascending) and stabiliser
  * sizes; valid while the basis lives and is built. */
 int spined_basis_device_states(const ls_spin_basis *basis, const uint64_t **reps,
@@ -115,7 +126,36 @@ int spined_basis_get_bucket_bits(const ls_spin_basis *basis);
 /* Exact sector dimension by character counting (before ls_build). */
 int spined_basis_sector_dimension(const ls_spin_basis *basis, double *out);
 
+/* ---- learned start vector (no counterpart in the reference) ------------ */
+/* Periodic 1D CNN over +-1 spins: 3 circular conv layers (16 channels,
+ * kernel 5, tanh-GELU), global sum, heads log|psi| and sign logit; 2,722
+ * FP32 master parameters, BF16 MFMA kernels with FP32 accumulation. */
+typedef struct spined_cnn spined_cnn;
+int spined_cnn_num_params(void);
+int spined_cnn_create(spined_cnn **out, const float *init_params);
+void spined_cnn_destroy(spined_cnn *model);
+int spined_cnn_get_params(const spined_cnn *model, float *out);
+int spined_cnn_set_params(spined_cnn *model, const float *in);
+/* Device buffers: states[count]; out_a/out_t[count]; z_store (may be NULL)
+ * holds 3 * ceil(count/16)*16 * n * 16 floats of pre-activations. */
+int spined_cnn_forward_device(const spined_cnn *model, int n, uint64_t count,
+                              const uint64_t *states, float *out_a, float *out_t, float *z_store);
+/* Gradient of sum_s ga[s] a_s + gt[s] t_s (device buffers) into grad_out
+ * (device, num_params floats) or, if NULL, into the model for adam_step. */
+int spined_cnn_backward_device(spined_cnn *model, int n, uint64_t count, const uint64_t *states,
+                               const float *z_store, const float *ga, const float *gt,
+                               float *grad_out);
+int spined_cnn_adam_step(spined_cnn *model, float lr);
+/* Start vector over the basis (device FP64, length = dimension):
+ * c_r = (-1)^popcount(r & marshall_mask) sign(t_r) exp(a_r - max a) / N_r. */
+int spined_cnn_start_vector(const spined_cnn *model, const ls_spin_basis *basis,
+                            uint64_t marshall_mask, double *v_device);
+
 /* ---- test hooks (no counterpart in the reference) ---------------------- */
+/* One v_mfma_f32_16x16x32_bf16 on known operands (lane-layout check):
+ * out[16 x 16] row-major = A B with A[i][k] = (3i + k) % 7 - 3 and
+ * B[k][j] = (3k + 7j) % 5 - 2. */
+int spined_debug_mfma_probe(float *out_host);
 /* Runs the device orbit code (the same inline functions the kernels use) on
  * the host for `count` states: method 0 = generic programs, 1 = ring.  For
  * CPU-side verification against the oracle; no compute path uses it. */

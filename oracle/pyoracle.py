@@ -78,8 +78,30 @@ def lib():
         L.orc_lanczos.restype = C.c_int
         L.orc_lanczos.argtypes = [P, C.c_int, C.c_int, C.c_double, C.c_uint64, P, f64p, P, f64p,
                                   C.POINTER(C.c_int), P, P]
+        L.cnn_num_params.restype = C.c_int
+        L.cnn_forward.argtypes = [f64p, C.c_int, C.c_int64, u64p, f64p]
+        L.cnn_backward.argtypes = [f64p, C.c_int, C.c_int64, u64p, f64p, f64p, f64p]
         _lib = L
     return _lib
+
+
+def cnn_forward(params, n, states):
+    """FP64 (log|psi|, sign logit) per state of the learned-start network."""
+    p = np.ascontiguousarray(params, dtype=np.float64)
+    s = np.ascontiguousarray(states, dtype=np.uint64)
+    out = np.empty(2 * len(s))
+    lib().cnn_forward(p, n, len(s), s, out)
+    return out[0::2].copy(), out[1::2].copy()
+
+
+def cnn_backward(params, n, states, ga, gt):
+    """FP64 gradient of sum(ga * a + gt * t) with respect to the parameters."""
+    p = np.ascontiguousarray(params, dtype=np.float64)
+    s = np.ascontiguousarray(states, dtype=np.uint64)
+    g = np.zeros(len(p))
+    lib().cnn_backward(p, n, len(s), s, np.ascontiguousarray(ga, dtype=np.float64),
+                       np.ascontiguousarray(gt, dtype=np.float64), g)
+    return g
 
 
 class OracleError(RuntimeError):

@@ -6,28 +6,9 @@
 #include <cstring>
 #include <vector>
 
-#include "engine.hpp"
+#include "handles.hpp"
 
 using namespace spined;
-
-struct ls_symmetry {
-  std::shared_ptr<const Symmetry> p;
-};
-struct ls_group {
-  std::shared_ptr<const Group> p;
-};
-struct ls_spin_basis {
-  std::shared_ptr<Basis> p;
-};
-struct ls_states {
-  std::shared_ptr<std::vector<uint64_t>> data;
-};
-struct ls_interaction {
-  std::shared_ptr<const Interaction> p;
-};
-struct ls_operator {
-  std::shared_ptr<Operator> p;
-};
 
 namespace spined {
 
@@ -451,6 +432,51 @@ void *spined_stream(void) {
 
 int spined_synchronize(void) {
   return guarded([] { SPINED_HIP(hipStreamSynchronize(stream())); });
+}
+
+int spined_device_synchronize(void) {
+  return guarded([] { SPINED_HIP(hipDeviceSynchronize()); });
+}
+
+int spined_device_malloc(void **out, uint64_t bytes) {
+  return guarded([&] {
+    if (!out) throw Error(LS_INVALID_ARGUMENT, "NULL argument");
+    void *p = nullptr;
+    const hipError_t e = hipMalloc(&p, bytes ? bytes : 1);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      throw Error(LS_OUT_OF_MEMORY, "hipMalloc failed");
+    }
+    *out = p;
+  });
+}
+
+int spined_device_free(void *ptr) {
+  return guarded([&] {
+    if (ptr) SPINED_HIP(hipFree(ptr));
+  });
+}
+
+int spined_memcpy(void *dst, const void *src, uint64_t bytes, int kind) {
+  return guarded([&] {
+    if (bytes == 0) return;
+    if (!dst || !src) throw Error(LS_INVALID_ARGUMENT, "NULL argument");
+    hipMemcpyKind k = kind == 0 ? hipMemcpyHostToDevice
+                      : kind == 1 ? hipMemcpyDeviceToHost
+                                  : hipMemcpyDeviceToDevice;
+    if (kind < 0 || kind > 2) throw Error(LS_INVALID_ARGUMENT, "kind must be 0, 1 or 2");
+    SPINED_HIP(hipMemcpyAsync(dst, src, bytes, k, stream()));
+    SPINED_HIP(hipStreamSynchronize(stream()));
+  });
+}
+
+int spined_memset(void *dst, int value, uint64_t bytes) {
+  return guarded([&] {
+    if (bytes == 0) return;
+    if (!dst) throw Error(LS_INVALID_ARGUMENT, "NULL argument");
+    SPINED_HIP(hipMemsetAsync(dst, value, bytes, stream()));
+    SPINED_HIP(hipStreamSynchronize(stream()));
+  });
 }
 
 int spined_basis_device_states(const ls_spin_basis *basis, const uint64_t **reps,

@@ -1,0 +1,692 @@
+// cnn.hip -- the learned start vector: a periodic 1D CNN over +-1 spins with
+// hand-written BF16 MFMA forward and backward kernels (FP32 accumulate),
+// Adam training on the GPU and evaluation over all representatives.
+//
+// Network (2,722 parameters; the oracle's cnn_oracle.c is its FP64 twin):
+//   h0[i] = +1 if bit i is set else -1
+//   z_l[c][i] = b_l[c] + sum_{c', d<5} W_l[c][c'][d] h_{l-1}[c'][(i+d-2) mod N]
+//   h_l = GELU_tanh(z_l), l = 1..3, 16 channels;  f[c] = sum_i h3[c][i]
+//   a = u0 + u.f  (log|psi|),  t = v0 + v.f  (sign logit)
+//
+// MFMA mapping (v_mfma_f32_16x16x32_bf16, one wave = 16 samples of equal N):
+//   D[co][s] += A[co][k] B[k][s], k = 16 d + ci (80 used of 96 = 3 steps)
+//   A = weights (registers), B = activations from LDS laid out [s][pos][c]
+//   (one 16-byte load per lane per step), D lands as 4 channels of one
+//   sample per lane (one 8-byte LDS store).
+// Fragment layout (checked on the GPU by spined_debug_mfma_probe):
+//   A: lane l holds row l%16, k = 8(l/16) .. +7;  B: column l%16, same k;
+//   D: column l%16, rows 4(l/16) .. +3.
+// Backward: the data gradient is the same conv with W'[ci][co][d] =
+// W[co][ci][4-d]; weight gradients are MFMAs with K running over 32
+// (sample, position) slots.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "handles.hpp"
+
+namespace spined {
+
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+constexpr int CH = 16;
+constexpr int KW = 5;
+constexpr int OFF_W1 = 0, OFF_B1 = OFF_W1 + CH * KW, OFF_W2 = OFF_B1 + CH,
+              OFF_B2 = OFF_W2 + CH * CH * KW, OFF_W3 = OFF_B2 + CH,
+              OFF_B3 = OFF_W3 + CH * CH * KW, OFF_U = OFF_B3 + CH, OFF_U0 = OFF_U + CH,
+              OFF_V = OFF_U0 + 1, OFF_V0 = OFF_V + CH, NPARAMS = OFF_V0 + 1;
+constexpr int kMaxN = 64;
+
+constexpr float GELU_A = 0.7978845608028654f;
+constexpr float GELU_B = 0.044715f;
+
+__device__ inline float gelu_f(float x) {
+  return 0.5f * x * (1.0f + tanhf(GELU_A * (x + GELU_B * x * x * x)));
+}
+__device__ inline float gelu_grad_f(float x) {
+  const float t = tanhf(GELU_A * (x + GELU_B * x * x * x));
+  return 0.5f * (1.0f + t) + 0.5f * x * (1.0f - t * t) * GELU_A * (1.0f + 3.0f * GELU_B * x * x);
+}
+
+__device__ inline floatx4 mfma(bf16x8 a, bf16x8 b, floatx4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+__device__ inline bf16x8 zero8() {
+  bf16x8 z;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) z[j] = static_cast<__bf16>(0.0f);
+  return z;
+}
+
+// A fragments of a 16->16 conv: A[co][k] = W[co][ci][d], k = 16 d + ci.
+// transpose = true builds the data-gradient weights W'[ci][co][d] =
+// W[co][ci][4-d] (rows indexed by ci, k = 16 d + co).
+__device__ inline void load_conv_weights(const float *W, bool transpose, bf16x8 (&a)[3]) {
+  const int lane = threadIdx.x & 63, row = lane & 15, g = lane >> 4;
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int k = 32 * q + 8 * g + j;
+      float v = 0.0f;
+      if (k < 16 * KW) {
+        const int d = k >> 4, c = k & 15;
+        v = transpose ? W[(c * CH + row) * KW + (KW - 1 - d)] : W[(row * CH + c) * KW + d];
+      }
+      a[q][j] = static_cast<__bf16>(v);
+    }
+  }
+}
+
+// A fragment of layer 1: A[co][k] = W1[co][0][k] for k < 5.
+__device__ inline bf16x8 load_first_weights(const float *W1) {
+  const int lane = threadIdx.x & 63, row = lane & 15, g = lane >> 4;
+  bf16x8 a;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    const int k = 8 * g + j;
+    a[j] = static_cast<__bf16>(k < KW ? W1[row * KW + k] : 0.0f);
+  }
+  return a;
+}
+
+// B fragment for layer 1 at position pos: spins of sample s at pos + k - 2.
+__device__ inline bf16x8 spin_fragment(uint64_t state, int n, int pos) {
+  const int g = (threadIdx.x & 63) >> 4;
+  bf16x8 b = zero8();
+  if (g == 0) {
+#pragma unroll
+    for (int k = 0; k < KW; ++k) {
+      int p = pos + k - 2;
+      p = p < 0 ? p + n : (p >= n ? p - n : p);
+      b[k] = static_cast<__bf16>(((state >> p) & 1) ? 1.0f : -1.0f);
+    }
+  }
+  return b;
+}
+
+// act: [16][n][16] bf16 in LDS; B fragment of step q at position pos.
+__device__ inline bf16x8 act_fragment(const __bf16 *act, int n, int pos, int q) {
+  const int lane = threadIdx.x & 63, s = lane & 15, g = lane >> 4;
+  const int k0 = 32 * q + 8 * g;
+  const int d = k0 >> 4, c0 = k0 & 15;
+  if (d >= KW) return zero8();
+  int p = pos + d - 2;
+  p = p < 0 ? p + n : (p >= n ? p - n : p);
+  return *reinterpret_cast<const bf16x8 *>(act + (static_cast<size_t>(s) * n + p) * CH + c0);
+}
+
+__device__ inline floatx4 conv_at(const bf16x8 (&a)[3], const __bf16 *act, int n, int pos) {
+  floatx4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int q = 0; q < 3; ++q) acc = mfma(a[q], act_fragment(act, n, pos, q), acc);
+  return acc;
+}
+
+// store 4 channels (rows 4g..4g+3) of sample s at pos
+__device__ inline void store4(__bf16 *act, int n, int pos, floatx4 v) {
+  const int lane = threadIdx.x & 63, s = lane & 15, g = lane >> 4;
+  bf16x4 o;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) o[r] = static_cast<__bf16>(v[r]);
+  *reinterpret_cast<bf16x4 *>(act + (static_cast<size_t>(s) * n + pos) * CH + 4 * g) = o;
+}
+
+__device__ inline void store4_f32(float *z, int n, int pos, floatx4 v) {
+  // z: [16][n][16] fp32 (global), same indexing as the LDS activations
+  const int lane = threadIdx.x & 63, s = lane & 15, g = lane >> 4;
+  *reinterpret_cast<floatx4 *>(z + (static_cast<size_t>(s) * n + pos) * CH + 4 * g) = v;
+}
+
+__device__ inline floatx4 load4_f32(const float *z, int n, int pos) {
+  const int lane = threadIdx.x & 63, s = lane & 15, g = lane >> 4;
+  return *reinterpret_cast<const floatx4 *>(z + (static_cast<size_t>(s) * n + pos) * CH + 4 * g);
+}
+
+struct CnnArgs {
+  const float *params; // FP32 master parameters
+  int n;
+  uint64_t count;      // samples
+};
+
+// Forward: one wave per 16 samples; writes a, t per sample and optionally
+// the pre-activations z1..z3 ([sample][pos][c], fp32) for the backward pass.
+__global__ void __launch_bounds__(64)
+    cnn_forward_kernel(CnnArgs c, const uint64_t *__restrict__ states, float *__restrict__ out_a,
+                       float *__restrict__ out_t, float *__restrict__ z_store) {
+  extern __shared__ __attribute__((aligned(16))) __bf16 lds[];
+  const int n = c.n;
+  __bf16 *h1 = lds;
+  __bf16 *h2 = lds + 16 * n * CH;
+  const int lane = threadIdx.x, s = lane & 15, g = lane >> 4;
+  const uint64_t base = static_cast<uint64_t>(blockIdx.x) * 16;
+  const uint64_t sample = base + s;
+  const uint64_t state = sample < c.count ? states[sample] : 0;
+  const float *P = c.params;
+  float *z1 = nullptr, *z2 = nullptr, *z3 = nullptr;
+  if (z_store) {
+    const size_t per = static_cast<size_t>(n) * CH;
+    z1 = z_store + base * per;
+    z2 = z_store + (c.count + 15) / 16 * 16 * per + base * per;
+    z3 = z_store + 2 * ((c.count + 15) / 16 * 16) * per + base * per;
+  }
+  float bias[4];
+  // layer 1
+  {
+    const bf16x8 a = load_first_weights(P + OFF_W1);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) bias[r] = P[OFF_B1 + 4 * g + r];
+    for (int pos = 0; pos < n; ++pos) {
+      floatx4 acc = {0.f, 0.f, 0.f, 0.f};
+      acc = mfma(a, spin_fragment(state, n, pos), acc);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) acc[r] += bias[r];
+      if (z1) store4_f32(z1, n, pos, acc);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) acc[r] = gelu_f(acc[r]);
+      store4(h1, n, pos, acc);
+    }
+  }
+  __syncthreads();
+  // layer 2
+  {
+    bf16x8 a[3];
+    load_conv_weights(P + OFF_W2, false, a);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) bias[r] = P[OFF_B2 + 4 * g + r];
+    for (int pos = 0; pos < n; ++pos) {
+      floatx4 acc = conv_at(a, h1, n, pos);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) acc[r] += bias[r];
+      if (z2) store4_f32(z2, n, pos, acc);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) acc[r] = gelu_f(acc[r]);
+      store4(h2, n, pos, acc);
+    }
+  }
+  __syncthreads();
+  // layer 3 + global sum
+  float f[4] = {0.f, 0.f, 0.f, 0.f};
+  {
+    bf16x8 a[3];
+    load_conv_weights(P + OFF_W3, false, a);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) bias[r] = P[OFF_B3 + 4 * g + r];
+    for (int pos = 0; pos < n; ++pos) {
+      floatx4 acc = conv_at(a, h2, n, pos);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) acc[r] += bias[r];
+      if (z3) store4_f32(z3, n, pos, acc);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) f[r] += gelu_f(acc[r]);
+    }
+  }
+  // heads: reduce over the 4 lane groups holding channels 4g..4g+3
+  float pa = 0.f, pt = 0.f;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    pa += P[OFF_U + 4 * g + r] * f[r];
+    pt += P[OFF_V + 4 * g + r] * f[r];
+  }
+  pa += __shfl_xor(pa, 16, 64);
+  pa += __shfl_xor(pa, 32, 64);
+  pt += __shfl_xor(pt, 16, 64);
+  pt += __shfl_xor(pt, 32, 64);
+  if (g == 0 && sample < c.count) {
+    out_a[sample] = pa + P[OFF_U0];
+    out_t[sample] = pt + P[OFF_V0];
+  }
+}
+
+// Weight gradient of one conv layer: C_d[co][ci] += sum_slots dz[slot][co] *
+// h[slot shifted by d][ci]; 32 (sample, position) slots per MFMA.
+// Layer 1 (first = true): C[co][d] with h = spins.
+__device__ inline void weight_grad(const __bf16 *dz, const __bf16 *h, const uint64_t *st, int n,
+                                   bool first, floatx4 (&cacc)[KW], float &dbias) {
+  const int lane = threadIdx.x & 63, col = lane & 15, g = lane >> 4;
+  const int nslots = 16 * n;
+  for (int c0 = 0; c0 < nslots; c0 += 32) {
+    bf16x8 a;
+    int ss[8], pp[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int slot = c0 + 8 * g + j;
+      const bool ok = slot < nslots;
+      ss[j] = ok ? slot / n : 0;
+      pp[j] = ok ? slot % n : 0;
+      const float v = ok ? static_cast<float>(dz[(static_cast<size_t>(ss[j]) * n + pp[j]) * CH + col]) : 0.0f;
+      a[j] = static_cast<__bf16>(v);
+      dbias += v;
+      if (!ok) ss[j] = -1;
+    }
+    if (first) {
+      bf16x8 b;
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        float v = 0.0f;
+        if (ss[j] >= 0 && col < KW) {
+          int p = pp[j] + col - 2;
+          p = p < 0 ? p + n : (p >= n ? p - n : p);
+          v = ((st[ss[j]] >> p) & 1) ? 1.0f : -1.0f;
+        }
+        b[j] = static_cast<__bf16>(v);
+      }
+      cacc[0] = mfma(a, b, cacc[0]);
+    } else {
+#pragma unroll
+      for (int d = 0; d < KW; ++d) {
+        bf16x8 b;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+          float v = 0.0f;
+          if (ss[j] >= 0) {
+            int p = pp[j] + d - 2;
+            p = p < 0 ? p + n : (p >= n ? p - n : p);
+            v = static_cast<float>(h[(static_cast<size_t>(ss[j]) * n + p) * CH + col]);
+          }
+          b[j] = static_cast<__bf16>(v);
+        }
+        cacc[d] = mfma(a, b, cacc[d]);
+      }
+    }
+  }
+}
+
+// Backward for one wave of 16 samples: per-wave partial gradients
+// (NPARAMS floats) of sum_s ga[s] a_s + gt[s] t_s.
+__global__ void __launch_bounds__(64)
+    cnn_backward_kernel(CnnArgs c, const uint64_t *__restrict__ states,
+                        const float *__restrict__ z_store, const float *__restrict__ ga,
+                        const float *__restrict__ gt, float *__restrict__ partial) {
+  extern __shared__ __attribute__((aligned(16))) __bf16 lds[];
+  const int n = c.n;
+  const size_t act = static_cast<size_t>(16) * n * CH;
+  __bf16 *bufA = lds, *bufB = lds + act, *bufC = lds + 2 * act;
+  __shared__ uint64_t st[16];
+  __shared__ float gsh[2][16];
+  const int lane = threadIdx.x, s = lane & 15, g = lane >> 4;
+  const uint64_t base = static_cast<uint64_t>(blockIdx.x) * 16;
+  const uint64_t sample = base + s;
+  const bool valid = sample < c.count;
+  if (g == 0) {
+    st[s] = valid ? states[sample] : 0;
+    gsh[0][s] = valid ? ga[sample] : 0.0f;
+    gsh[1][s] = valid ? gt[sample] : 0.0f;
+  }
+  __syncthreads();
+  const float *P = c.params;
+  const size_t per = static_cast<size_t>(n) * CH;
+  const uint64_t padded = (c.count + 15) / 16 * 16;
+  const float *z1 = z_store + base * per;
+  const float *z2 = z_store + padded * per + base * per;
+  const float *z3 = z_store + 2 * padded * per + base * per;
+  float *out = partial + static_cast<size_t>(blockIdx.x) * NPARAMS;
+  const float gas = gsh[0][s], gts = gsh[1][s];
+
+  // heads and d3 = (ga u + gt v) gelu'(z3)  -> bufA; h2 = gelu(z2) -> bufB
+  float f[4] = {0.f, 0.f, 0.f, 0.f};
+  float coef[4];
+#pragma unroll
+  for (int r = 0; r < 4; ++r) coef[r] = gas * P[OFF_U + 4 * g + r] + gts * P[OFF_V + 4 * g + r];
+  for (int pos = 0; pos < n; ++pos) {
+    floatx4 z = load4_f32(z3, n, pos), d;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      f[r] += gelu_f(z[r]);
+      d[r] = valid ? coef[r] * gelu_grad_f(z[r]) : 0.0f;
+    }
+    store4(bufA, n, pos, d);
+    floatx4 z2v = load4_f32(z2, n, pos), h;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) h[r] = gelu_f(z2v[r]);
+    store4(bufB, n, pos, h);
+  }
+  // du[co] = sum_s ga[s] f[s][co]: reduce over the 16 samples of a lane group
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    float du = valid ? gas * f[r] : 0.0f, dv = valid ? gts * f[r] : 0.0f;
+    for (int off = 8; off > 0; off >>= 1) {
+      du += __shfl_xor(du, off, 64);
+      dv += __shfl_xor(dv, off, 64);
+    }
+    if (s == 0) {
+      out[OFF_U + 4 * g + r] = du;
+      out[OFF_V + 4 * g + r] = dv;
+    }
+  }
+  {
+    float a0 = gas, t0 = gts;
+    for (int off = 8; off > 0; off >>= 1) {
+      a0 += __shfl_xor(a0, off, 64);
+      t0 += __shfl_xor(t0, off, 64);
+    }
+    if (lane == 0) {
+      out[OFF_U0] = a0;
+      out[OFF_V0] = t0;
+    }
+  }
+  __syncthreads();
+
+  auto write_wgrad = [&](int off_w, int off_b, floatx4 (&cacc)[KW], float dbias, bool first) {
+    // cacc[d]: rows co = 4g + r, column = ci (or d for the first layer)
+    const int col = lane & 15;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int co = 4 * g + r;
+      if (first) {
+        if (col < KW) out[off_w + co * KW + col] = cacc[0][r];
+      } else {
+#pragma unroll
+        for (int d = 0; d < KW; ++d) out[off_w + (co * CH + col) * KW + d] = cacc[d][r];
+      }
+    }
+    // bias: dbias holds this lane's share for channel `col` (A rows are co = col)
+    float b = dbias;
+    b += __shfl_xor(b, 16, 64);
+    b += __shfl_xor(b, 32, 64);
+    if (g == 0) out[off_b + col] = b;
+  };
+
+  // layer 3 weight gradient (dz = bufA, h = bufB)
+  {
+    floatx4 cacc[KW];
+#pragma unroll
+    for (int d = 0; d < KW; ++d) cacc[d] = floatx4{0.f, 0.f, 0.f, 0.f};
+    float db = 0.0f;
+    weight_grad(bufA, bufB, st, n, false, cacc, db);
+    write_wgrad(OFF_W3, OFF_B3, cacc, db, false);
+  }
+  __syncthreads();
+  // d2 = conv^T(d3) * gelu'(z2) -> bufC ; h1 = gelu(z1) -> bufB
+  {
+    bf16x8 a[3];
+    load_conv_weights(P + OFF_W3, true, a);
+    for (int pos = 0; pos < n; ++pos) {
+      floatx4 acc = conv_at(a, bufA, n, pos);
+      floatx4 z = load4_f32(z2, n, pos);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) acc[r] *= gelu_grad_f(z[r]);
+      store4(bufC, n, pos, acc);
+    }
+  }
+  __syncthreads();
+  for (int pos = 0; pos < n; ++pos) {
+    floatx4 z = load4_f32(z1, n, pos), h;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) h[r] = gelu_f(z[r]);
+    store4(bufB, n, pos, h);
+  }
+  __syncthreads();
+  // layer 2 weight gradient (dz = bufC, h = bufB)
+  {
+    floatx4 cacc[KW];
+#pragma unroll
+    for (int d = 0; d < KW; ++d) cacc[d] = floatx4{0.f, 0.f, 0.f, 0.f};
+    float db = 0.0f;
+    weight_grad(bufC, bufB, st, n, false, cacc, db);
+    write_wgrad(OFF_W2, OFF_B2, cacc, db, false);
+  }
+  __syncthreads();
+  // d1 = conv^T(d2) * gelu'(z1) -> bufA
+  {
+    bf16x8 a[3];
+    load_conv_weights(P + OFF_W2, true, a);
+    for (int pos = 0; pos < n; ++pos) {
+      floatx4 acc = conv_at(a, bufC, n, pos);
+      floatx4 z = load4_f32(z1, n, pos);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) acc[r] *= gelu_grad_f(z[r]);
+      store4(bufA, n, pos, acc);
+    }
+  }
+  __syncthreads();
+  // layer 1 weight gradient (dz = bufA, input spins)
+  {
+    floatx4 cacc[KW];
+#pragma unroll
+    for (int d = 0; d < KW; ++d) cacc[d] = floatx4{0.f, 0.f, 0.f, 0.f};
+    float db = 0.0f;
+    weight_grad(bufA, nullptr, st, n, true, cacc, db);
+    write_wgrad(OFF_W1, OFF_B1, cacc, db, true);
+  }
+}
+
+// grad[p] = sum_w partial[w][p] (fixed order)
+__global__ void reduce_grad_kernel(const float *__restrict__ partial, int nwaves,
+                                   float *__restrict__ grad) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= NPARAMS) return;
+  float s = 0.0f;
+  for (int w = 0; w < nwaves; ++w) s += partial[static_cast<size_t>(w) * NPARAMS + p];
+  grad[p] = s;
+}
+
+// Adam (Kingma & Ba 2015) on the FP32 master parameters.
+__global__ void adam_kernel(float *__restrict__ p, const float *__restrict__ g,
+                            float *__restrict__ m, float *__restrict__ v, float lr, float b1,
+                            float b2, float eps, float c1, float c2) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= NPARAMS) return;
+  const float gi = g[i];
+  const float mi = b1 * m[i] + (1.0f - b1) * gi;
+  const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+  m[i] = mi;
+  v[i] = vi;
+  p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+}
+
+// start vector c_r = marshall(r) * sign(t_r) * exp(a_r - amax) / N_r
+__global__ void start_vector_kernel(const float *__restrict__ a, const float *__restrict__ t,
+                                    const uint64_t *__restrict__ reps,
+                                    const uint32_t *__restrict__ stab, uint64_t D, float amax,
+                                    uint64_t marshall_mask, double inv_group, double *__restrict__ v) {
+  for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < D;
+       i += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+    double amp = exp(static_cast<double>(a[i] - amax));
+    if (t[i] < 0.0f) amp = -amp;
+    if (__popcll(reps[i] & marshall_mask) & 1) amp = -amp;
+    v[i] = amp / sqrt(static_cast<double>(stab[i]) * inv_group);
+  }
+}
+
+__global__ void max_partial_kernel(const float *__restrict__ a, uint64_t n, float *__restrict__ out) {
+  __shared__ float lds[256];
+  float m = -3.0e38f;
+  for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < n;
+       i += static_cast<uint64_t>(gridDim.x) * blockDim.x)
+    m = fmaxf(m, a[i]);
+  lds[threadIdx.x] = m;
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if (threadIdx.x < static_cast<unsigned>(off)) lds[threadIdx.x] = fmaxf(lds[threadIdx.x], lds[threadIdx.x + off]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[blockIdx.x] = lds[0];
+}
+
+// One MFMA with known operands, for the lane-layout check.
+__global__ void mfma_probe_kernel(float *out) {
+  const int l = threadIdx.x, row = l & 15, g = l >> 4;
+  bf16x8 a, b;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    const int k = 8 * g + j;
+    a[j] = static_cast<__bf16>(static_cast<float>((row * 3 + k) % 7 - 3));     // A[row][k]
+    b[j] = static_cast<__bf16>(static_cast<float>((k * 3 + row * 7) % 5 - 2)); // B[k][col=row]
+  }
+  floatx4 c = {0.f, 0.f, 0.f, 0.f};
+  c = mfma(a, b, c);
+#pragma unroll
+  for (int r = 0; r < 4; ++r) out[(4 * g + r) * 16 + row] = c[r]; // D[4g+r][col]
+}
+
+size_t lds_bytes_forward(int n) { return static_cast<size_t>(2) * 16 * n * CH * 2; }
+size_t lds_bytes_backward(int n) { return static_cast<size_t>(3) * 16 * n * CH * 2; }
+
+} // namespace
+
+struct CnnModel {
+  DeviceBuffer<float> params, m, v, grad;
+  int64_t step = 0;
+};
+
+} // namespace spined
+
+using spined::CnnModel;
+using spined::DeviceBuffer;
+using spined::Error;
+
+struct spined_cnn {
+  CnnModel m;
+};
+
+extern "C" {
+
+int spined_cnn_num_params(void) { return spined::NPARAMS; }
+
+int spined_cnn_create(spined_cnn **out, const float *init_params) {
+  return spined::guarded([&] {
+    if (!out || !init_params) throw Error(LS_INVALID_ARGUMENT, "NULL argument");
+    auto *c = new spined_cnn();
+    c->m.params.allocate(spined::NPARAMS);
+    c->m.m.allocate(spined::NPARAMS);
+    c->m.v.allocate(spined::NPARAMS);
+    c->m.grad.allocate(spined::NPARAMS);
+    SPINED_HIP(hipMemcpy(c->m.params.ptr, init_params, 4 * spined::NPARAMS, hipMemcpyHostToDevice));
+    SPINED_HIP(hipMemset(c->m.m.ptr, 0, 4 * spined::NPARAMS));
+    SPINED_HIP(hipMemset(c->m.v.ptr, 0, 4 * spined::NPARAMS));
+    *out = c;
+  });
+}
+
+void spined_cnn_destroy(spined_cnn *c) { delete c; }
+
+int spined_cnn_get_params(const spined_cnn *c, float *out) {
+  return spined::guarded([&] {
+    if (!c || !out) throw Error(LS_INVALID_ARGUMENT, "NULL argument");
+    SPINED_HIP(hipMemcpy(out, c->m.params.ptr, 4 * spined::NPARAMS, hipMemcpyDeviceToHost));
+  });
+}
+
+int spined_cnn_set_params(spined_cnn *c, const float *in) {
+  return spined::guarded([&] {
+    if (!c || !in) throw Error(LS_INVALID_ARGUMENT, "NULL argument");
+    SPINED_HIP(hipMemcpy(c->m.params.ptr, in, 4 * spined::NPARAMS, hipMemcpyHostToDevice));
+  });
+}
+
+// Forward on device buffers: states (count), out_a / out_t (count floats);
+// z_store (3 * ceil16(count) * n * 16 floats) may be NULL.
+int spined_cnn_forward_device(const spined_cnn *c, int n, uint64_t count, const uint64_t *states,
+                              float *out_a, float *out_t, float *z_store) {
+  return spined::guarded([&] {
+    if (!c || n < 5 || n > spined::kMaxN) throw Error(LS_INVALID_ARGUMENT, "n must be in [5, 64]");
+    if (count == 0) return;
+    spined::CnnArgs a{c->m.params.ptr, n, count};
+    const unsigned blocks = static_cast<unsigned>((count + 15) / 16);
+    hipLaunchKernelGGL(spined::cnn_forward_kernel, dim3(blocks), dim3(64),
+                       spined::lds_bytes_forward(n), spined::stream(), a, states, out_a, out_t,
+                       z_store);
+    SPINED_CHECK_LAUNCH();
+  });
+}
+
+// Gradient (device buffer grad_out, NPARAMS floats) of sum ga*a + gt*t over
+// `count` samples whose pre-activations were stored by a forward call.
+int spined_cnn_backward_device(spined_cnn *c, int n, uint64_t count, const uint64_t *states,
+                               const float *z_store, const float *ga, const float *gt,
+                               float *grad_out) {
+  return spined::guarded([&] {
+    if (!c || n < 5 || n > spined::kMaxN) throw Error(LS_INVALID_ARGUMENT, "n must be in [5, 64]");
+    if (count == 0) return;
+    spined::CnnArgs a{c->m.params.ptr, n, count};
+    const unsigned blocks = static_cast<unsigned>((count + 15) / 16);
+    DeviceBuffer<float> partial(static_cast<size_t>(blocks) * spined::NPARAMS);
+    hipStream_t s = spined::stream();
+    hipLaunchKernelGGL(spined::cnn_backward_kernel, dim3(blocks), dim3(64),
+                       spined::lds_bytes_backward(n), s, a, states, z_store, ga, gt, partial.ptr);
+    SPINED_CHECK_LAUNCH();
+    float *dst = grad_out ? grad_out : c->m.grad.ptr;
+    hipLaunchKernelGGL(spined::reduce_grad_kernel, dim3((spined::NPARAMS + 255) / 256), dim3(256),
+                       0, s, partial.ptr, static_cast<int>(blocks), dst);
+    SPINED_CHECK_LAUNCH();
+    SPINED_HIP(hipStreamSynchronize(s)); // `partial` is released at scope exit
+  });
+}
+
+// One Adam step with the gradient left in the model by backward (grad_out = NULL).
+int spined_cnn_adam_step(spined_cnn *c, float lr) {
+  return spined::guarded([&] {
+    if (!c) throw Error(LS_INVALID_ARGUMENT, "NULL model");
+    const float b1 = 0.9f, b2 = 0.999f;
+    c->m.step += 1;
+    const float c1 = 1.0f - std::pow(b1, static_cast<float>(c->m.step));
+    const float c2 = 1.0f - std::pow(b2, static_cast<float>(c->m.step));
+    hipLaunchKernelGGL(spined::adam_kernel, dim3((spined::NPARAMS + 255) / 256), dim3(256), 0,
+                       spined::stream(), c->m.params.ptr, c->m.grad.ptr, c->m.m.ptr, c->m.v.ptr,
+                       lr, b1, b2, 1e-8f, c1, c2);
+    SPINED_CHECK_LAUNCH();
+  });
+}
+
+} // extern "C"
+
+extern "C" {
+
+// Learned start vector over the whole basis (device, FP64, length D):
+// c_r = marshall(r) sign(t_r) exp(a_r - max a) / N_r.
+int spined_cnn_start_vector(const spined_cnn *c, const ls_spin_basis *basis,
+                            uint64_t marshall_mask, double *v_device) {
+  return spined::guarded([&] {
+    if (!c || !basis || !v_device) throw Error(LS_INVALID_ARGUMENT, "NULL argument");
+    spined::Basis &b = *basis->p;
+    if (!b.built) throw Error(LS_CACHE_NOT_BUILT, "basis has not been built");
+    spined::cnn_start_vector(c, b, marshall_mask, v_device);
+  });
+}
+
+int spined_debug_mfma_probe(float *out_host) {
+  return spined::guarded([&] {
+    DeviceBuffer<float> d(256);
+    hipLaunchKernelGGL(spined::mfma_probe_kernel, dim3(1), dim3(64), 0, spined::stream(), d.ptr);
+    SPINED_CHECK_LAUNCH();
+    SPINED_HIP(hipMemcpyAsync(out_host, d.ptr, 1024, hipMemcpyDeviceToHost, spined::stream()));
+    SPINED_HIP(hipStreamSynchronize(spined::stream()));
+  });
+}
+
+} // extern "C"
+
+namespace spined {
+// launched from api.cpp (needs the basis internals)
+void cnn_start_vector(const spined_cnn *c, const Basis &b, uint64_t marshall_mask, double *v) {
+  hipStream_t s = stream();
+  const uint64_t D = b.D;
+  if (D == 0) return;
+  DeviceBuffer<float> a(D), t(D);
+  CnnArgs args{c->m.params.ptr, b.n, D};
+  const unsigned blocks = static_cast<unsigned>((D + 15) / 16);
+  hipLaunchKernelGGL(cnn_forward_kernel, dim3(blocks), dim3(64), lds_bytes_forward(b.n), s, args,
+                     b.reps.ptr, a.ptr, t.ptr, static_cast<float *>(nullptr));
+  SPINED_CHECK_LAUNCH();
+  const unsigned mb = 1024;
+  DeviceBuffer<float> part(mb);
+  max_partial_kernel<<<mb, 256, 0, s>>>(a.ptr, D, part.ptr);
+  SPINED_CHECK_LAUNCH();
+  std::vector<float> hp(mb);
+  SPINED_HIP(hipMemcpyAsync(hp.data(), part.ptr, 4 * mb, hipMemcpyDeviceToHost, s));
+  SPINED_HIP(hipStreamSynchronize(s));
+  const float amax = *std::max_element(hp.begin(), hp.end());
+  const double inv_group = 1.0 / static_cast<double>(b.tables.nperm * (b.tables.has_flip ? 2 : 1));
+  start_vector_kernel<<<4096, 256, 0, s>>>(a.ptr, t.ptr, b.reps.ptr, b.stab.ptr, D, amax,
+                                           marshall_mask, inv_group, v);
+  SPINED_CHECK_LAUNCH();
+  SPINED_HIP(hipStreamSynchronize(s));
+}
+} // namespace spined

@@ -165,4 +165,7 @@ int eigsh(Operator &op, int dtype, uint64_t k, double tol, const spined_eigsh_op
 // ---- misc kernels (solver.hip) ----------------------------------------------
 void launch_random_fill(double *v, uint64_t n, uint64_t seed, hipStream_t s);
 
+// ---- learned start (cnn.hip) ------------------------------------------------
+void cnn_start_vector(const spined_cnn *c, const Basis &b, uint64_t marshall_mask, double *v);
+
 } // namespace spined

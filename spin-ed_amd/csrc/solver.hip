@@ -352,6 +352,7 @@ template <class VT> struct Phase {
   double matvec_seconds = 0.0;
   unsigned G;
   bool small;
+  int mlast = 0; // size of the last projected problem (Ritz assembly)
 
   Phase(Operator &o, uint64_t n_, int m_, int k_, int keep_, hipStream_t s_)
       : op(o), n(n_), m(m_), k(k_), keep(keep_), s(s_) {
@@ -478,25 +479,48 @@ template <class VT> struct Phase {
     double best = 1e300;
     int stalled = 0;
     std::vector<double> th(m), Sh(static_cast<size_t>(m) * m);
-    for (;;) {
-      for (int j = j0; j < m; ++j) step(j, seed);
-      jacobi_eigh_kernel<<<1, 256, 0, s>>>(T.ptr, kMaxBasis, m, theta.ptr, S.ptr, 40);
+    // Convergence is also checked inside a cycle (on the leading block of T)
+    // so that a good start vector is not held back to a restart boundary:
+    // every step when a matvec is expensive, every 4th otherwise.
+    const int check_every = n >= (1u << 20) ? 1 : 4;
+    // solve the leading msz x msz projected problem; estimates of the k lowest
+    auto project = [&](int msz, double &worst, int &nconv) {
+      jacobi_eigh_kernel<<<1, 256, 0, s>>>(T.ptr, kMaxBasis, msz, theta.ptr, S.ptr, 40);
       SPINED_CHECK_LAUNCH();
       double bm = 0.0;
-      SPINED_HIP(hipMemcpyAsync(th.data(), theta.ptr, 8 * m, hipMemcpyDeviceToHost, s));
-      SPINED_HIP(hipMemcpyAsync(Sh.data(), S.ptr, 8ull * m * m, hipMemcpyDeviceToHost, s));
-      SPINED_HIP(hipMemcpyAsync(&bm, beta.ptr + (m - 1), 8, hipMemcpyDeviceToHost, s));
-      harvest_events();
-      ++res.cycles;
+      SPINED_HIP(hipMemcpyAsync(th.data(), theta.ptr, 8 * msz, hipMemcpyDeviceToHost, s));
+      SPINED_HIP(hipMemcpyAsync(Sh.data(), S.ptr, 8ull * msz * msz, hipMemcpyDeviceToHost, s));
+      SPINED_HIP(hipMemcpyAsync(&bm, beta.ptr + (msz - 1), 8, hipMemcpyDeviceToHost, s));
+      SPINED_HIP(hipStreamSynchronize(s));
       res.theta.assign(th.begin(), th.begin() + k);
       res.est.resize(k);
-      double worst = 0.0;
-      int nconv = 0;
+      worst = 0.0;
+      nconv = 0;
       for (int i = 0; i < k; ++i) {
-        res.est[i] = std::fabs(bm * Sh[static_cast<size_t>(i) * m + (m - 1)]);
+        res.est[i] = std::fabs(bm * Sh[static_cast<size_t>(i) * msz + (msz - 1)]);
         worst = std::max(worst, res.est[i]);
         if (res.est[i] <= tol) ++nconv;
       }
+      mlast = msz;
+    };
+    for (;;) {
+      double worst = 0.0;
+      int nconv = 0;
+      bool early = false;
+      for (int j = j0; j < m; ++j) {
+        step(j, seed);
+        const int msz = j + 1;
+        if (msz < m && msz > k && (msz - j0) % check_every == 0) {
+          project(msz, worst, nconv);
+          if (nconv == k) {
+            early = true;
+            break;
+          }
+        }
+      }
+      if (!early) project(m, worst, nconv);
+      harvest_events();
+      ++res.cycles;
       if (verbose)
         log_line("phase %d cycle %d matvecs %lld theta0 %.15g est %.3e", phase_id, res.cycles,
                  static_cast<long long>(matvecs), th[0], worst);
@@ -531,9 +555,10 @@ template <class VT> struct Phase {
     }
   }
 
-  // FP64 Ritz vector i into out
+  // FP64 Ritz vector i of the last projected problem into out
   void ritz(int i, double *out) {
-    ritz_kernel<VT><<<G, kBlock, 0, s>>>(vptr.ptr, m, S.ptr + static_cast<size_t>(i) * m, out, n);
+    ritz_kernel<VT><<<G, kBlock, 0, s>>>(vptr.ptr, mlast, S.ptr + static_cast<size_t>(i) * mlast,
+                                         out, n);
     SPINED_CHECK_LAUNCH();
   }
 };
@@ -583,7 +608,9 @@ int eigsh(Operator &op, int dtype, uint64_t k, double tol, const spined_eigsh_op
     if (!opts.start_vector) throw Error(LS_INVALID_ARGUMENT, "start_vector is NULL");
     SPINED_HIP(hipMemcpyAsync(start.ptr, opts.start_vector, n * 8, hipMemcpyHostToDevice, s));
   } else if (opts.start == SPINED_START_LEARNED) {
-    throw Error(LS_NOT_IMPLEMENTED, "learned start vectors are evaluated by spined_cnn_start");
+    if (!opts.model) throw Error(LS_INVALID_ARGUMENT, "learned start needs a model");
+    cnn_start_vector(static_cast<const spined_cnn *>(opts.model), b, opts.start_sign_mask,
+                     start.ptr);
   } else {
     launch_random_fill(start.ptr, n, opts.seed, s);
   }

@@ -34,7 +34,11 @@ SPINED_SYMBOLS = (
     "spined_operator_count_offdiag", "spined_operator_orbit_method",
     "spined_operator_set_orbit_method", "spined_basis_set_bucket_bits",
     "spined_basis_get_bucket_bits", "spined_basis_sector_dimension", "spined_debug_orbit",
-    "spined_debug_ring_available",
+    "spined_debug_ring_available", "spined_device_synchronize", "spined_device_malloc",
+    "spined_device_free", "spined_memcpy", "spined_memset", "spined_cnn_num_params",
+    "spined_cnn_create", "spined_cnn_destroy", "spined_cnn_get_params", "spined_cnn_set_params",
+    "spined_cnn_forward_device", "spined_cnn_backward_device", "spined_cnn_adam_step",
+    "spined_cnn_start_vector", "spined_debug_mfma_probe",
 )
 
 
@@ -50,6 +54,7 @@ class SpinedEigshOptions(C.Structure):
         ("max_matvecs", C.c_int64),
         ("start_vector", C.c_void_p),
         ("model", C.c_void_p),
+        ("start_sign_mask", C.c_uint64),
     ]
 
 
@@ -141,6 +146,21 @@ def load():
         "spined_basis_sector_dimension": (I, [P, C.POINTER(D)]),
         "spined_debug_orbit": (I, [P, I, U64, P, P, P, P, P]),
         "spined_debug_ring_available": (I, [P]),
+        "spined_device_synchronize": (I, []),
+        "spined_device_malloc": (I, [PP, U64]),
+        "spined_device_free": (I, [P]),
+        "spined_memcpy": (I, [P, P, U64, I]),
+        "spined_memset": (I, [P, I, U64]),
+        "spined_cnn_num_params": (I, []),
+        "spined_cnn_create": (I, [PP, P]),
+        "spined_cnn_destroy": (None, [P]),
+        "spined_cnn_get_params": (I, [P, P]),
+        "spined_cnn_set_params": (I, [P, P]),
+        "spined_cnn_forward_device": (I, [P, I, U64, P, P, P, P]),
+        "spined_cnn_backward_device": (I, [P, I, U64, P, P, P, P, P]),
+        "spined_cnn_adam_step": (I, [P, C.c_float]),
+        "spined_cnn_start_vector": (I, [P, P, U64, P]),
+        "spined_debug_mfma_probe": (I, [P]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

@@ -28,7 +28,8 @@ def eigsh(op: Operator_, k: int = 1, tol: float = 1e-10, *, precision: str = "fp
           start: str = "random", seed: int = 0x5EED, start_vector: Optional[np.ndarray] = None,
           max_basis_size: int = 0, min_restart_size: int = 0, max_block_size: int = 0,
           max_matvecs: int = 0, want_vectors: bool = True, dtype=np.float64,
-          verbose: bool = False, monitor: Optional[Callable[[dict], bool]] = None) -> EigshResult:
+          verbose: bool = False, monitor: Optional[Callable[[dict], bool]] = None,
+          model=None, sign_mask: int = 0) -> EigshResult:
     L = _lib.load()
     D = get_number_states(op.basis)
     opts = _lib.SpinedEigshOptions()
@@ -42,6 +43,10 @@ def eigsh(op: Operator_, k: int = 1, tol: float = 1e-10, *, precision: str = "fp
     opts.max_matvecs = max_matvecs
     opts.verbose = int(verbose)
     sv = None
+    if model is not None:
+        opts.start = START["learned"]
+        opts.model = model.handle.value
+        opts.start_sign_mask = sign_mask
     if start_vector is not None:
         sv = np.ascontiguousarray(start_vector, dtype=np.float64)
         if sv.shape != (D,):

@@ -75,7 +75,7 @@ def test_matvec_fp64(spec):
 
 
 def test_matvec_fp32_and_mixed():
-    import torch
+    from spined.device import DeviceArray
     spec = ring(24)
     basis, op = engine(spec)
     o = oracle(spec)
@@ -85,12 +85,11 @@ def test_matvec_fp32_and_mixed():
     y32 = spined.apply(op, x)
     assert y32.dtype == np.float32
     assert np.max(np.abs(y32 - yr)) <= 1e-6 * np.max(np.abs(yr))
-    xd = torch.from_numpy(x).cuda()
-    yd = torch.zeros(D, dtype=torch.float64, device="cuda")
-    torch.cuda.synchronize()
+    xd = DeviceArray.from_numpy(x)
+    yd = DeviceArray.zeros(D, np.float64)
     spined.matvec_device(op, xd, yd, 0, y_is_fp64=True)
     spined.synchronize()
-    ym = yd.cpu().numpy()
+    ym = yd.numpy()
     # FP32 storage, FP64 accumulation: identical to the FP64 oracle on the same x
     assert np.max(np.abs(ym - yr)) <= 1e-13 * np.max(np.abs(yr))
 

@@ -55,6 +55,10 @@ def parse():
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--traffic-json", default=None,
                    help="JSON with measured FETCH/WRITE bytes per matvec launch")
+    p.add_argument("--copy-calibration-bytes", type=int, default=0,
+                   help="run the plain copy kernel once on this many bytes (counter calibration)")
+    p.add_argument("--matvecs-only", type=int, default=0,
+                   help="profiling mode: run this many FP64 matvecs instead of solves")
     return p.parse_args()
 
 
@@ -92,11 +96,29 @@ def main():
     e_off = spined.count_offdiag(op)
     log(f"off-diagonal gathers per matvec: {e_off}")
 
+    if a.copy_calibration_bytes:
+        src_b = DeviceArray.zeros(a.copy_calibration_bytes // 8, np.float64)
+        dst_b = DeviceArray(a.copy_calibration_bytes // 8, np.float64)
+        spined.check_status(spined._lib.load().spined_debug_copy_device(
+            dst_b.ptr, src_b.ptr, a.copy_calibration_bytes))
+        src_b.free()
+        dst_b.free()
+
     # one untimed matvec warms the kernels even when warmup = 0
     x = DeviceArray.from_numpy(np.ones(D))
     y = DeviceArray(D, np.float64)
     spined.matvec_device(op, x, y, 1)
     spined.synchronize()
+    if a.matvecs_only:
+        t = time.perf_counter()
+        for _ in range(a.matvecs_only):
+            spined.matvec_device(op, x, y, 1)
+        spined.synchronize()
+        dt = (time.perf_counter() - t) / a.matvecs_only
+        print(json.dumps({"mode": "matvecs-only", "config": desc, "dimension": D,
+                          "matvecs": a.matvecs_only, "ms_per_matvec": dt * 1e3,
+                          "offdiag_gathers_per_launch": e_off}))
+        return 0
     x.free()
     y.free()
 

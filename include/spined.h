@@ -164,6 +164,9 @@ int spined_debug_orbit(const ls_spin_basis *basis, int method, uint64_t count,
                        uint8_t *valid);
 /* 1 if the basis' group is a ring (rotation/reflection) group. */
 int spined_debug_ring_available(const ls_spin_basis *basis);
+/* Plain device copy of `bytes` (multiple of 16): the calibration kernel for
+ * HBM byte counters in profiles. */
+int spined_debug_copy_device(void *dst, const void *src, uint64_t bytes);
 
 #ifdef __cplusplus
 }

@@ -585,6 +585,14 @@ int spined_debug_orbit(const ls_spin_basis *basis, int method, uint64_t count,
   });
 }
 
+int spined_debug_copy_device(void *dst, const void *src, uint64_t bytes) {
+  return guarded([&] {
+    if (!dst || !src || bytes % 16) throw Error(LS_INVALID_ARGUMENT, "bytes must be a multiple of 16");
+    launch_copy_calibration(src, dst, bytes, stream());
+    SPINED_HIP(hipStreamSynchronize(stream()));
+  });
+}
+
 int spined_debug_ring_available(const ls_spin_basis *basis) {
   return basis && basis->p->tables.ring_available ? 1 : 0;
 }

@@ -166,21 +166,30 @@ __global__ void __launch_bounds__(kBlock)
   stab[i] = static_cast<uint32_t>(r.count);
 }
 
+// buckets[b] = lower_bound(reps, b << shift) for b < nbuckets, D at b =
+// nbuckets: one thread per bucket (the top levels of the searches are shared
+// and stay in cache).
 __global__ void __launch_bounds__(kBlock)
-    bucket_kernel(const uint64_t *__restrict__ reps, uint64_t D, int shift,
+    bucket_kernel(const uint64_t *__restrict__ reps, uint64_t D, int shift, uint64_t nbuckets,
                   uint32_t *__restrict__ buckets) {
-  const uint64_t i = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
-  if (i >= D) return;
-  const int64_t bi = static_cast<int64_t>(reps[i] >> shift);
-  const int64_t bp = i ? static_cast<int64_t>(reps[i - 1] >> shift) : -1;
-  for (int64_t b = bp + 1; b <= bi; ++b) buckets[b] = static_cast<uint32_t>(i);
-}
-
-__global__ void __launch_bounds__(kBlock)
-    bucket_tail_kernel(uint64_t first, uint64_t last_inclusive, uint32_t value,
-                       uint32_t *__restrict__ buckets) {
-  const uint64_t b = first + static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
-  if (b <= last_inclusive) buckets[b] = value;
+  const uint64_t b = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
+  if (b > nbuckets) return;
+  if (b == nbuckets) {
+    buckets[b] = static_cast<uint32_t>(D);
+    return;
+  }
+  const uint64_t key = b << shift;
+  uint64_t lo = 0, n = D;
+  while (n > 0) {
+    const uint64_t half = n >> 1;
+    if (reps[lo + half] < key) {
+      lo += half + 1;
+      n -= half + 1;
+    } else {
+      n = half;
+    }
+  }
+  buckets[b] = static_cast<uint32_t>(lo);
 }
 
 uint64_t binomial(int n, int k) {
@@ -230,14 +239,8 @@ void build_index(Basis &b) {
   b.bucket_bits = bits;
   b.nbuckets = (last >> b.shift) + 1;
   b.buckets.allocate(b.nbuckets + 1);
-  if (b.D) {
-    bucket_kernel<<<grid_for(b.D), kBlock, 0, s>>>(b.reps.ptr, b.D, b.shift, b.buckets.ptr);
-    SPINED_CHECK_LAUNCH();
-  }
-  const uint64_t first_tail = b.D ? (last >> b.shift) + 1 : 0;
-  const uint64_t ntail = b.nbuckets + 1 - first_tail;
-  bucket_tail_kernel<<<grid_for(ntail), kBlock, 0, s>>>(first_tail, b.nbuckets,
-                                                        static_cast<uint32_t>(b.D), b.buckets.ptr);
+  bucket_kernel<<<grid_for(b.nbuckets + 1), kBlock, 0, s>>>(b.reps.ptr, b.D, b.shift, b.nbuckets,
+                                                           b.buckets.ptr);
   SPINED_CHECK_LAUNCH();
   SPINED_HIP(hipStreamSynchronize(s));
 }

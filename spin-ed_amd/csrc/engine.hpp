@@ -62,6 +62,13 @@ struct OrbitTables {
   std::vector<int32_t> h_phases, h_ring_phases;
   int ring_has_rev = 0;
   uint32_t total_terms = 0;
+  // sqrt(c / s) for stabiliser sizes c, s (divisors of |G|): ratio[i * nd + j]
+  // with i = div_idx[c], j = div_idx[s]; nd = 0 when |G| is too large.
+  int nd = 0;
+  std::vector<uint16_t> h_div_idx;
+  std::vector<double> h_ratio;
+  DeviceBuffer<uint16_t> div_idx;
+  DeviceBuffer<double> ratio;
   bool on_device = false;
 
   GenericOrbitArgs generic_args(bool device) const;
@@ -126,6 +133,14 @@ struct OperatorTables {
   DeviceBuffer<uint32_t> tuple_sites, tuple_n, tuple_off, range;
   DeviceBuffer<double> diag_re, diag_im, coef_re, coef_im;
   DeviceBuffer<uint64_t> flip;
+  // exchange-bond form (see compile_operator): one record per tuple
+  bool bonds = false;
+  std::vector<uint32_t> h_bond_sites;        // s0 | s1 << 8
+  std::vector<double> h_bond_dp, h_bond_da, h_bond_o;
+  std::vector<uint64_t> h_bond_flip;
+  DeviceBuffer<uint32_t> bond_sites;
+  DeviceBuffer<double> bond_dp, bond_da, bond_o;
+  DeviceBuffer<uint64_t> bond_flip;
   bool on_device = false;
 };
 
@@ -164,6 +179,7 @@ int eigsh(Operator &op, int dtype, uint64_t k, double tol, const spined_eigsh_op
 
 // ---- misc kernels (solver.hip) ----------------------------------------------
 void launch_random_fill(double *v, uint64_t n, uint64_t seed, hipStream_t s);
+void launch_copy_calibration(const void *src, void *dst, uint64_t bytes, hipStream_t s);
 
 // ---- learned start (cnn.hip) ------------------------------------------------
 void cnn_start_vector(const spined_cnn *c, const Basis &b, uint64_t marshall_mask, double *v);

@@ -3,6 +3,7 @@
 // ls_create_spin_basis / ls_create_interactionN / ls_create_operator do
 // behind src/SpinED/Internal.hs:69-402.
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <map>
 #include <numeric>
@@ -131,6 +132,7 @@ RingOrbitArgs OrbitTables::ring_args(bool device) const {
   a.has_flip = has_flip;
   a.flip_phase = flip_phase;
   a.L = L;
+  a.trivial = trivial_chars ? 1 : 0;
   return a;
 }
 
@@ -225,6 +227,26 @@ void build_orbit_tables(const Group &g, int n, int spin_inversion, OrbitTables &
     }
   }
 
+  // stabiliser sizes divide |G|: tabulate sqrt(c / s) over its divisors,
+  // computed exactly as the row gather would (IEEE division and sqrt)
+  const int64_t G = static_cast<int64_t>(t.nperm) * (t.has_flip ? 2 : 1);
+  t.nd = 0;
+  t.h_div_idx.clear();
+  t.h_ratio.clear();
+  if (G <= 8192) {
+    std::vector<int64_t> divs;
+    for (int64_t d = 1; d <= G; ++d)
+      if (G % d == 0) divs.push_back(d);
+    t.nd = static_cast<int>(divs.size());
+    t.h_div_idx.assign(G + 1, 0);
+    for (int i = 0; i < t.nd; ++i) t.h_div_idx[divs[i]] = static_cast<uint16_t>(i);
+    t.h_ratio.resize(static_cast<size_t>(t.nd) * t.nd);
+    for (int i = 0; i < t.nd; ++i)
+      for (int j = 0; j < t.nd; ++j)
+        t.h_ratio[static_cast<size_t>(i) * t.nd + j] =
+            std::sqrt(static_cast<double>(divs[i]) / static_cast<double>(divs[j]));
+  }
+
 }
 
 void ensure_device(OrbitTables &t) {
@@ -235,6 +257,8 @@ void ensure_device(OrbitTables &t) {
   upload(t.flags, t.h_flags);
   upload(t.phases, t.h_phases);
   upload(t.ring_phases, t.h_ring_phases);
+  upload(t.div_idx, t.h_div_idx);
+  upload(t.ratio, t.h_ratio);
   t.on_device = true;
 }
 
@@ -353,6 +377,52 @@ void compile_operator(Operator &op) {
   t.ntemplates = static_cast<int>(t.h_flip.size());
   // the range table holds 2^n + 1 entries per tuple: the entries of tuple i
   // start at tuple_off[i] + i.
+
+  // Exchange-bond form: every tuple is a 2-site real matrix with
+  // M00 = M33 (parallel), M11 = M22 (antiparallel), M12 = M21 (swap) and no
+  // other off-diagonal entry -- Heisenberg and XXZ bonds of any strength.
+  // Then the diagonal is one select per bond and the (single) active
+  // template of an antiparallel bond is the bond itself, in tuple order.
+  t.bonds = t.is_real && t.ntuples > 0;
+  t.h_bond_sites.clear();
+  t.h_bond_dp.clear();
+  t.h_bond_da.clear();
+  t.h_bond_o.clear();
+  t.h_bond_flip.clear();
+  for (const auto &term : op.terms) {
+    if (!t.bonds) break;
+    if (term->n != 2 || !term->is_real) {
+      t.bonds = false;
+      break;
+    }
+    const auto &M = term->matrix;
+    auto at = [&](int r, int c) { return M[r * 4 + c].real(); };
+    bool ok = at(0, 0) == at(3, 3) && at(1, 1) == at(2, 2) && at(1, 2) == at(2, 1);
+    for (int r = 0; r < 4 && ok; ++r)
+      for (int c = 0; c < 4 && ok; ++c) {
+        if (r == c) continue;
+        if ((r == 1 && c == 2) || (r == 2 && c == 1)) continue;
+        if (at(r, c) != 0.0) ok = false;
+      }
+    if (!ok) {
+      t.bonds = false;
+      break;
+    }
+    for (const auto &s : term->sites) {
+      t.h_bond_sites.push_back(static_cast<uint32_t>(s[0]) | (static_cast<uint32_t>(s[1]) << 8));
+      t.h_bond_dp.push_back(at(0, 0));
+      t.h_bond_da.push_back(at(1, 1));
+      t.h_bond_o.push_back(at(1, 2));
+      t.h_bond_flip.push_back((1ull << s[0]) | (1ull << s[1]));
+    }
+  }
+  if (!t.bonds) {
+    t.h_bond_sites.clear();
+    t.h_bond_dp.clear();
+    t.h_bond_da.clear();
+    t.h_bond_o.clear();
+    t.h_bond_flip.clear();
+  }
 }
 
 void ensure_device(OperatorTables &t) {
@@ -366,6 +436,11 @@ void ensure_device(OperatorTables &t) {
   upload(t.flip, t.h_flip);
   upload(t.coef_re, t.h_coef_re);
   upload(t.coef_im, t.h_coef_im);
+  upload(t.bond_sites, t.h_bond_sites);
+  upload(t.bond_dp, t.h_bond_dp);
+  upload(t.bond_da, t.h_bond_da);
+  upload(t.bond_o, t.h_bond_o);
+  upload(t.bond_flip, t.h_bond_flip);
   t.on_device = true;
 }
 

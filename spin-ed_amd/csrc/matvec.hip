@@ -118,12 +118,30 @@ template <int NW> __device__ inline bool any_set(const uint64_t (&act)[NW]) {
   return o != 0;
 }
 
+// Exchange bonds (Heisenberg / XXZ): diagonal dp (parallel) or da
+// (antiparallel) per bond, one off-diagonal template per antiparallel bond.
+struct BondArgs {
+  const uint32_t *__restrict__ sites; // s0 | s1 << 8
+  const double *__restrict__ dp;
+  const double *__restrict__ da;
+  const double *__restrict__ o;
+  const uint64_t *__restrict__ flip;
+  int32_t nbonds;
+};
+
+// sqrt(|Stab r'| / |Stab r|) from a table over the divisors of |G|
+struct RatioArgs {
+  const uint16_t *__restrict__ div_idx;
+  const double *__restrict__ ratio;
+  int32_t nd;
+};
+
 // ---- real rows -------------------------------------------------------------
-template <int METHOD, int NW, class XT, class YT>
+template <int METHOD, int NW, bool BONDS, class XT, class YT>
 __global__ void __launch_bounds__(kBlock)
-    matvec_real_kernel(OpArgs o, GenericOrbitArgs ga, RingOrbitArgs ra, IndexArgs ix,
-                       const uint32_t *__restrict__ stab, uint64_t D,
-                       const XT *__restrict__ x, YT *__restrict__ y) {
+    matvec_real_kernel(OpArgs o, BondArgs bd, RatioArgs rt, GenericOrbitArgs ga,
+                       RingOrbitArgs ra, IndexArgs ix, const uint32_t *__restrict__ stab,
+                       uint64_t D, const XT *__restrict__ x, YT *__restrict__ y) {
   const uint64_t lb = xcd_block(blockIdx.x, gridDim.x);
   const uint64_t i = lb * kBlock + threadIdx.x;
   if (i >= D) return;
@@ -132,26 +150,53 @@ __global__ void __launch_bounds__(kBlock)
 #pragma unroll
   for (int w = 0; w < NW; ++w) act[w] = 0;
   double diag = 0.0;
-  for (int t = 0; t < o.ntuples; ++t) {
-    const int n = static_cast<int>(o.tuple_n[t]);
-    const uint32_t off = o.tuple_off[t];
-    const int a = local_state(o.tuple_sites[t], n, r);
-    diag += o.diag_re[off + a];
-    const uint32_t* rg = o.range + off + t + a;
-    mark_range<NW>(act, rg[0], rg[1]);
+  if constexpr (BONDS) {
+#pragma unroll
+    for (int w = 0; w < NW; ++w) {
+      const int tb = 64 * w;
+      const int te = bd.nbonds < tb + 64 ? bd.nbonds : tb + 64;
+      for (int t = tb; t < te; ++t) {
+        const uint32_t s = bd.sites[t];
+        const uint64_t anti = ((r >> (s & 0xffu)) ^ (r >> (s >> 8))) & 1u;
+        diag += anti ? bd.da[t] : bd.dp[t];
+        if (bd.o[t] != 0.0) act[w] |= anti << (t - tb);
+      }
+    }
+  } else {
+    for (int t = 0; t < o.ntuples; ++t) {
+      const int n = static_cast<int>(o.tuple_n[t]);
+      const uint32_t off = o.tuple_off[t];
+      const int a = local_state(o.tuple_sites[t], n, r);
+      diag += o.diag_re[off + a];
+      const uint32_t *rg = o.range + off + t + a;
+      mark_range<NW>(act, rg[0], rg[1]);
+    }
   }
   double acc = diag * ld_re(x, i);
-  const double sr = static_cast<double>(stab[i]);
+  const uint32_t si = stab[i];
+  const double sr = static_cast<double>(si);
+  const int row_di = rt.nd ? static_cast<int>(rt.div_idx[si]) : 0;
+  // two templates per iteration: two orbit scans, then both searches and
+  // gathers in flight together; accumulation stays in template order
+  auto coef_of = [&](int j, const OrbitResult &orb) {
+    const double f = rt.nd ? rt.ratio[rt.div_idx[orb.count] * rt.nd + row_di]
+                           : sqrt(static_cast<double>(orb.count) / sr);
+    double c = (BONDS ? bd.o[j] : o.coef_re[j]) * f;
+    return orb.phase != 0 ? -c : c;
+  };
   while (any_set<NW>(act)) {
-    const int j = pop_first<NW>(act);
-    const uint64_t sigma = r ^ o.flip[j];
-    const OrbitResult orb = do_orbit<METHOD>(ga, ra, sigma);
-    if (!orb.valid) continue;
-    const int64_t k = find_index(ix, orb.rep);
-    if (k < 0) continue;
-    double coef = o.coef_re[j] * sqrt(static_cast<double>(orb.count) / sr);
-    if (orb.phase != 0) coef = -coef;
-    acc = fma(coef, ld_re(x, static_cast<uint64_t>(k)), acc);
+    const int ja = pop_first<NW>(act);
+    const int jb = pop_first<NW>(act);
+    const OrbitResult oa = do_orbit<METHOD>(ga, ra, r ^ (BONDS ? bd.flip[ja] : o.flip[ja]));
+    OrbitResult ob{0, 0, 0, false};
+    if (jb >= 0) ob = do_orbit<METHOD>(ga, ra, r ^ (BONDS ? bd.flip[jb] : o.flip[jb]));
+    int64_t ka, kb;
+    find_index2(ix, oa.rep, oa.valid, ob.rep, ob.valid, ka, kb);
+    double xa = 0.0, xb = 0.0;
+    if (ka >= 0) xa = ld_re(x, static_cast<uint64_t>(ka));
+    if (kb >= 0) xb = ld_re(x, static_cast<uint64_t>(kb));
+    if (ka >= 0) acc = fma(coef_of(ja, oa), xa, acc);
+    if (kb >= 0) acc = fma(coef_of(jb, ob), xb, acc);
   }
   st_re(y, i, acc);
 }
@@ -305,52 +350,83 @@ OpArgs op_args(const Operator &op, const double *pc, const double *ps) {
 
 unsigned grid_for(uint64_t n) { return static_cast<unsigned>((n + kBlock - 1) / kBlock); }
 
+// Everything a launch needs, gathered once per call.
+struct Ctx {
+  OpArgs o;
+  BondArgs bd;
+  RatioArgs rt;
+  GenericOrbitArgs ga;
+  RingOrbitArgs ra;
+  IndexArgs ix;
+  const uint32_t *stab;
+  uint64_t D;
+  bool bonds;
+};
+
+template <int METHOD, int NW, bool BONDS, class XT, class YT>
+void launch_real_nw(const Ctx &c, const XT *x, YT *y, hipStream_t s) {
+  matvec_real_kernel<METHOD, NW, BONDS, XT, YT><<<grid_for(c.D), kBlock, 0, s>>>(
+      c.o, c.bd, c.rt, c.ga, c.ra, c.ix, c.stab, c.D, x, y);
+}
+
 template <int METHOD, class XT, class YT>
-void launch_real(const OpArgs &o, const GenericOrbitArgs &ga, const RingOrbitArgs &ra,
-                 const IndexArgs &ix, const uint32_t *stab, uint64_t D, const XT *x, YT *y,
-                 hipStream_t s) {
-  const unsigned g = grid_for(D);
-  if (o.ntemplates <= 64)
-    matvec_real_kernel<METHOD, 1, XT, YT><<<g, kBlock, 0, s>>>(o, ga, ra, ix, stab, D, x, y);
-  else if (o.ntemplates <= 128)
-    matvec_real_kernel<METHOD, 2, XT, YT><<<g, kBlock, 0, s>>>(o, ga, ra, ix, stab, D, x, y);
-  else if (o.ntemplates <= 256)
-    matvec_real_kernel<METHOD, 4, XT, YT><<<g, kBlock, 0, s>>>(o, ga, ra, ix, stab, D, x, y);
+void launch_real(const Ctx &c, const XT *x, YT *y, hipStream_t s) {
+  const int units = c.bonds ? c.bd.nbonds : c.o.ntemplates;
+  if (c.bonds && units <= 64)
+    launch_real_nw<METHOD, 1, true>(c, x, y, s);
+  else if (c.bonds && units <= 128)
+    launch_real_nw<METHOD, 2, true>(c, x, y, s);
+  else if (!c.bonds && units <= 64)
+    launch_real_nw<METHOD, 1, false>(c, x, y, s);
+  else if (!c.bonds && units <= 128)
+    launch_real_nw<METHOD, 2, false>(c, x, y, s);
+  else if (!c.bonds && units <= 256)
+    launch_real_nw<METHOD, 4, false>(c, x, y, s);
   else
-    matvec_real_direct_kernel<METHOD, XT, YT><<<g, kBlock, 0, s>>>(o, ga, ra, ix, stab, D, x, y);
+    matvec_real_direct_kernel<METHOD, XT, YT><<<grid_for(c.D), kBlock, 0, s>>>(
+        c.o, c.ga, c.ra, c.ix, c.stab, c.D, x, y);
   SPINED_CHECK_LAUNCH();
 }
 
 template <int METHOD, class XT, class YT>
-void launch_complex(const OpArgs &o, const GenericOrbitArgs &ga, const RingOrbitArgs &ra,
-                    const IndexArgs &ix, const uint32_t *stab, uint64_t D, const XT *x, YT *y,
-                    hipStream_t s) {
-  matvec_complex_kernel<METHOD, XT, YT><<<grid_for(D), kBlock, 0, s>>>(o, ga, ra, ix, stab, D, x, y);
+void launch_complex(const Ctx &c, const XT *x, YT *y, hipStream_t s) {
+  matvec_complex_kernel<METHOD, XT, YT><<<grid_for(c.D), kBlock, 0, s>>>(c.o, c.ga, c.ra, c.ix,
+                                                                        c.stab, c.D, x, y);
   SPINED_CHECK_LAUNCH();
 }
 
-template <int METHOD>
-void dispatch(const MatvecCall &c, const OpArgs &o, const GenericOrbitArgs &ga,
-              const RingOrbitArgs &ra, const IndexArgs &ix, const uint32_t *stab, uint64_t D) {
-  hipStream_t s = c.stream;
+template <int METHOD> void dispatch(const MatvecCall &m, const Ctx &c) {
+  hipStream_t s = m.stream;
   using VT = VecType;
-  if (c.xt == VT::F64 && c.yt == VT::F64)
-    launch_real<METHOD>(o, ga, ra, ix, stab, D, static_cast<const double *>(c.x),
-                        static_cast<double *>(c.y), s);
-  else if (c.xt == VT::F32 && c.yt == VT::F32)
-    launch_real<METHOD>(o, ga, ra, ix, stab, D, static_cast<const float *>(c.x),
-                        static_cast<float *>(c.y), s);
-  else if (c.xt == VT::F32 && c.yt == VT::F64)
-    launch_real<METHOD>(o, ga, ra, ix, stab, D, static_cast<const float *>(c.x),
-                        static_cast<double *>(c.y), s);
-  else if (c.xt == VT::C128 && c.yt == VT::C128)
-    launch_complex<METHOD>(o, ga, ra, ix, stab, D, static_cast<const double2 *>(c.x),
-                           static_cast<double2 *>(c.y), s);
-  else if (c.xt == VT::C64 && c.yt == VT::C64)
-    launch_complex<METHOD>(o, ga, ra, ix, stab, D, static_cast<const float2 *>(c.x),
-                           static_cast<float2 *>(c.y), s);
+  if (m.xt == VT::F64 && m.yt == VT::F64)
+    launch_real<METHOD>(c, static_cast<const double *>(m.x), static_cast<double *>(m.y), s);
+  else if (m.xt == VT::F32 && m.yt == VT::F32)
+    launch_real<METHOD>(c, static_cast<const float *>(m.x), static_cast<float *>(m.y), s);
+  else if (m.xt == VT::F32 && m.yt == VT::F64)
+    launch_real<METHOD>(c, static_cast<const float *>(m.x), static_cast<double *>(m.y), s);
+  else if (m.xt == VT::C128 && m.yt == VT::C128)
+    launch_complex<METHOD>(c, static_cast<const double2 *>(m.x), static_cast<double2 *>(m.y), s);
+  else if (m.xt == VT::C64 && m.yt == VT::C64)
+    launch_complex<METHOD>(c, static_cast<const float2 *>(m.x), static_cast<float2 *>(m.y), s);
   else
     throw Error(LS_INVALID_DATATYPE, "unsupported matvec type combination");
+}
+
+Ctx make_ctx(const Operator &op, const double *pc, const double *ps) {
+  const Basis &b = *op.basis;
+  const auto &t = op.tables;
+  Ctx c{};
+  c.o = op_args(op, pc, ps);
+  c.bonds = t.bonds;
+  c.bd = BondArgs{t.bond_sites.ptr, t.bond_dp.ptr, t.bond_da.ptr, t.bond_o.ptr, t.bond_flip.ptr,
+                  static_cast<int32_t>(t.h_bond_sites.size())};
+  c.rt = RatioArgs{b.tables.div_idx.ptr, b.tables.ratio.ptr, b.tables.nd};
+  c.ga = b.tables.generic_args(true);
+  c.ra = b.tables.ring_args(true);
+  c.ix = b.index_args();
+  c.stab = b.stab.ptr;
+  c.D = b.D;
+  return c;
 }
 
 std::mutex g_phase_mu;
@@ -378,12 +454,12 @@ const PhaseTable &phase_table(const Basis &b) {
 
 } // namespace
 
-void launch_matvec(const MatvecCall &c) {
-  const Operator &op = *c.op;
+void launch_matvec(const MatvecCall &m) {
+  const Operator &op = *m.op;
   const Basis &b = *op.basis;
-  if (c.size != b.D) throw Error(LS_DIMENSION_MISMATCH, "vector length differs from the basis dimension");
+  if (m.size != b.D) throw Error(LS_DIMENSION_MISMATCH, "vector length differs from the basis dimension");
   if (b.D == 0) return;
-  const bool cplx = c.xt == VecType::C64 || c.xt == VecType::C128;
+  const bool cplx = m.xt == VecType::C64 || m.xt == VecType::C128;
   if (!cplx && !op.tables.is_real)
     throw Error(LS_INVALID_DATATYPE, "operator is complex; use a complex datatype");
   const double *pc = nullptr, *ps = nullptr;
@@ -392,14 +468,11 @@ void launch_matvec(const MatvecCall &c) {
     pc = pt.cos_t.ptr;
     ps = pt.sin_t.ptr;
   }
-  const OpArgs o = op_args(op, pc, ps);
-  const auto ga = b.tables.generic_args(true);
-  const auto ra = b.tables.ring_args(true);
-  const IndexArgs ix = b.index_args();
+  const Ctx c = make_ctx(op, pc, ps);
   if (op.orbit_method() == 1)
-    dispatch<1>(c, o, ga, ra, ix, b.stab.ptr, b.D);
+    dispatch<1>(m, c);
   else
-    dispatch<0>(c, o, ga, ra, ix, b.stab.ptr, b.D);
+    dispatch<0>(m, c);
 }
 
 uint64_t count_offdiag_device(const Operator &op) {

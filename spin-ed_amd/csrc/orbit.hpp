@@ -148,6 +148,7 @@ struct RingOrbitArgs {
   int32_t has_flip;
   int32_t flip_phase;
   int32_t L;
+  int32_t trivial; // every phase is 0: the table is not read
 };
 
 __host__ __device__ inline uint64_t rotl_n(uint64_t z, int j, int n, uint64_t mask) {
@@ -218,13 +219,21 @@ __host__ __device__ inline void ring_candidates(const RingOrbitArgs &a, uint64_t
     tops &= tops - 1;
     int k = q + 1;
     if (k >= n) k -= n;
-    int32_t p = a.phase_tab[k] + fph;
-    if (p >= a.L) p -= a.L;
+    int32_t p = 0;
+    if (!a.trivial) {
+      p = a.phase_tab[k] + fph;
+      if (p >= a.L) p -= a.L;
+    }
     orbit_update(best, ph, cnt, ok, rotr_n(u, k, n, a.mask), p);
     if (a.has_rev) {
-      int k2 = (n - q + len - 1) % n;
-      int32_t p2 = a.phase_tab[n + k2] + fph;
-      if (p2 >= a.L) p2 -= a.L;
+      int k2 = n - q + len - 1;
+      k2 = k2 >= n ? k2 - n : k2; // n - q + len - 1 lies in [len, n + len - 1]
+      k2 = k2 >= n ? k2 - n : k2;
+      int32_t p2 = 0;
+      if (!a.trivial) {
+        p2 = a.phase_tab[n + k2] + fph;
+        if (p2 >= a.L) p2 -= a.L;
+      }
       orbit_update(best, ph, cnt, ok, rotr_n(ru, k2, n, a.mask), p2);
     }
   }
@@ -270,6 +279,39 @@ __host__ __device__ inline int64_t find_index(const IndexArgs &t, uint64_t v) {
     n -= half;
   }
   return t.reps[lo] == v ? static_cast<int64_t>(lo) : -1;
+}
+
+// Two independent searches advanced in lockstep, so a lane keeps two
+// dependent load chains in flight instead of one.  use_a / use_b switch a
+// search off (its result is then -1).
+__host__ __device__ inline void find_index2(const IndexArgs &t, uint64_t va, bool use_a,
+                                            uint64_t vb, bool use_b, int64_t &ka, int64_t &kb) {
+  const uint64_t ba = va >> t.shift, bb = vb >> t.shift;
+  use_a = use_a && ba < t.nbuckets;
+  use_b = use_b && bb < t.nbuckets;
+  uint32_t loa = 0, na = 0, lob = 0, nb = 0;
+  if (use_a) {
+    loa = t.buckets[ba];
+    na = t.buckets[ba + 1] - loa;
+  }
+  if (use_b) {
+    lob = t.buckets[bb];
+    nb = t.buckets[bb + 1] - lob;
+  }
+  while (na > 1 || nb > 1) {
+    if (na > 1) {
+      const uint32_t h = na >> 1;
+      loa = (t.reps[loa + h] <= va) ? loa + h : loa;
+      na -= h;
+    }
+    if (nb > 1) {
+      const uint32_t h = nb >> 1;
+      lob = (t.reps[lob + h] <= vb) ? lob + h : lob;
+      nb -= h;
+    }
+  }
+  ka = (na == 1 && t.reps[loa] == va) ? static_cast<int64_t>(loa) : -1;
+  kb = (nb == 1 && t.reps[lob] == vb) ? static_cast<int64_t>(lob) : -1;
 }
 
 // Counter-hash start vector (SplitMix64 finaliser, Steele, Lea & Flood

@@ -583,6 +583,24 @@ void launch_random_fill(double *v, uint64_t n, uint64_t seed, hipStream_t s) {
   SPINED_CHECK_LAUNCH();
 }
 
+namespace {
+// Plain streaming copy (16-byte elements): the calibration kernel for the
+// HBM byte counters (a copy of B bytes should read B and write B).
+__global__ void __launch_bounds__(kBlock)
+    copy_calibration_kernel(const uint4 *__restrict__ src, uint4 *__restrict__ dst, uint64_t n) {
+  for (uint64_t i = blockIdx.x * static_cast<uint64_t>(kBlock) + threadIdx.x; i < n;
+       i += static_cast<uint64_t>(gridDim.x) * kBlock)
+    dst[i] = src[i];
+}
+} // namespace
+
+void launch_copy_calibration(const void *src, void *dst, uint64_t bytes, hipStream_t s) {
+  const uint64_t n = bytes / 16;
+  copy_calibration_kernel<<<grid_for(n, 1u << 16), kBlock, 0, s>>>(
+      static_cast<const uint4 *>(src), static_cast<uint4 *>(dst), n);
+  SPINED_CHECK_LAUNCH();
+}
+
 int eigsh(Operator &op, int dtype, uint64_t k, double tol, const spined_eigsh_options &opts,
           double *evals, void *evecs, uint64_t ld, double *rnorms, spined_monitor_fn monitor,
           void *ctx) {

@@ -38,7 +38,7 @@ SPINED_SYMBOLS = (
     "spined_device_free", "spined_memcpy", "spined_memset", "spined_cnn_num_params",
     "spined_cnn_create", "spined_cnn_destroy", "spined_cnn_get_params", "spined_cnn_set_params",
     "spined_cnn_forward_device", "spined_cnn_backward_device", "spined_cnn_adam_step",
-    "spined_cnn_start_vector", "spined_debug_mfma_probe",
+    "spined_cnn_start_vector", "spined_debug_mfma_probe", "spined_debug_copy_device",
 )
 
 
@@ -161,6 +161,7 @@ def load():
         "spined_cnn_adam_step": (I, [P, C.c_float]),
         "spined_cnn_start_vector": (I, [P, P, U64, P]),
         "spined_debug_mfma_probe": (I, [P]),
+        "spined_debug_copy_device": (I, [P, P, U64]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

@@ -37,6 +37,8 @@ CONFIGS = {
     "ring32": (32, "Heisenberg ring N=32, w=16, k=0, p=+1, z=+1"),
     "ring28": (28, "Heisenberg ring N=28, w=14, k=0, p=+1, z=+1"),
     "ring24": (24, "Heisenberg ring N=24, w=12, k=0, p=+1, z=+1 (28,968 states)"),
+    "xxz512": (0, "512 XXZ rings, N uniform over {12..26}, Delta~U(0,2), J_b~U(0.8,1.2), "
+                  "w=N/2, z=+1, residual 1e-9, one batched graph-replayed solve"),
 }
 PEAK_HBM = 8.0e12
 
@@ -48,7 +50,12 @@ def parse():
     p.add_argument("--warmup", type=int, default=None)
     p.add_argument("--config", default=os.environ.get("SPINED_BENCH_CONFIG", "ring40"),
                    choices=sorted(CONFIGS))
-    p.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
+    p.add_argument("--precision", default="mixed", choices=["fp64", "mixed"])
+    p.add_argument("--start", default="learned", choices=["random", "learned"])
+    p.add_argument("--train-sizes", default="16,18,20,22,24,26,28,30,32")
+    p.add_argument("--train-steps", type=int, default=800)
+    p.add_argument("--no-compare", action="store_true",
+                   help="skip the plain FP64 random-start reference solve")
     p.add_argument("--tol", type=float, default=1e-10)
     p.add_argument("--basis-size", type=int, default=0)
     p.add_argument("--cpu-sample-seconds", type=float, default=15.0)
@@ -78,6 +85,8 @@ def main():
     from spined.device import DeviceArray, device_synchronize
 
     n, desc = CONFIGS[a.config]
+    if a.config == "xxz512":
+        return bench_batch(a, desc)
     big = n >= 36
     steps = a.steps if a.steps is not None else (1 if big else 5)
     warmup = a.warmup if a.warmup is not None else (0 if big else 1)
@@ -122,10 +131,26 @@ def main():
     x.free()
     y.free()
 
-    def solve():
-        return spined.eigsh(op, k=1, tol=a.tol, precision=a.precision, seed=0x5EED,
-                            want_vectors=False, max_basis_size=a.basis_size,
-                            verbose=True)
+    model = None
+    train_info = None
+    if a.start == "learned":
+        from spined import learned
+        t = time.perf_counter()
+        sizes = [int(s) for s in a.train_sizes.split(",") if s]
+        data = [learned.ground_state(m) for m in sizes]
+        t_data = time.perf_counter() - t
+        model = learned.train(data, steps=a.train_steps, batch=2048, lr=3e-3, log=log)
+        t_train = time.perf_counter() - t - t_data
+        ov = {m.n: round(learned.overlap(model, m), 6) for m in data[-2:]}
+        train_info = {"sizes": sizes, "steps": a.train_steps, "data_s": round(t_data, 2),
+                      "train_s": round(t_train, 2), "overlap_on_training_sizes": ov}
+        log(f"learned start: {train_info}")
+
+    def solve(precision=a.precision, use_model=True):
+        return spined.eigsh(op, k=1, tol=a.tol, precision=precision, seed=0x5EED,
+                            want_vectors=False, max_basis_size=a.basis_size, verbose=True,
+                            model=model if use_model else None,
+                            sign_mask=sum(1 << i for i in range(0, n, 2)))
 
     for i in range(warmup):
         r = solve()
@@ -159,6 +184,20 @@ def main():
     if a.traffic_json and Path(a.traffic_json).exists():
         traffic = json.loads(Path(a.traffic_json).read_text()).get("bytes_per_launch")
 
+    compare = None
+    if not a.no_compare and (a.precision != "fp64" or a.start != "random"):
+        device_synchronize()
+        tc = time.perf_counter()
+        rc = solve("fp64", use_model=False)
+        device_synchronize()
+        t_ref = time.perf_counter() - tc
+        log(f"reference fp64/random: E0={rc.evals[0]:.15f} res={rc.rnorms[0]:.2e} "
+            f"matvecs={rc.stats['matvecs']} t={t_ref:.3f}s")
+        compare = {"fp64_random_s": t_ref, "fp64_random_matvecs": rc.stats["matvecs"],
+                   "fp64_random_E0": float(rc.evals[0]),
+                   "speedup_vs_fp64_random": t_ref / per_step,
+                   "E0_difference_vs_fp64": float(last.evals[0] - rc.evals[0])}
+
     cpu = None
     if not a.no_cpu:
         cpu = cpu_baseline(n, basis, gens, lattice, a.cpu_sample_seconds, D,
@@ -179,10 +218,14 @@ def main():
         "dtype": "f64" if a.precision == "fp64" else "f32-storage/f64-accumulate",
         "data": "synthetic (deterministic YAML-equivalent deck; random start vector seed 0x5EED)",
         "config": {"workload": desc, "dimension": D, "precision": a.precision,
-                   "start": "random", "tolerance": a.tol, "basis_build_s": round(t_build, 3)},
+                   "start": a.start, "tolerance": a.tol, "basis_build_s": round(t_build, 3),
+                   "learned_start": train_info},
+        "comparison": compare,
         "result": {"E0": float(last.evals[0]), "residual": float(last.rnorms[0]),
                    "E0_error_vs_bethe": (float(last.evals[0]) - e_exact) if e_exact else None,
                    "matvecs_per_solve": matvecs / len(results),
+                   "matvecs_fp32_phase": mixed_mv / len(results),
+                   "start_vector_s": last.stats["seconds_start"],
                    "matvec_ms": mv_avg * 1e3,
                    "matvec_rows_per_s": D / mv_avg if mv_avg > 0 else None,
                    "matvec_fraction_of_solve": mv_seconds / elapsed},
@@ -194,6 +237,87 @@ def main():
     }
     print(json.dumps(out))
     return 0
+
+
+def bench_batch(a, desc):
+    """Config 4: all 512 instances per step, timed as one batched solve."""
+    from spined import batch
+    from spined.device import device_synchronize
+    t = time.perf_counter()
+    inst = batch.xxz_instances(512)
+    sysb = batch.BatchSystem(inst)
+    device_synchronize()
+    t_setup = time.perf_counter() - t
+    dims = sysb.dimensions
+    log(f"xxz512: {len(inst)} instances, sum of dimensions {int(dims.sum())}, setup {t_setup:.2f} s")
+    steps = a.steps if a.steps is not None else 1
+    warmup = a.warmup if a.warmup is not None else 0
+    for _ in range(warmup):
+        batch.batch_eigsh(sysb.ops, tol=1e-9)
+    device_synchronize()
+    t0 = time.perf_counter()
+    res = None
+    for _ in range(steps):
+        res = batch.batch_eigsh(sysb.ops, tol=1e-9)
+        log(f"batch: {res['stats']}")
+    device_synchronize()
+    per = (time.perf_counter() - t0) / steps
+    sec = res["seconds"]
+    cpu = None
+    if not a.no_cpu:
+        cpu = batch_cpu_baseline(inst, dims, a.cpu_sample_seconds, per, len(inst))
+    out = {
+        "metric": "instances/s to residual<=1e-9 (batched uneven XXZ solves), 1 GPU",
+        "value": len(inst) / per, "unit": "instances/s", "n_gpus": 1, "steps": steps,
+        "warmup": warmup, "ms_per_step": per * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic (numpy default_rng(20261202))",
+        "config": {"workload": desc, "instances": len(inst), "sum_dimensions": int(dims.sum()),
+                   "setup_s": round(t_setup, 2)},
+        "result": {"converged": int(res["stats"]["converged"]),
+                   "max_residual": float(np.max(res["rnorms"])),
+                   "cycles": int(res["stats"]["cycles"]),
+                   "p50_instance_s": float(np.percentile(sec, 50)),
+                   "p99_instance_s": float(np.percentile(sec, 99)),
+                   "max_matvecs_per_instance": int(res["stats"]["matvecs_per_instance_max"])},
+        "roofline": None,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+    return 0
+
+
+def batch_cpu_baseline(inst, dims, budget_s, gpu_s, count):
+    """Oracle Lanczos on all host cores, one instance at a time, over the
+    smallest instances that fit the time budget; extrapolated by dimension."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import pyoracle
+    from spined import lattice
+    cores = os.cpu_count() or 1
+    order = np.argsort(dims)
+    t_all = 0.0
+    rows = 0
+    done = 0
+    for i in order:
+        x = inst[i]
+        n = x["n"]
+        t = time.perf_counter()
+        o = pyoracle.System(n, n // 2, 1, [])
+        for k in range(n):
+            o.add_interaction(np.asarray(lattice.xxz_matrix(x["delta"], x["J"][k]),
+                                         dtype=np.complex128), [[k, (k + 1) % n]])
+        o.build()
+        o.lanczos(k=1, tol=1e-9, max_iter=300)
+        t_all += time.perf_counter() - t
+        rows += int(dims[i])
+        done += 1
+        if t_all > budget_s:
+            break
+    rate = rows / t_all
+    est = float(np.sum(dims)) / rate
+    return {"value": count / est, "unit": "instances/s (extrapolated)", "cores": cores,
+            "kind": "port",
+            "sample": f"{done} smallest instances solved one at a time by the oracle "
+                      f"({rows} rows in {t_all:.1f} s); extrapolated by total dimension"}
 
 
 def cpu_baseline(n, basis, gens, lattice, budget_s, D, matvecs_per_solve):

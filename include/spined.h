@@ -126,6 +126,27 @@ int spined_basis_get_bucket_bits(const ls_spin_basis *basis);
 /* Exact sector dimension by character counting (before ls_build). */
 int spined_basis_sector_dimension(const ls_spin_basis *basis, double *out);
 
+/* ---- batched solver (no counterpart in the reference) ------------------ */
+/* Lowest eigenpair of `count` independent real exchange-bond operators
+ * (instances may have different sizes and may share a basis), solved
+ * together: every Lanczos step is one launch per kernel over all instances'
+ * row tiles, one restart cycle is a replayed hipGraph, converged instances
+ * drop out.  Replaces one SpinED run per Hamiltonian. */
+typedef struct spined_batch_stats {
+  int32_t cycles;                   /* restart cycles run */
+  int32_t converged;                /* instances with residual <= tol */
+  double seconds_total;             /* setup + solve + finalisation */
+  double seconds_cycles;            /* graph replays only */
+  uint64_t total_rows;              /* sum of the instances' dimensions */
+  int64_t matvecs_per_instance_max; /* matvecs of the slowest instance */
+} spined_batch_stats;
+/* evals, rnorms: [count]; cycles[i]: restart cycle at which instance i
+ * converged (-1: not converged); seconds[i]: time from the first replay to
+ * the end of that cycle.  max_basis_size <= 32 (0: 20). */
+int spined_batch_eigsh(const ls_operator *const ops[], uint64_t count, double tol,
+                       int32_t max_basis_size, uint64_t seed, int32_t max_cycles, double *evals,
+                       double *rnorms, int32_t *cycles, double *seconds, spined_batch_stats *stats);
+
 /* ---- learned start vector (no counterpart in the reference) ------------ */
 /* Periodic 1D CNN over +-1 spins: 3 circular conv layers (16 channels,
  * kernel 5, tanh-GELU), global sum, heads log|psi| and sign logit; 2,722

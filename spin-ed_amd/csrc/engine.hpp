@@ -177,6 +177,11 @@ int eigsh(Operator &op, int dtype, uint64_t k, double tol, const spined_eigsh_op
           double *evals, void *evecs, uint64_t ld, double *rnorms, spined_monitor_fn monitor,
           void *ctx);
 
+// ---- batched solver (batch.hip) ---------------------------------------------
+int batch_eigsh(const std::vector<Operator *> &ops, double tol, int m_req, uint64_t seed,
+                int max_cycles, double *evals, double *rnorms, int32_t *cycles_out,
+                double *seconds_out, spined_batch_stats *stats);
+
 // ---- misc kernels (solver.hip) ----------------------------------------------
 void launch_random_fill(double *v, uint64_t n, uint64_t seed, hipStream_t s);
 void launch_copy_calibration(const void *src, void *dst, uint64_t bytes, hipStream_t s);

@@ -159,6 +159,54 @@ __host__ __device__ inline uint64_t rotr_n(uint64_t u, int k, int n, uint64_t ma
   return k == 0 ? u : (((u >> k) | (u << (n - k))) & mask);
 }
 
+// Rotation by a power of two known at compile time (meaningful for S < n;
+// the right shift is kept in range so the unselected value is harmless).
+template <int S> __host__ __device__ inline uint64_t rotl_c(uint64_t z, int n, uint64_t mask) {
+  return ((z << S) | (z >> ((n - S) & 63))) & mask;
+}
+
+// Branch-free variant of longest_run_tops (same result): all doubling
+// levels and refinement steps are evaluated with selects, so a wave runs one
+// straight-line sequence instead of divergent branches.
+__host__ __device__ inline uint64_t longest_run_tops_bf(uint64_t z, int n, uint64_t mask,
+                                                        int &len) {
+  // doubling: b_k = tops of windows of 2^k ones, valid while 2^k <= n - 1
+  const uint64_t b0 = z;
+  const uint64_t b1 = n > 2 ? b0 & rotl_c<1>(b0, n, mask) : 0;
+  const uint64_t b2 = n > 4 ? b1 & rotl_c<2>(b1, n, mask) : 0;
+  const uint64_t b3 = n > 8 ? b2 & rotl_c<4>(b2, n, mask) : 0;
+  const uint64_t b4 = n > 16 ? b3 & rotl_c<8>(b3, n, mask) : 0;
+  const uint64_t b5 = n > 32 ? b4 & rotl_c<16>(b4, n, mask) : 0;
+  // highest non-empty level p (z != 0, z != mask handled by the caller)
+  uint64_t cur = b0;
+  int l = 1;
+  cur = b1 ? b1 : cur;
+  l = b1 ? 2 : l;
+  cur = b2 ? b2 : cur;
+  l = b2 ? 4 : l;
+  cur = b3 ? b3 : cur;
+  l = b3 ? 8 : l;
+  cur = b4 ? b4 : cur;
+  l = b4 ? 16 : l;
+  cur = b5 ? b5 : cur;
+  l = b5 ? 32 : l;
+  // refinement a_{l+s} = a_l & rotl(a_s, l) for s = l/2, l/4, ..., 1
+  // (s < l, and windows stay below n: a run of n - 1 ones is the maximum)
+  const uint64_t bs[5] = {b0, b1, b2, b3, b4};
+#pragma unroll
+  for (int q = 4; q >= 0; --q) {
+    const int s = 1 << q;
+    const bool use = s < l && l + s <= n - 1;
+    const int sh = use ? l : 1; // keep the shift amounts in range
+    const uint64_t t = cur & (((bs[q] << sh) | (bs[q] >> (n - sh))) & mask);
+    const bool take = use && t != 0;
+    cur = take ? t : cur;
+    l = take ? l + s : l;
+  }
+  len = l;
+  return cur;
+}
+
 // Positions q that are the top (highest cyclic position) of a longest
 // cyclic run of ones in z; len receives that length.  z == 0 yields every
 // position with len 0 (all rotations of an all-ones / all-zeros word tie).
@@ -171,42 +219,7 @@ __host__ __device__ inline uint64_t longest_run_tops(uint64_t z, int n, uint64_t
     len = n;
     return mask;
   }
-  // a_L = tops of windows of L ones; a_{2L} = a_L & rotl(a_L, L).
-  uint64_t bs[6];
-  bs[0] = z;
-  uint64_t cur = z;
-  int l = 1;
-  bool grow = true;
-#pragma unroll
-  for (int q = 1; q < 6; ++q) {
-    bs[q] = 0;
-    if (grow && 2 * l <= n - 1) {
-      const uint64_t nb = bs[q - 1] & rotl_n(bs[q - 1], l, n, mask);
-      if (nb) {
-        bs[q] = nb;
-        cur = nb;
-        l *= 2;
-      } else {
-        grow = false;
-      }
-    } else {
-      grow = false;
-    }
-  }
-  // binary refinement: a_{l+s} = a_l & rotl(a_s, l)
-#pragma unroll
-  for (int q = 4; q >= 0; --q) {
-    const int s = 1 << q;
-    if (s < l && l + s <= n - 1) {
-      const uint64_t t = cur & rotl_n(bs[q], l, n, mask);
-      if (t) {
-        cur = t;
-        l += s;
-      }
-    }
-  }
-  len = l;
-  return cur;
+  return longest_run_tops_bf(z, n, mask, len);
 }
 
 __host__ __device__ inline void ring_candidates(const RingOrbitArgs &a, uint64_t u, uint64_t tops,

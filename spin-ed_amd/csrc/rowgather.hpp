@@ -1,0 +1,189 @@
+// rowgather.hpp -- the per-row arithmetic of the real matrix-free H*x,
+// shared by the single-operator and the batched matvec kernels.
+// For representative r (row i) with |Stab r| = s_r:
+//   y_i = D(r) x_i + sum_sigma conj(c_sigma) chi(h) sqrt(|Stab r'| / s_r) x_{idx(r')}
+// (H|r> = sum c_sigma |sigma>, r' = h.sigma = min orbit(sigma)); terms whose
+// orbit has inconsistent phases or whose representative is not in the basis
+// are dropped.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "orbit.hpp"
+
+namespace spined {
+
+struct OpArgs {
+  const uint32_t *__restrict__ tuple_sites;
+  const uint32_t *__restrict__ tuple_n;
+  const uint32_t *__restrict__ tuple_off;
+  const uint32_t *__restrict__ range;
+  const double *__restrict__ diag_re;
+  const double *__restrict__ diag_im;
+  const uint64_t *__restrict__ flip;
+  const double *__restrict__ coef_re;
+  const double *__restrict__ coef_im;
+  const double *__restrict__ phase_cos; // complex sectors: chi(phase)
+  const double *__restrict__ phase_sin;
+  int32_t ntuples;
+  int32_t ntemplates;
+};
+
+template <class T> struct is_cplx : std::false_type {};
+template <> struct is_cplx<float2> : std::true_type {};
+template <> struct is_cplx<double2> : std::true_type {};
+
+__device__ inline double ld_re(const float *x, uint64_t i) { return static_cast<double>(x[i]); }
+__device__ inline double ld_re(const double *x, uint64_t i) { return x[i]; }
+__device__ inline double2 ld_c(const float2 *x, uint64_t i) {
+  const float2 v = x[i];
+  return make_double2(v.x, v.y);
+}
+__device__ inline double2 ld_c(const double2 *x, uint64_t i) { return x[i]; }
+__device__ inline void st_re(float *y, uint64_t i, double v) { y[i] = static_cast<float>(v); }
+__device__ inline void st_re(double *y, uint64_t i, double v) { y[i] = v; }
+__device__ inline void st_c(float2 *y, uint64_t i, double2 v) {
+  y[i] = make_float2(static_cast<float>(v.x), static_cast<float>(v.y));
+}
+__device__ inline void st_c(double2 *y, uint64_t i, double2 v) { y[i] = v; }
+
+// Bijective block remap: each of the 8 XCDs gets a contiguous range of rows,
+// so neighbouring rows (which share much of their gather targets) meet in
+// the same L2.
+__device__ inline uint64_t xcd_block(uint64_t b, uint64_t nblocks) {
+  const uint64_t xcd = b & 7, idx = b >> 3;
+  const uint64_t q = nblocks >> 3, r = nblocks & 7;
+  return xcd * q + (xcd < r ? xcd : r) + idx;
+}
+
+template <int METHOD>
+__device__ inline OrbitResult do_orbit(const GenericOrbitArgs &ga, const RingOrbitArgs &ra,
+                                       uint64_t s) {
+  if constexpr (METHOD == 1)
+    return orbit_ring(ra, s);
+  else
+    return orbit_generic(ga, s);
+}
+
+__device__ inline int local_state(uint32_t sites, int n, uint64_t r) {
+  int a = 0;
+  for (int k = 0; k < n; ++k) a = (a << 1) | static_cast<int>((r >> ((sites >> (8 * k)) & 0xff)) & 1);
+  return a;
+}
+
+template <int NW>
+__device__ inline void mark_range(uint64_t (&act)[NW], uint32_t b, uint32_t e) {
+#pragma unroll
+  for (int w = 0; w < NW; ++w) {
+    const uint32_t lo = 64u * w, hi = lo + 64u;
+    const uint32_t s = b > lo ? b : lo, t = e < hi ? e : hi;
+    if (s < t) {
+      const uint32_t len = t - s;
+      const uint64_t m = len == 64 ? ~0ull : (((1ull << len) - 1) << (s - lo));
+      act[w] |= m;
+    }
+  }
+}
+
+template <int NW> __device__ inline int pop_first(uint64_t (&act)[NW]) {
+  int j = -1;
+#pragma unroll
+  for (int w = 0; w < NW; ++w) {
+    if (j < 0 && act[w]) {
+      j = 64 * w + __builtin_ctzll(act[w]);
+      act[w] &= act[w] - 1;
+    }
+  }
+  return j;
+}
+
+template <int NW> __device__ inline bool any_set(const uint64_t (&act)[NW]) {
+  uint64_t o = 0;
+#pragma unroll
+  for (int w = 0; w < NW; ++w) o |= act[w];
+  return o != 0;
+}
+
+// Exchange bonds (Heisenberg / XXZ): diagonal dp (parallel) or da
+// (antiparallel) per bond, one off-diagonal template per antiparallel bond.
+struct BondArgs {
+  const uint32_t *__restrict__ sites; // s0 | s1 << 8
+  const double *__restrict__ dp;
+  const double *__restrict__ da;
+  const double *__restrict__ o;
+  const uint64_t *__restrict__ flip;
+  int32_t nbonds;
+};
+
+// sqrt(|Stab r'| / |Stab r|) from a table over the divisors of |G|
+struct RatioArgs {
+  const uint16_t *__restrict__ div_idx;
+  const double *__restrict__ ratio;
+  int32_t nd;
+};
+
+// One row of the real row gather (shared by the single and the batched
+// matvec): diagonal + active templates, two orbit scans / searches / gathers
+// in flight per iteration, accumulation in template order.
+template <int METHOD, int NW, bool BONDS, class XT>
+__device__ inline double gather_row_real(const OpArgs &o, const BondArgs &bd, const RatioArgs &rt,
+                                         const GenericOrbitArgs &ga, const RingOrbitArgs &ra,
+                                         const IndexArgs &ix, const uint32_t *__restrict__ stab,
+                                         uint64_t i, const XT *__restrict__ x) {
+  const uint64_t r = ix.reps[i];
+  uint64_t act[NW];
+#pragma unroll
+  for (int w = 0; w < NW; ++w) act[w] = 0;
+  double diag = 0.0;
+  if constexpr (BONDS) {
+#pragma unroll
+    for (int w = 0; w < NW; ++w) {
+      const int tb = 64 * w;
+      const int te = bd.nbonds < tb + 64 ? bd.nbonds : tb + 64;
+      for (int t = tb; t < te; ++t) {
+        const uint32_t s = bd.sites[t];
+        const uint64_t anti = ((r >> (s & 0xffu)) ^ (r >> (s >> 8))) & 1u;
+        diag += anti ? bd.da[t] : bd.dp[t];
+        if (bd.o[t] != 0.0) act[w] |= anti << (t - tb);
+      }
+    }
+  } else {
+    for (int t = 0; t < o.ntuples; ++t) {
+      const int n = static_cast<int>(o.tuple_n[t]);
+      const uint32_t off = o.tuple_off[t];
+      const int a = local_state(o.tuple_sites[t], n, r);
+      diag += o.diag_re[off + a];
+      const uint32_t *rg = o.range + off + t + a;
+      mark_range<NW>(act, rg[0], rg[1]);
+    }
+  }
+  double acc = diag * ld_re(x, i);
+  const uint32_t si = stab[i];
+  const double sr = static_cast<double>(si);
+  const int row_di = rt.nd ? static_cast<int>(rt.div_idx[si]) : 0;
+  // two templates per iteration: two orbit scans, then both searches and
+  // gathers in flight together; accumulation stays in template order
+  auto coef_of = [&](int j, const OrbitResult &orb) {
+    const double f = rt.nd ? rt.ratio[rt.div_idx[orb.count] * rt.nd + row_di]
+                           : sqrt(static_cast<double>(orb.count) / sr);
+    double c = (BONDS ? bd.o[j] : o.coef_re[j]) * f;
+    return orb.phase != 0 ? -c : c;
+  };
+  while (any_set<NW>(act)) {
+    const int ja = pop_first<NW>(act);
+    const int jb = pop_first<NW>(act);
+    const OrbitResult oa = do_orbit<METHOD>(ga, ra, r ^ (BONDS ? bd.flip[ja] : o.flip[ja]));
+    OrbitResult ob{0, 0, 0, false};
+    if (jb >= 0) ob = do_orbit<METHOD>(ga, ra, r ^ (BONDS ? bd.flip[jb] : o.flip[jb]));
+    int64_t ka, kb;
+    find_index2(ix, oa.rep, oa.valid, ob.rep, ob.valid, ka, kb);
+    double xa = 0.0, xb = 0.0;
+    if (ka >= 0) xa = ld_re(x, static_cast<uint64_t>(ka));
+    if (kb >= 0) xb = ld_re(x, static_cast<uint64_t>(kb));
+    if (ka >= 0) acc = fma(coef_of(ja, oa), xa, acc);
+    if (kb >= 0) acc = fma(coef_of(jb, ob), xb, acc);
+  }
+  return acc;
+}
+
+} // namespace spined

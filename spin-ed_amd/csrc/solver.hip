@@ -24,6 +24,7 @@
 #include <vector>
 
 #include "engine.hpp"
+#include "jacobi.hpp"
 
 namespace spined {
 
@@ -137,126 +138,10 @@ __global__ void store_column_kernel(const double *__restrict__ h, int j, double 
   }
 }
 
-// Parallel cyclic Jacobi (round-robin pairing) for the symmetric m x m
-// projected matrix, m <= 64, in LDS.  evals ascending; S column-major.
 __global__ void __launch_bounds__(256)
     jacobi_eigh_kernel(const double *__restrict__ T, int ldt, int m, double *__restrict__ evals,
                        double *__restrict__ S, int max_sweeps) {
-  __shared__ double A[kMaxBasis][kMaxBasis + 1];
-  __shared__ double V[kMaxBasis][kMaxBasis + 1];
-  __shared__ double cs[kMaxBasis / 2], sn[kMaxBasis / 2];
-  __shared__ int pp[kMaxBasis / 2], qq[kMaxBasis / 2];
-  __shared__ double red[256];
-  __shared__ int done;
-  const int tid = threadIdx.x;
-  const int mp = m + (m & 1);
-  for (int e = tid; e < mp * mp; e += blockDim.x) {
-    const int i = e / mp, j = e % mp;
-    double a = 0.0;
-    if (i < m && j < m) a = 0.5 * (T[i * ldt + j] + T[j * ldt + i]);
-    A[i][j] = a;
-    V[i][j] = i == j ? 1.0 : 0.0;
-  }
-  __syncthreads();
-  for (int sweep = 0; sweep < max_sweeps; ++sweep) {
-    // convergence: off-diagonal mass vs diagonal mass
-    double off = 0.0, dia = 0.0;
-    for (int e = tid; e < mp * mp; e += blockDim.x) {
-      const int i = e / mp, j = e % mp;
-      const double a = A[i][j];
-      if (i == j)
-        dia += a * a;
-      else
-        off += a * a;
-    }
-    red[tid] = off;
-    __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-      if (tid < s) red[tid] += red[tid + s];
-      __syncthreads();
-    }
-    const double off_tot = red[0];
-    __syncthreads();
-    red[tid] = dia;
-    __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-      if (tid < s) red[tid] += red[tid + s];
-      __syncthreads();
-    }
-    const double dia_tot = red[0];
-    if (tid == 0) done = off_tot <= 1e-30 * (dia_tot + off_tot) || off_tot == 0.0;
-    __syncthreads();
-    if (done) break;
-    for (int step = 0; step < mp - 1; ++step) {
-      const int half = mp / 2;
-      if (tid < half) {
-        // round robin: position 0 holds player 0, others rotate
-        auto player = [&](int pos) { return pos == 0 ? 0 : 1 + ((pos - 1 + step) % (mp - 1)); };
-        int p = player(tid), q = player(mp - 1 - tid);
-        if (p > q) {
-          const int t = p;
-          p = q;
-          q = t;
-        }
-        pp[tid] = p;
-        qq[tid] = q;
-        const double apq = A[p][q];
-        double c = 1.0, s = 0.0;
-        if (apq != 0.0) {
-          const double app = A[p][p], aqq = A[q][q];
-          const double tau = (aqq - app) / (2.0 * apq);
-          const double t = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
-          c = 1.0 / sqrt(1.0 + t * t);
-          s = t * c;
-        }
-        cs[tid] = c;
-        sn[tid] = s;
-      }
-      __syncthreads();
-      // A <- A J (columns p, q) and V <- V J
-      for (int e = tid; e < half * mp; e += blockDim.x) {
-        const int k = e / mp, i = e % mp;
-        const int p = pp[k], q = qq[k];
-        const double c = cs[k], s = sn[k];
-        const double aip = A[i][p], aiq = A[i][q];
-        A[i][p] = c * aip - s * aiq;
-        A[i][q] = s * aip + c * aiq;
-        const double vip = V[i][p], viq = V[i][q];
-        V[i][p] = c * vip - s * viq;
-        V[i][q] = s * vip + c * viq;
-      }
-      __syncthreads();
-      // A <- J^T A (rows p, q)
-      for (int e = tid; e < half * mp; e += blockDim.x) {
-        const int k = e / mp, j = e % mp;
-        const int p = pp[k], q = qq[k];
-        const double c = cs[k], s = sn[k];
-        const double apj = A[p][j], aqj = A[q][j];
-        A[p][j] = c * apj - s * aqj;
-        A[q][j] = s * apj + c * aqj;
-      }
-      __syncthreads();
-    }
-  }
-  __syncthreads();
-  if (tid == 0) {
-    // selection sort of the m real eigenpairs (the padding index is dropped)
-    int order[kMaxBasis];
-    for (int i = 0; i < m; ++i) order[i] = i;
-    for (int i = 0; i < m; ++i) {
-      int best = i;
-      for (int j = i + 1; j < m; ++j)
-        if (A[order[j]][order[j]] < A[order[best]][order[best]]) best = j;
-      const int t = order[i];
-      order[i] = order[best];
-      order[best] = t;
-    }
-    for (int i = 0; i < m; ++i) {
-      const int c = order[i];
-      evals[i] = A[c][c];
-      for (int r = 0; r < m; ++r) S[i * m + r] = V[r][c];
-    }
-  }
+  jacobi_eigh_device(T, ldt, m, evals, S, max_sweeps);
 }
 
 // In place, row-local: V_i[r] <- sum_q S[i*m + q] V_q[r] for i < keep.

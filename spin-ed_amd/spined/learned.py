@@ -193,12 +193,15 @@ def train(data: List[GroundState], steps: int = 600, batch: int = 2048, lr: floa
     """Adam on batches drawn with probability c_r^2 from each size in turn."""
     rng = np.random.default_rng(seed)
     model = model or CNN(seed=seed)
-    probs = [d.weight / d.weight.sum() for d in data]
+    cdfs = []
+    for d in data:
+        c = np.cumsum(d.weight)
+        cdfs.append(c / c[-1])
     t0 = time.perf_counter()
     for it in range(steps):
         j = it % len(data)
         d = data[j]
-        idx = rng.choice(len(d.reps), size=batch, p=probs[j])
+        idx = np.minimum(np.searchsorted(cdfs[j], rng.random(batch)), len(d.reps) - 1)
         cur_lr = lr * (0.5 * (1 + math.cos(math.pi * it / steps)) * 0.9 + 0.1)
         loss = model.train_step(d.n, d.reps[idx], d.logamp[idx], d.sign[idx], cur_lr)
         if log and (it % 100 == 0 or it == steps - 1):

@@ -66,6 +66,10 @@ def parse():
                    help="run the plain copy kernel once on this many bytes (counter calibration)")
     p.add_argument("--matvecs-only", type=int, default=0,
                    help="profiling mode: run this many FP64 matvecs instead of solves")
+    p.add_argument("--block", type=int, default=1, choices=(1, 2, 4, 8),
+                   help="matvecs-only mode: row-interleaved block width")
+    p.add_argument("--block-precision", default="fp64", choices=("fp64", "mixed"),
+                   help="matvecs-only block mode: FP64 X, or FP32 X with FP64 Y")
     return p.parse_args()
 
 
@@ -118,6 +122,26 @@ def main():
     y = DeviceArray(D, np.float64)
     spined.matvec_device(op, x, y, 1)
     spined.synchronize()
+    if a.matvecs_only and a.block > 1:
+        x.free()
+        y.free()
+        K = a.block
+        mixed = a.block_precision == "mixed"
+        xb = DeviceArray.from_numpy(np.ones(D * K, np.float32 if mixed else np.float64))
+        yb = DeviceArray(D * K, np.float64)
+        code = 0 if mixed else 1
+        spined.matvec_rows_device(op, xb, yb, code, K, y_is_fp64=mixed)
+        spined.synchronize()
+        t = time.perf_counter()
+        for _ in range(a.matvecs_only):
+            spined.matvec_rows_device(op, xb, yb, code, K, y_is_fp64=mixed)
+        spined.synchronize()
+        dt = (time.perf_counter() - t) / a.matvecs_only
+        print(json.dumps({"mode": "matvecs-only", "config": desc, "dimension": D, "block": K,
+                          "block_precision": a.block_precision, "launches": a.matvecs_only,
+                          "ms_per_launch": dt * 1e3, "ms_per_vector": dt * 1e3 / K,
+                          "offdiag_gathers_per_launch": e_off}))
+        return 0
     if a.matvecs_only:
         t = time.perf_counter()
         for _ in range(a.matvecs_only):

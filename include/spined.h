@@ -112,6 +112,17 @@ int spined_basis_get_stabilizers(const ls_spin_basis *basis, uint32_t *out);
 int spined_operator_matmat_device(const ls_operator *op, int dtype, uint64_t size,
                                   uint64_t block_size, const void *x, uint64_t x_stride,
                                   void *y, uint64_t y_stride, int y_is_fp64);
+/* Y = H X for a row-interleaved block of block_size in {1, 2, 4, 8} real
+ * vectors (element (row, c) at row * block_size + c; DEVICE pointers): one
+ * orbit scan and index search per term serve the whole block, and every
+ * gather reads one row of the block.  Per column the sums are those of
+ * spined_operator_matmat_device.  ls_operator_matmat and
+ * spined_operator_matmat_device use it for real blocks (chunks of <= 8
+ * columns), the block form of ls_operator_matmat's block_size
+ * (src/SpinED/Internal.hs:411-429). */
+int spined_operator_matmat_rows_device(const ls_operator *op, int dtype, uint64_t size,
+                                       uint64_t block_size, const void *x, void *y,
+                                       int y_is_fp64);
 /* Number of off-diagonal terms (sigma states) H generates from the basis:
  * the per-matvec unit count behind the algorithmic byte figure. */
 int spined_operator_count_offdiag(const ls_operator *op, uint64_t *out);

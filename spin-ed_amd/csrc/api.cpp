@@ -1,6 +1,7 @@
 // api.cpp -- the ls_* C ABI (include/lattice_symmetries.h) and the spined_*
 // entry points (include/spined.h).  Each function cites the reference
 // binding it serves in src/SpinED/Internal.hs.
+#include <algorithm>
 #include <cstdarg>
 #include <cstdlib>
 #include <cstring>
@@ -333,6 +334,27 @@ int ls_operator_matmat(const ls_operator *op, int dtype, uint64_t size, uint64_t
       throw Error(LS_INVALID_ARGUMENT, "stride smaller than size");
     const size_t es = elem_size(dtype);
     hipStream_t s = stream();
+    if (block_size >= 2 && (vt == VecType::F64 || vt == VecType::F32) &&
+        matvec_block_supported(o)) {
+      // chunks of up to 8 columns through the row-interleaved block matvec
+      const uint64_t w = std::min<uint64_t>(block_size, 8);
+      DeviceBuffer<char> dc(size * es * w), dxr(size * es * w), dyr(size * es * w);
+      for (uint64_t c0 = 0; c0 < block_size;) {
+        const int kc = block_chunk(block_size - c0);
+        const char *xc = static_cast<const char *>(x) + c0 * x_stride * es;
+        char *yc = static_cast<char *>(y) + c0 * y_stride * es;
+        SPINED_HIP(hipMemcpy2DAsync(dc.ptr, size * es, xc, x_stride * es, size * es, kc,
+                                    hipMemcpyHostToDevice, s));
+        launch_cols_to_rows(vt, dc.ptr, size, size, kc, dxr.ptr, s);
+        launch_matvec_block(MatvecBlockCall{&o, vt, vt, dxr.ptr, dyr.ptr, size, kc, s});
+        launch_rows_to_cols(vt, dyr.ptr, size, kc, dc.ptr, size, s);
+        SPINED_HIP(hipMemcpy2DAsync(yc, y_stride * es, dc.ptr, size * es, size * es, kc,
+                                    hipMemcpyDeviceToHost, s));
+        c0 += kc;
+      }
+      SPINED_HIP(hipStreamSynchronize(s));
+      return;
+    }
     DeviceBuffer<char> dx(size * es), dy(size * es);
     for (uint64_t c = 0; c < block_size; ++c) {
       const char *xc = static_cast<const char *>(x) + c * x_stride * es;
@@ -533,9 +555,49 @@ int spined_operator_matmat_device(const ls_operator *op, int dtype, uint64_t siz
     if (x_stride < size || y_stride < size)
       throw Error(LS_INVALID_ARGUMENT, "stride smaller than size");
     const size_t xs = elem_size(dtype), ys = y_is_fp64 ? 8 : elem_size(dtype);
+    if (block_size >= 2 && (xt == VecType::F64 || xt == VecType::F32) &&
+        matvec_block_supported(o)) {
+      const uint64_t w = std::min<uint64_t>(block_size, 8);
+      DeviceBuffer<char> dxr(size * xs * w), dyr(size * ys * w);
+      hipStream_t s = stream();
+      for (uint64_t c0 = 0; c0 < block_size;) {
+        const int kc = block_chunk(block_size - c0);
+        launch_cols_to_rows(xt, static_cast<const char *>(x) + c0 * x_stride * xs, x_stride, size,
+                            kc, dxr.ptr, s);
+        launch_matvec_block(MatvecBlockCall{&o, xt, yt, dxr.ptr, dyr.ptr, size, kc, s});
+        launch_rows_to_cols(yt, dyr.ptr, size, kc, static_cast<char *>(y) + c0 * y_stride * ys,
+                            y_stride, s);
+        c0 += kc;
+      }
+      // the temporaries are freed when this returns: wait for their users
+      SPINED_HIP(hipStreamSynchronize(s));
+      return;
+    }
     for (uint64_t c = 0; c < block_size; ++c)
       launch_matvec(MatvecCall{&o, xt, yt, static_cast<const char *>(x) + c * x_stride * xs,
                                static_cast<char *>(y) + c * y_stride * ys, size, stream()});
+  });
+}
+
+int spined_operator_matmat_rows_device(const ls_operator *op, int dtype, uint64_t size,
+                                       uint64_t block_size, const void *x, void *y,
+                                       int y_is_fp64) {
+  return guarded([&] {
+    Operator &o = prepare_operator(op);
+    const VecType xt = vec_type(dtype);
+    if (xt != VecType::F64 && xt != VecType::F32)
+      throw Error(LS_INVALID_DATATYPE, "row-interleaved blocks are real");
+    VecType yt = xt;
+    if (y_is_fp64) {
+      if (xt != VecType::F32) throw Error(LS_INVALID_DATATYPE, "y_is_fp64 needs float32 x");
+      yt = VecType::F64;
+    }
+    if (size != o.basis->D) throw Error(LS_DIMENSION_MISMATCH, "size differs from dimension");
+    if (block_size != 1 && block_size != 2 && block_size != 4 && block_size != 8)
+      throw Error(LS_INVALID_ARGUMENT, "block width must be 1, 2, 4 or 8");
+    if (size && (!x || !y)) throw Error(LS_INVALID_ARGUMENT, "NULL vector");
+    launch_matvec_block(MatvecBlockCall{&o, xt, yt, x, y, size, static_cast<int>(block_size),
+                                        stream()});
   });
 }
 

@@ -170,6 +170,26 @@ struct MatvecCall {
   hipStream_t stream;
 };
 void launch_matvec(const MatvecCall &c);
+// Row-interleaved block of K in {1, 2, 4, 8} real vectors: x[row * K + c].
+struct MatvecBlockCall {
+  const Operator *op;
+  VecType xt, yt;
+  const void *x;
+  void *y;
+  uint64_t size;
+  int K;
+  hipStream_t stream;
+};
+bool matvec_block_supported(const Operator &op);
+void launch_matvec_block(const MatvecBlockCall &c);
+// Layout changes between column-major (stride in elements) and
+// row-interleaved blocks of real vectors of type t.
+void launch_cols_to_rows(VecType t, const void *x, uint64_t stride, uint64_t D, int K, void *xr,
+                         hipStream_t s);
+void launch_rows_to_cols(VecType t, const void *yr, uint64_t D, int K, void *y, uint64_t stride,
+                         hipStream_t s);
+// Largest supported block width (8, 4, 2, 1) not above `remaining`.
+int block_chunk(uint64_t remaining);
 uint64_t count_offdiag_device(const Operator &op);
 
 // ---- eigensolver (solver.hip) ---------------------------------------------

@@ -40,6 +40,44 @@ __global__ void __launch_bounds__(kBlock)
   st_re(y, i, gather_row_real<METHOD, NW, BONDS>(o, bd, rt, ga, ra, ix, stab, i, x));
 }
 
+// Row-interleaved block of K real vectors (x[row * K + c]); per column the
+// same sums as matvec_real_kernel.
+template <int METHOD, int NW, bool BONDS, int K, class XT, class YT>
+__global__ void __launch_bounds__(kBlock)
+    matvec_block_kernel(OpArgs o, BondArgs bd, RatioArgs rt, GenericOrbitArgs ga,
+                        RingOrbitArgs ra, IndexArgs ix, const uint32_t *__restrict__ stab,
+                        uint64_t D, const XT *__restrict__ x, YT *__restrict__ y) {
+  const uint64_t lb = xcd_block(blockIdx.x, gridDim.x);
+  const uint64_t i = lb * kBlock + threadIdx.x;
+  if (i >= D) return;
+  double acc[K];
+  gather_row_real_block<METHOD, NW, BONDS, K>(o, bd, rt, ga, ra, ix, stab, i, x, acc);
+  RowK<YT, K> out;
+#pragma unroll
+  for (int c = 0; c < K; ++c) out.v[c] = static_cast<YT>(acc[c]);
+  reinterpret_cast<RowK<YT, K> *>(y)[i] = out;
+}
+
+// Column-major block (column c at x + c * stride) <-> row-interleaved block
+// (xr[row * K + c]); one thread per row.
+template <class A, class B>
+__global__ void __launch_bounds__(kBlock)
+    cols_to_rows_kernel(const A *__restrict__ x, uint64_t stride, uint64_t D, int K,
+                        B *__restrict__ xr) {
+  const uint64_t i = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
+  if (i >= D) return;
+  for (int c = 0; c < K; ++c) xr[i * K + c] = static_cast<B>(x[c * stride + i]);
+}
+
+template <class A, class B>
+__global__ void __launch_bounds__(kBlock)
+    rows_to_cols_kernel(const A *__restrict__ yr, uint64_t D, int K, B *__restrict__ y,
+                        uint64_t stride) {
+  const uint64_t i = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
+  if (i >= D) return;
+  for (int c = 0; c < K; ++c) y[c * stride + i] = static_cast<B>(yr[i * K + c]);
+}
+
 // Rows with more than 256 off-diagonal templates: per-tuple loop.
 template <int METHOD, class XT, class YT>
 __global__ void __launch_bounds__(kBlock)
@@ -251,6 +289,54 @@ template <int METHOD> void dispatch(const MatvecCall &m, const Ctx &c) {
     throw Error(LS_INVALID_DATATYPE, "unsupported matvec type combination");
 }
 
+template <int METHOD, int NW, bool BONDS, int K, class XT, class YT>
+void launch_block_nw(const Ctx &c, const XT *x, YT *y, hipStream_t s) {
+  matvec_block_kernel<METHOD, NW, BONDS, K, XT, YT><<<grid_for(c.D), kBlock, 0, s>>>(
+      c.o, c.bd, c.rt, c.ga, c.ra, c.ix, c.stab, c.D, x, y);
+}
+
+template <int METHOD, int K, class XT, class YT>
+void launch_block_k(const Ctx &c, const XT *x, YT *y, hipStream_t s) {
+  const int units = c.bonds ? c.bd.nbonds : c.o.ntemplates;
+  if (c.bonds && units <= 64)
+    launch_block_nw<METHOD, 1, true, K>(c, x, y, s);
+  else if (c.bonds && units <= 128)
+    launch_block_nw<METHOD, 2, true, K>(c, x, y, s);
+  else if (!c.bonds && units <= 64)
+    launch_block_nw<METHOD, 1, false, K>(c, x, y, s);
+  else if (!c.bonds && units <= 128)
+    launch_block_nw<METHOD, 2, false, K>(c, x, y, s);
+  else if (!c.bonds && units <= 256)
+    launch_block_nw<METHOD, 4, false, K>(c, x, y, s);
+  else
+    throw Error(LS_NOT_IMPLEMENTED, "block matvec with more than 256 off-diagonal templates");
+  SPINED_CHECK_LAUNCH();
+}
+
+template <int METHOD, class XT, class YT>
+void launch_block(const Ctx &c, int K, const XT *x, YT *y, hipStream_t s) {
+  switch (K) {
+  case 1: launch_real<METHOD>(c, x, y, s); break;
+  case 2: launch_block_k<METHOD, 2>(c, x, y, s); break;
+  case 4: launch_block_k<METHOD, 4>(c, x, y, s); break;
+  case 8: launch_block_k<METHOD, 8>(c, x, y, s); break;
+  default: throw Error(LS_INVALID_ARGUMENT, "block width must be 1, 2, 4 or 8");
+  }
+}
+
+template <int METHOD> void dispatch_block(const MatvecBlockCall &m, const Ctx &c) {
+  hipStream_t s = m.stream;
+  using VT = VecType;
+  if (m.xt == VT::F64 && m.yt == VT::F64)
+    launch_block<METHOD>(c, m.K, static_cast<const double *>(m.x), static_cast<double *>(m.y), s);
+  else if (m.xt == VT::F32 && m.yt == VT::F32)
+    launch_block<METHOD>(c, m.K, static_cast<const float *>(m.x), static_cast<float *>(m.y), s);
+  else if (m.xt == VT::F32 && m.yt == VT::F64)
+    launch_block<METHOD>(c, m.K, static_cast<const float *>(m.x), static_cast<double *>(m.y), s);
+  else
+    throw Error(LS_INVALID_DATATYPE, "block matvec supports real float32/float64 vectors");
+}
+
 Ctx make_ctx(const Operator &op, const double *pc, const double *ps) {
   const Basis &b = *op.basis;
   const auto &t = op.tables;
@@ -313,6 +399,57 @@ void launch_matvec(const MatvecCall &m) {
   else
     dispatch<0>(m, c);
 }
+
+bool matvec_block_supported(const Operator &op) {
+  const auto &t = op.tables;
+  if (!t.is_real) return false;
+  const int units = t.bonds ? static_cast<int>(t.h_bond_sites.size()) : t.ntemplates;
+  return units <= (t.bonds ? 128 : 256);
+}
+
+void launch_matvec_block(const MatvecBlockCall &m) {
+  const Operator &op = *m.op;
+  const Basis &b = *op.basis;
+  if (m.size != b.D) throw Error(LS_DIMENSION_MISMATCH, "vector length differs from the basis dimension");
+  if (b.D == 0) return;
+  if (!matvec_block_supported(op))
+    throw Error(LS_NOT_IMPLEMENTED, "block matvec needs a real operator with <= 256 templates");
+  const Ctx c = make_ctx(op, nullptr, nullptr);
+  if (op.orbit_method() == 1)
+    dispatch_block<1>(m, c);
+  else
+    dispatch_block<0>(m, c);
+}
+
+void launch_cols_to_rows(VecType t, const void *x, uint64_t stride, uint64_t D, int K, void *xr,
+                         hipStream_t s) {
+  if (D == 0 || K == 0) return;
+  if (t == VecType::F64)
+    cols_to_rows_kernel<<<grid_for(D), kBlock, 0, s>>>(static_cast<const double *>(x), stride, D,
+                                                       K, static_cast<double *>(xr));
+  else if (t == VecType::F32)
+    cols_to_rows_kernel<<<grid_for(D), kBlock, 0, s>>>(static_cast<const float *>(x), stride, D,
+                                                       K, static_cast<float *>(xr));
+  else
+    throw Error(LS_INVALID_DATATYPE, "real vectors expected");
+  SPINED_CHECK_LAUNCH();
+}
+
+void launch_rows_to_cols(VecType t, const void *yr, uint64_t D, int K, void *y, uint64_t stride,
+                         hipStream_t s) {
+  if (D == 0 || K == 0) return;
+  if (t == VecType::F64)
+    rows_to_cols_kernel<<<grid_for(D), kBlock, 0, s>>>(static_cast<const double *>(yr), D, K,
+                                                       static_cast<double *>(y), stride);
+  else if (t == VecType::F32)
+    rows_to_cols_kernel<<<grid_for(D), kBlock, 0, s>>>(static_cast<const float *>(yr), D, K,
+                                                       static_cast<float *>(y), stride);
+  else
+    throw Error(LS_INVALID_DATATYPE, "real vectors expected");
+  SPINED_CHECK_LAUNCH();
+}
+
+int block_chunk(uint64_t remaining) { return remaining >= 8 ? 8 : remaining >= 4 ? 4 : remaining >= 2 ? 2 : 1; }
 
 uint64_t count_offdiag_device(const Operator &op) {
   const Basis &b = *op.basis;

@@ -122,16 +122,11 @@ struct RatioArgs {
   int32_t nd;
 };
 
-// One row of the real row gather (shared by the single and the batched
-// matvec): diagonal + active templates, two orbit scans / searches / gathers
-// in flight per iteration, accumulation in template order.
-template <int METHOD, int NW, bool BONDS, class XT>
-__device__ inline double gather_row_real(const OpArgs &o, const BondArgs &bd, const RatioArgs &rt,
-                                         const GenericOrbitArgs &ga, const RingOrbitArgs &ra,
-                                         const IndexArgs &ix, const uint32_t *__restrict__ stab,
-                                         uint64_t i, const XT *__restrict__ x) {
-  const uint64_t r = ix.reps[i];
-  uint64_t act[NW];
+// Pass A of a row: the diagonal and the active off-diagonal templates (in
+// exchange-bond form: the antiparallel bonds), one bit per template.
+template <int NW, bool BONDS>
+__device__ inline double row_pass_a(const OpArgs &o, const BondArgs &bd, uint64_t r,
+                                    uint64_t (&act)[NW]) {
 #pragma unroll
   for (int w = 0; w < NW; ++w) act[w] = 0;
   double diag = 0.0;
@@ -157,6 +152,20 @@ __device__ inline double gather_row_real(const OpArgs &o, const BondArgs &bd, co
       mark_range<NW>(act, rg[0], rg[1]);
     }
   }
+  return diag;
+}
+
+// One row of the real row gather (shared by the single and the batched
+// matvec): diagonal + active templates, two orbit scans / searches / gathers
+// in flight per iteration, accumulation in template order.
+template <int METHOD, int NW, bool BONDS, class XT>
+__device__ inline double gather_row_real(const OpArgs &o, const BondArgs &bd, const RatioArgs &rt,
+                                         const GenericOrbitArgs &ga, const RingOrbitArgs &ra,
+                                         const IndexArgs &ix, const uint32_t *__restrict__ stab,
+                                         uint64_t i, const XT *__restrict__ x) {
+  const uint64_t r = ix.reps[i];
+  uint64_t act[NW];
+  const double diag = row_pass_a<NW, BONDS>(o, bd, r, act);
   double acc = diag * ld_re(x, i);
   const uint32_t si = stab[i];
   const double sr = static_cast<double>(si);
@@ -184,6 +193,66 @@ __device__ inline double gather_row_real(const OpArgs &o, const BondArgs &bd, co
     if (kb >= 0) acc = fma(coef_of(jb, ob), xb, acc);
   }
   return acc;
+}
+
+
+// K values of one row of a row-interleaved block, loaded as one aligned
+// record (16-byte loads).
+template <class XT, int K> struct alignas(K * sizeof(XT) < 16 ? K * sizeof(XT) : 16) RowK {
+  XT v[K];
+};
+
+// Block variant: K vectors stored row-interleaved (x[row * K + c]); one
+// orbit scan and one index search per term serve all K columns, and each
+// gather reads the K values of one row (one 64-byte line for K = 8 FP64).
+// Per column the arithmetic (order and fma pattern) equals gather_row_real.
+template <int METHOD, int NW, bool BONDS, int K, class XT>
+__device__ inline void gather_row_real_block(const OpArgs &o, const BondArgs &bd,
+                                             const RatioArgs &rt, const GenericOrbitArgs &ga,
+                                             const RingOrbitArgs &ra, const IndexArgs &ix,
+                                             const uint32_t *__restrict__ stab, uint64_t i,
+                                             const XT *__restrict__ x, double (&acc)[K]) {
+  using Row = RowK<XT, K>;
+  const Row *__restrict__ xr = reinterpret_cast<const Row *>(x);
+  const uint64_t r = ix.reps[i];
+  uint64_t act[NW];
+  const double diag = row_pass_a<NW, BONDS>(o, bd, r, act);
+  {
+    const Row xi = xr[i];
+#pragma unroll
+    for (int c = 0; c < K; ++c) acc[c] = diag * static_cast<double>(xi.v[c]);
+  }
+  const uint32_t si = stab[i];
+  const double sr = static_cast<double>(si);
+  const int row_di = rt.nd ? static_cast<int>(rt.div_idx[si]) : 0;
+  auto coef_of = [&](int j, const OrbitResult &orb) {
+    const double f = rt.nd ? rt.ratio[rt.div_idx[orb.count] * rt.nd + row_di]
+                           : sqrt(static_cast<double>(orb.count) / sr);
+    double c = (BONDS ? bd.o[j] : o.coef_re[j]) * f;
+    return orb.phase != 0 ? -c : c;
+  };
+  while (any_set<NW>(act)) {
+    const int ja = pop_first<NW>(act);
+    const int jb = pop_first<NW>(act);
+    const OrbitResult oa = do_orbit<METHOD>(ga, ra, r ^ (BONDS ? bd.flip[ja] : o.flip[ja]));
+    OrbitResult ob{0, 0, 0, false};
+    if (jb >= 0) ob = do_orbit<METHOD>(ga, ra, r ^ (BONDS ? bd.flip[jb] : o.flip[jb]));
+    int64_t ka, kb;
+    find_index2(ix, oa.rep, oa.valid, ob.rep, ob.valid, ka, kb);
+    Row xa{}, xb{};
+    if (ka >= 0) xa = xr[ka];
+    if (kb >= 0) xb = xr[kb];
+    if (ka >= 0) {
+      const double ca = coef_of(ja, oa);
+#pragma unroll
+      for (int c = 0; c < K; ++c) acc[c] = fma(ca, static_cast<double>(xa.v[c]), acc[c]);
+    }
+    if (kb >= 0) {
+      const double cb = coef_of(jb, ob);
+#pragma unroll
+      for (int c = 0; c < K; ++c) acc[c] = fma(cb, static_cast<double>(xb.v[c]), acc[c]);
+    }
+  }
 }
 
 } // namespace spined

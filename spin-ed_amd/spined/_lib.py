@@ -30,7 +30,7 @@ LS_SYMBOLS = (
 SPINED_SYMBOLS = (
     "spined_eigsh_options_init", "spined_eigsh", "spined_get_last_stats", "spined_stream",
     "spined_synchronize", "spined_basis_device_states", "spined_basis_get_stabilizers",
-    "spined_operator_matmat_device",
+    "spined_operator_matmat_device", "spined_operator_matmat_rows_device",
     "spined_operator_count_offdiag", "spined_operator_orbit_method",
     "spined_operator_set_orbit_method", "spined_basis_set_bucket_bits",
     "spined_basis_get_bucket_bits", "spined_basis_sector_dimension", "spined_debug_orbit",
@@ -39,6 +39,7 @@ SPINED_SYMBOLS = (
     "spined_cnn_create", "spined_cnn_destroy", "spined_cnn_get_params", "spined_cnn_set_params",
     "spined_cnn_forward_device", "spined_cnn_backward_device", "spined_cnn_adam_step",
     "spined_cnn_start_vector", "spined_debug_mfma_probe", "spined_debug_copy_device",
+    "spined_batch_eigsh",
 )
 
 
@@ -138,6 +139,7 @@ def load():
         "spined_basis_device_states": (I, [P, PP, PP, C.POINTER(U64)]),
         "spined_basis_get_stabilizers": (I, [P, P]),
         "spined_operator_matmat_device": (I, [P, I, U64, U64, P, U64, P, U64, I]),
+        "spined_operator_matmat_rows_device": (I, [P, I, U64, U64, P, P, I]),
         "spined_operator_count_offdiag": (I, [P, C.POINTER(U64)]),
         "spined_operator_orbit_method": (I, [P]),
         "spined_operator_set_orbit_method": (I, [P, I]),

@@ -325,6 +325,17 @@ def matvec_device(op: Operator_, x, y, dtype_code: int, y_is_fp64: bool = False,
                                                            size, yp, size, int(y_is_fp64)))
 
 
+def matvec_rows_device(op: Operator_, x, y, dtype_code: int, block: int,
+                       y_is_fp64: bool = False) -> None:
+    """Y <- H X for a row-interleaved device block (element (row, c) at
+    row * block + c), block in {1, 2, 4, 8}; enqueued on the library stream."""
+    xp = x if isinstance(x, int) else x.data_ptr()
+    yp = y if isinstance(y, int) else y.data_ptr()
+    size = get_number_states(op.basis)
+    check_status(_lib.load().spined_operator_matmat_rows_device(op.ptr, dtype_code, size, block,
+                                                                xp, yp, int(y_is_fp64)))
+
+
 def synchronize() -> None:
     check_status(_lib.load().spined_synchronize())
 

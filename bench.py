@@ -68,6 +68,10 @@ def parse():
                    help="profiling mode: run this many FP64 matvecs instead of solves")
     p.add_argument("--block", type=int, default=1, choices=(1, 2, 4, 8),
                    help="matvecs-only mode: row-interleaved block width")
+    p.add_argument("--index", default="auto", choices=("auto", "bucket", "hash"),
+                   help="index of representatives (spined_basis_set_index_kind)")
+    p.add_argument("--solve-block", type=int, default=0, choices=(0, 2, 4, 8),
+                   help="block width of the solver's first phase (max_block_size); 0 = one-vector")
     p.add_argument("--block-precision", default="fp64", choices=("fp64", "mixed"),
                    help="matvecs-only block mode: FP64 X, or FP32 X with FP64 Y")
     return p.parse_args()
@@ -101,6 +105,7 @@ def main():
                             n // 2, 1)
     op = spined.mk_operator(basis, [spined.mk_interaction(lattice.HEISENBERG,
                                                           lattice.ring_bonds(n))])
+    spined.set_index_kind(basis, {"auto": -1, "bucket": 0, "hash": 1}[a.index])
     spined.build_basis(basis)
     device_synchronize()
     t_build = time.perf_counter() - t0
@@ -137,7 +142,7 @@ def main():
             spined.matvec_rows_device(op, xb, yb, code, K, y_is_fp64=mixed)
         spined.synchronize()
         dt = (time.perf_counter() - t) / a.matvecs_only
-        print(json.dumps({"mode": "matvecs-only", "config": desc, "dimension": D, "block": K,
+        print(json.dumps({"mode": "matvecs-only", "config": desc, "dimension": D, "block": K, "index": a.index,
                           "block_precision": a.block_precision, "launches": a.matvecs_only,
                           "ms_per_launch": dt * 1e3, "ms_per_vector": dt * 1e3 / K,
                           "offdiag_gathers_per_launch": e_off}))
@@ -148,7 +153,7 @@ def main():
             spined.matvec_device(op, x, y, 1)
         spined.synchronize()
         dt = (time.perf_counter() - t) / a.matvecs_only
-        print(json.dumps({"mode": "matvecs-only", "config": desc, "dimension": D,
+        print(json.dumps({"mode": "matvecs-only", "config": desc, "dimension": D, "index": a.index,
                           "matvecs": a.matvecs_only, "ms_per_matvec": dt * 1e3,
                           "offdiag_gathers_per_launch": e_off}))
         return 0
@@ -173,6 +178,7 @@ def main():
     def solve(precision=a.precision, use_model=True):
         return spined.eigsh(op, k=1, tol=a.tol, precision=precision, seed=0x5EED,
                             want_vectors=False, max_basis_size=a.basis_size, verbose=True,
+                            max_block_size=a.solve_block if use_model else 0,
                             model=model if use_model else None,
                             sign_mask=sum(1 << i for i in range(0, n, 2)))
 

@@ -42,7 +42,10 @@ typedef struct spined_eigsh_options {
   uint64_t seed;
   int32_t max_basis_size;   /* Lanczos basis m (max_primme_basis_size); 0 = auto */
   int32_t min_restart_size; /* vectors kept on restart (min_primme_restart_size); 0 = auto */
-  int32_t max_block_size;   /* max_primme_block_size; accepted, block solver not used */
+  int32_t max_block_size;   /* max_primme_block_size: 0/1 = one-vector recurrence;
+                               2, 4, 8 = block Lanczos phase on row-interleaved
+                               blocks of that width (block matvec), then the
+                               one-vector FP64 phase finishes */
   int32_t verbose;          /* 1 = one log line per restart on stderr */
   int64_t max_matvecs;      /* 0 = auto */
   const double *start_vector;
@@ -134,6 +137,12 @@ int spined_operator_set_orbit_method(ls_operator *op, int method);
 /* Tunables picked by the autotuner: bucket bits of the index table. */
 int spined_basis_set_bucket_bits(ls_spin_basis *basis, int bits);
 int spined_basis_get_bucket_bits(const ls_spin_basis *basis);
+/* Index of representatives used by every lookup: 0 = bucket table on the
+ * top bits + binary search; 1 = open-addressing hash table (2^H slots of
+ * 8 bytes, H = ceil(log2 D) + 1, one slot line + one confirming load per
+ * lookup); -1 = automatic.  Setting it on a built basis rebuilds the index. */
+int spined_basis_set_index_kind(ls_spin_basis *basis, int kind);
+int spined_basis_get_index_kind(const ls_spin_basis *basis);
 /* Exact sector dimension by character counting (before ls_build). */
 int spined_basis_sector_dimension(const ls_spin_basis *basis, double *out);
 

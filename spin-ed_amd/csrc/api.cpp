@@ -634,6 +634,22 @@ int spined_basis_get_bucket_bits(const ls_spin_basis *basis) {
   return basis ? basis->p->bucket_bits : -1;
 }
 
+int spined_basis_set_index_kind(ls_spin_basis *basis, int kind) {
+  return guarded([&] {
+    if (!basis || kind < -1 || kind > 1) throw Error(LS_INVALID_ARGUMENT, "bad argument");
+    Basis &b = *basis->p;
+    std::lock_guard<std::mutex> lk(b.mu);
+    b.index_kind_request = kind;
+    if (b.built) build_index(b);
+  });
+}
+
+int spined_basis_get_index_kind(const ls_spin_basis *basis) {
+  if (!basis) return -1;
+  const Basis &b = *basis->p;
+  return b.built ? b.index_kind : (b.index_kind_request >= 0 ? b.index_kind_request : 0);
+}
+
 int spined_basis_sector_dimension(const ls_spin_basis *basis, double *out) {
   return guarded([&] {
     if (!basis || !out) throw Error(LS_INVALID_ARGUMENT, "NULL argument");

@@ -192,6 +192,31 @@ __global__ void __launch_bounds__(kBlock)
   buckets[b] = static_cast<uint32_t>(lo);
 }
 
+__global__ void __launch_bounds__(kBlock) fill_slots_kernel(uint64_t *__restrict__ p, uint64_t n) {
+  for (uint64_t i = blockIdx.x * static_cast<uint64_t>(kBlock) + threadIdx.x; i < n;
+       i += static_cast<uint64_t>(gridDim.x) * kBlock)
+    p[i] = kEmptySlot;
+}
+
+// Linear-probing insert of every representative (64-bit compare-and-swap on
+// the slot word).  The load factor is <= 1/2, so every probe meets a free
+// slot; the slot order depends on the CAS order, the lookups do not.
+__global__ void __launch_bounds__(kBlock)
+    hash_insert_kernel(const uint64_t *__restrict__ reps, uint64_t D, uint64_t *slots,
+                       uint64_t mask) {
+  const uint64_t i = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
+  if (i >= D) return;
+  const uint64_t h = mix64(reps[i]);
+  const unsigned long long e = ((h >> 32) << 32) | i;
+  uint64_t s = h & mask;
+  for (uint64_t step = 0; step <= mask; ++step) {
+    const unsigned long long old =
+        atomicCAS(reinterpret_cast<unsigned long long *>(slots + s), kEmptySlot, e);
+    if (old == kEmptySlot) return;
+    s = (s + 1) & mask;
+  }
+}
+
 uint64_t binomial(int n, int k) {
   if (k < 0 || k > n) return 0;
   static uint64_t table[65][65];
@@ -221,10 +246,39 @@ void compute_stabilizers(Basis &b, hipStream_t s) {
 
 } // namespace
 
+// Open-addressing table with 2^H slots, H = ceil(log2 D) + 1.
+void build_hash_index(Basis &b, hipStream_t s) {
+  int H = 1;
+  while ((1ull << H) < 2 * b.D) ++H;
+  const uint64_t nslots = 1ull << H;
+  b.slots.release();
+  b.slots.allocate(nslots);
+  b.hmask = nslots - 1;
+  fill_slots_kernel<<<static_cast<unsigned>(std::min<uint64_t>(grid_for(nslots), 1u << 16)), kBlock,
+                      0, s>>>(b.slots.ptr, nslots);
+  SPINED_CHECK_LAUNCH();
+  if (b.D) {
+    hash_insert_kernel<<<grid_for(b.D), kBlock, 0, s>>>(b.reps.ptr, b.D, b.slots.ptr, b.hmask);
+    SPINED_CHECK_LAUNCH();
+  }
+  SPINED_HIP(hipStreamSynchronize(s));
+}
+
 void build_index(Basis &b) {
   hipStream_t s = stream();
   if (b.D >= 0xffffffffull)
     throw Error(LS_NOT_IMPLEMENTED, "dimensions of 2^32 - 1 or more are not supported");
+  b.index_kind = b.index_kind_request >= 0 ? b.index_kind_request : 0;
+  if (b.index_kind == 1) {
+    b.buckets.release();
+    b.nbuckets = 0;
+    b.shift = 0;
+    b.bucket_bits = 0;
+    build_hash_index(b, s);
+    return;
+  }
+  b.slots.release();
+  b.hmask = 0;
   int bits = b.bucket_bits_request;
   if (bits < 0) {
     int lg = 0;

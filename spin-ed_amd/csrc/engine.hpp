@@ -94,11 +94,17 @@ struct Basis {
   int bucket_bits = 0;
   int shift = 0;
   uint64_t nbuckets = 0;
+  // index form: 0 = bucket table + binary search, 1 = hash table
+  int index_kind_request = -1; // -1: automatic
+  int index_kind = 0;
+  DeviceBuffer<uint64_t> slots;
+  uint64_t hmask = 0;
   std::shared_ptr<std::vector<uint64_t>> host_reps; // lazily downloaded
   double seconds_build = 0.0;
 
   IndexArgs index_args() const {
-    return IndexArgs{reps.ptr, buckets.ptr, nbuckets, shift};
+    return IndexArgs{reps.ptr, buckets.ptr, nbuckets, shift, index_kind == 1 ? slots.ptr : nullptr,
+                     hmask};
   }
 };
 

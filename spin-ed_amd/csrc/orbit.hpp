@@ -268,19 +268,71 @@ __host__ __device__ inline OrbitResult orbit_ring(const RingOrbitArgs &a, uint64
   return OrbitResult{best, ph, cnt, ok};
 }
 
+// SplitMix64 finaliser (Steele, Lea & Flood 2014): the counter hash of the
+// random start vectors and the key hash of the hash index.
+__host__ __device__ inline uint64_t mix64(uint64_t z) {
+  z += 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
 // ---------------------------------------------------------------------------
-// Index of a representative: bucket table on the top bits plus a branchless
-// binary search inside the bucket.  buckets[b] = first i with
-// reps[i] >> shift >= b, for b in [0, nbuckets]; buckets[nbuckets] = D.
+// Index of a representative.  Two forms:
+//  * bucket table on the top bits plus a branchless binary search inside the
+//    bucket: buckets[b] = first i with reps[i] >> shift >= b, for b in
+//    [0, nbuckets]; buckets[nbuckets] = D.  Representatives cluster (orbit
+//    minima start with long zero runs), so buckets hold tens of entries.
+//  * open-addressing hash table (slots != nullptr): 2^H 8-byte slots, H =
+//    ceil(log2 D) + 1 (load <= 1/2), linear probing from the low H bits of
+//    h = mix64(rep); a slot holds (h >> 32) << 32 | index, or kEmptySlot.  A
+//    lookup reads one slot line for the fingerprint, then reps[index] to
+//    confirm, which the row gather issues together with the x gather.
 // ---------------------------------------------------------------------------
+constexpr uint64_t kEmptySlot = ~0ull;
+
 struct IndexArgs {
   const uint64_t *__restrict__ reps;
   const uint32_t *__restrict__ buckets;
   uint64_t nbuckets;
   int32_t shift;
+  const uint64_t *__restrict__ slots;
+  uint64_t hmask;
 };
 
+// First slot index whose fingerprint matches v (its stored index is a
+// candidate, to be confirmed with reps), or -1 when the probe meets an empty
+// slot.  *pos is the slot where the probe stopped.
+__host__ __device__ inline int64_t hash_probe_from(const IndexArgs &t, uint64_t v, uint64_t s,
+                                                   uint64_t *pos) {
+  const uint32_t fp = static_cast<uint32_t>(mix64(v) >> 32);
+  for (uint64_t step = 0; step <= t.hmask; ++step) {
+    const uint64_t e = t.slots[s];
+    if (e == kEmptySlot) break;
+    if (static_cast<uint32_t>(e >> 32) == fp) {
+      *pos = s;
+      return static_cast<int64_t>(e & 0xffffffffull);
+    }
+    s = (s + 1) & t.hmask;
+  }
+  *pos = s;
+  return -1;
+}
+
+__host__ __device__ inline int64_t hash_probe(const IndexArgs &t, uint64_t v, uint64_t *pos) {
+  return hash_probe_from(t, v, mix64(v) & t.hmask, pos);
+}
+
+// Exact lookup: continue past fingerprint matches that are not v.
+__host__ __device__ inline int64_t hash_find(const IndexArgs &t, uint64_t v) {
+  uint64_t pos = 0;
+  int64_t k = hash_probe(t, v, &pos);
+  while (k >= 0 && t.reps[k] != v) k = hash_probe_from(t, v, (pos + 1) & t.hmask, &pos);
+  return k;
+}
+
 __host__ __device__ inline int64_t find_index(const IndexArgs &t, uint64_t v) {
+  if (t.slots) return hash_find(t, v);
   const uint64_t b = v >> t.shift;
   if (b >= t.nbuckets) return -1;
   uint32_t lo = t.buckets[b];
@@ -296,9 +348,19 @@ __host__ __device__ inline int64_t find_index(const IndexArgs &t, uint64_t v) {
 
 // Two independent searches advanced in lockstep, so a lane keeps two
 // dependent load chains in flight instead of one.  use_a / use_b switch a
-// search off (its result is then -1).
+// search off (its result is then -1).  Exact for both index forms.
 __host__ __device__ inline void find_index2(const IndexArgs &t, uint64_t va, bool use_a,
                                             uint64_t vb, bool use_b, int64_t &ka, int64_t &kb) {
+  if (t.slots) {
+    uint64_t pa = 0, pb = 0;
+    ka = use_a ? hash_probe(t, va, &pa) : -1;
+    kb = use_b ? hash_probe(t, vb, &pb) : -1;
+    const bool bad_a = ka >= 0 && t.reps[ka] != va;
+    const bool bad_b = kb >= 0 && t.reps[kb] != vb;
+    if (bad_a) ka = hash_find(t, va);
+    if (bad_b) kb = hash_find(t, vb);
+    return;
+  }
   const uint64_t ba = va >> t.shift, bb = vb >> t.shift;
   use_a = use_a && ba < t.nbuckets;
   use_b = use_b && bb < t.nbuckets;
@@ -327,15 +389,8 @@ __host__ __device__ inline void find_index2(const IndexArgs &t, uint64_t va, boo
   kb = (nb == 1 && t.reps[lob] == vb) ? static_cast<int64_t>(lob) : -1;
 }
 
-// Counter-hash start vector (SplitMix64 finaliser, Steele, Lea & Flood
-// 2014): v[r] = U(-1, 1) from (seed, r); identical on host and device.
-__host__ __device__ inline uint64_t mix64(uint64_t z) {
-  z += 0x9E3779B97F4A7C15ull;
-  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
-  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
-  return z ^ (z >> 31);
-}
-
+// Counter-hash start vector: v[r] = U(-1, 1) from (seed, r); identical on
+// host and device.
 __host__ __device__ inline double hash_uniform(uint64_t seed, uint64_t r) {
   const uint64_t h = mix64(seed ^ mix64(r));
   return 2.0 * (static_cast<double>(h >> 11) * 0x1.0p-53) - 1.0;

@@ -122,6 +122,44 @@ struct RatioArgs {
   int32_t nd;
 };
 
+// Index lookups of two orbit representatives and the gathers of their x
+// values (ld: index -> value; V{} when absent).  With the hash index the
+// confirming reps[k] loads are issued together with the x gathers; a
+// fingerprint false positive (rare) falls back to the exact lookup.
+template <class V, class LoadX>
+__device__ inline void lookup_gather2(const IndexArgs &ix, const OrbitResult &oa,
+                                      const OrbitResult &ob, LoadX ld, int64_t &ka, int64_t &kb,
+                                      V &xa, V &xb) {
+  xa = V{};
+  xb = V{};
+  if (ix.slots) {
+    uint64_t pa = 0, pb = 0;
+    ka = oa.valid ? hash_probe(ix, oa.rep, &pa) : -1;
+    kb = ob.valid ? hash_probe(ix, ob.rep, &pb) : -1;
+    uint64_t ra = oa.rep, rb = ob.rep;
+    if (ka >= 0) {
+      ra = ix.reps[ka];
+      xa = ld(ka);
+    }
+    if (kb >= 0) {
+      rb = ix.reps[kb];
+      xb = ld(kb);
+    }
+    if (ra != oa.rep) {
+      ka = hash_find(ix, oa.rep);
+      xa = ka >= 0 ? ld(ka) : V{};
+    }
+    if (rb != ob.rep) {
+      kb = hash_find(ix, ob.rep);
+      xb = kb >= 0 ? ld(kb) : V{};
+    }
+    return;
+  }
+  find_index2(ix, oa.rep, oa.valid, ob.rep, ob.valid, ka, kb);
+  if (ka >= 0) xa = ld(ka);
+  if (kb >= 0) xb = ld(kb);
+}
+
 // Pass A of a row: the diagonal and the active off-diagonal templates (in
 // exchange-bond form: the antiparallel bonds), one bit per template.
 template <int NW, bool BONDS>
@@ -185,10 +223,9 @@ __device__ inline double gather_row_real(const OpArgs &o, const BondArgs &bd, co
     OrbitResult ob{0, 0, 0, false};
     if (jb >= 0) ob = do_orbit<METHOD>(ga, ra, r ^ (BONDS ? bd.flip[jb] : o.flip[jb]));
     int64_t ka, kb;
-    find_index2(ix, oa.rep, oa.valid, ob.rep, ob.valid, ka, kb);
-    double xa = 0.0, xb = 0.0;
-    if (ka >= 0) xa = ld_re(x, static_cast<uint64_t>(ka));
-    if (kb >= 0) xb = ld_re(x, static_cast<uint64_t>(kb));
+    double xa, xb;
+    lookup_gather2(ix, oa, ob, [&](int64_t q) { return ld_re(x, static_cast<uint64_t>(q)); }, ka,
+                   kb, xa, xb);
     if (ka >= 0) acc = fma(coef_of(ja, oa), xa, acc);
     if (kb >= 0) acc = fma(coef_of(jb, ob), xb, acc);
   }
@@ -238,10 +275,8 @@ __device__ inline void gather_row_real_block(const OpArgs &o, const BondArgs &bd
     OrbitResult ob{0, 0, 0, false};
     if (jb >= 0) ob = do_orbit<METHOD>(ga, ra, r ^ (BONDS ? bd.flip[jb] : o.flip[jb]));
     int64_t ka, kb;
-    find_index2(ix, oa.rep, oa.valid, ob.rep, ob.valid, ka, kb);
-    Row xa{}, xb{};
-    if (ka >= 0) xa = xr[ka];
-    if (kb >= 0) xb = xr[kb];
+    Row xa, xb;
+    lookup_gather2(ix, oa, ob, [&](int64_t q) { return xr[q]; }, ka, kb, xa, xb);
     if (ka >= 0) {
       const double ca = coef_of(ja, oa);
 #pragma unroll

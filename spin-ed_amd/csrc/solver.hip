@@ -25,6 +25,7 @@
 
 #include "engine.hpp"
 #include "jacobi.hpp"
+#include "rowgather.hpp"
 
 namespace spined {
 
@@ -448,6 +449,14 @@ template <class VT> struct Phase {
   }
 };
 
+} // namespace
+} // namespace spined
+
+#include "blocklanczos.hpp"
+
+namespace spined {
+namespace {
+
 void pick_sizes(uint64_t n, uint64_t k, const spined_eigsh_options &o, int &m, int &keep) {
   int mm = o.max_basis_size > 0 ? o.max_basis_size : 0;
   if (mm <= 0) mm = static_cast<int>(std::max<uint64_t>(2 * k + 10, 24));
@@ -459,6 +468,83 @@ void pick_sizes(uint64_t n, uint64_t k, const spined_eigsh_options &o, int &m, i
   kp = std::max(kp, static_cast<int>(k));
   kp = std::min(kp, m - 1);
   keep = std::max(kp, 0);
+}
+
+// Block width from max_block_size (PRIMME's maxBlockSize): 1 (one-vector
+// recurrence), 2, 4 or 8.
+int block_width(int32_t req) { return req >= 8 ? 8 : req >= 4 ? 4 : req >= 2 ? 2 : 1; }
+
+// Blocks per cycle and kept blocks: M = mb*K <= 64 vectors, about half kept
+// (at least ceil(k/K) blocks, at most 32 vectors).
+void pick_block_sizes(uint64_t k, int K, const spined_eigsh_options &o, int &mb, int &keepb) {
+  int M = o.max_basis_size > 0 ? o.max_basis_size : 4 * K;
+  M = std::min(std::max(M, 3 * K), kMaxBasis);
+  mb = std::max(3, M / K);
+  while (mb * K > kMaxBasis) --mb;
+  const int need = static_cast<int>((k + K - 1) / K);
+  keepb = o.min_restart_size > 0 ? (o.min_restart_size + K - 1) / K : mb / 2;
+  keepb = std::max(keepb, need);
+  keepb = std::min(keepb, kMaxKeep / K);
+  keepb = std::min(keepb, mb - 1);
+}
+
+struct BlockOutcome {
+  bool ran = false;       // false: rank deficient, start left unchanged
+  int64_t matvecs = 0;
+  double matvec_seconds = 0.0;
+  int cycles = 0;
+  bool converged = false;
+};
+
+// Block phase with storage VT; on success start <- sum of the k lowest Ritz
+// vectors (FP64).
+template <class VT, int K>
+BlockOutcome run_block_phase(Operator &op, uint64_t n, uint64_t k, double tol, bool stall_detect,
+                             int phase_id, const spined_eigsh_options &opts, double *start,
+                             double *y, spined_monitor_fn monitor, void *ctx, hipStream_t s) {
+  BlockOutcome out;
+  int mb = 0, keepb = 0;
+  pick_block_sizes(k, K, opts, mb, keepb);
+  BlockPhase<VT, K> p(op, n, mb, static_cast<int>(k), keepb, s);
+  bool deficient = !p.set_start(start, opts.seed ^ 0xB10Cull);
+  PhaseResult r;
+  if (!deficient)
+    r = p.run(tol, opts.max_matvecs, stall_detect, phase_id, monitor, ctx, opts.verbose,
+              deficient);
+  out.matvecs = p.matvecs;
+  out.matvec_seconds = p.matvec_seconds;
+  out.cycles = r.cycles;
+  if (deficient) {
+    if (opts.verbose) log_line("block phase: rank-deficient block, one-vector recurrence instead");
+    return out;
+  }
+  SPINED_HIP(hipMemsetAsync(start, 0, n * 8, s));
+  for (uint64_t i = 0; i < k; ++i) {
+    p.ritz(static_cast<int>(i), y);
+    residual_kernel<<<grid_for(n), kBlock, 0, s>>>(start, y, -1.0, start, n);
+    SPINED_CHECK_LAUNCH();
+  }
+  SPINED_HIP(hipStreamSynchronize(s));
+  out.ran = true;
+  out.converged = r.converged;
+  return out;
+}
+
+template <class VT>
+BlockOutcome run_block(int K, Operator &op, uint64_t n, uint64_t k, double tol, bool stall_detect,
+                       int phase_id, const spined_eigsh_options &opts, double *start, double *y,
+                       spined_monitor_fn monitor, void *ctx, hipStream_t s) {
+  switch (K) {
+  case 2:
+    return run_block_phase<VT, 2>(op, n, k, tol, stall_detect, phase_id, opts, start, y, monitor,
+                                  ctx, s);
+  case 4:
+    return run_block_phase<VT, 4>(op, n, k, tol, stall_detect, phase_id, opts, start, y, monitor,
+                                  ctx, s);
+  default:
+    return run_block_phase<VT, 8>(op, n, k, tol, stall_detect, phase_id, opts, start, y, monitor,
+                                  ctx, s);
+  }
 }
 
 } // namespace
@@ -524,7 +610,26 @@ int eigsh(Operator &op, int dtype, uint64_t k, double tol, const spined_eigsh_op
   bool converged = false;
   DeviceBuffer<double> y(n), hy(n);
 
-  if (opts.precision == SPINED_PRECISION_MIXED && n > static_cast<uint64_t>(m)) {
+  // Block phase (max_block_size >= 2): FP32 storage in mixed mode, FP64
+  // otherwise; the one-vector FP64 phase below then finishes from the sum of
+  // its Ritz vectors (and owns the explicit residual check).
+  const int K = block_width(opts.max_block_size);
+  bool block_done = false;
+  if (K > 1 && matvec_block_supported(op) && n >= (1u << 14)) {
+    const bool mixed = opts.precision == SPINED_PRECISION_MIXED;
+    const BlockOutcome bo =
+        mixed ? run_block<float>(K, op, n, k, std::max(tol, 1e-5), true, 1, opts, start.ptr,
+                                 y.ptr, monitor, ctx, s)
+              : run_block<double>(K, op, n, k, tol, true, 0, opts, start.ptr, y.ptr, monitor,
+                                  ctx, s);
+    st.matvecs += bo.matvecs;
+    if (mixed) st.matvecs_mixed += bo.matvecs;
+    st.seconds_matvec += bo.matvec_seconds;
+    st.restarts += bo.cycles;
+    block_done = bo.ran;
+  }
+
+  if (!block_done && opts.precision == SPINED_PRECISION_MIXED && n > static_cast<uint64_t>(m)) {
     // FP32-storage phase down to its floor, then hand the FP64 Ritz vector
     // (sum of the k lowest) to the FP64 phase.
     Phase<float> p(op, n, m, static_cast<int>(k), keep, s);

@@ -33,7 +33,8 @@ SPINED_SYMBOLS = (
     "spined_operator_matmat_device", "spined_operator_matmat_rows_device",
     "spined_operator_count_offdiag", "spined_operator_orbit_method",
     "spined_operator_set_orbit_method", "spined_basis_set_bucket_bits",
-    "spined_basis_get_bucket_bits", "spined_basis_sector_dimension", "spined_debug_orbit",
+    "spined_basis_get_bucket_bits", "spined_basis_set_index_kind", "spined_basis_get_index_kind",
+    "spined_basis_sector_dimension", "spined_debug_orbit",
     "spined_debug_ring_available", "spined_device_synchronize", "spined_device_malloc",
     "spined_device_free", "spined_memcpy", "spined_memset", "spined_cnn_num_params",
     "spined_cnn_create", "spined_cnn_destroy", "spined_cnn_get_params", "spined_cnn_set_params",
@@ -145,6 +146,8 @@ def load():
         "spined_operator_set_orbit_method": (I, [P, I]),
         "spined_basis_set_bucket_bits": (I, [P, I]),
         "spined_basis_get_bucket_bits": (I, [P]),
+        "spined_basis_set_index_kind": (I, [P, I]),
+        "spined_basis_get_index_kind": (I, [P]),
         "spined_basis_sector_dimension": (I, [P, C.POINTER(D)]),
         "spined_debug_orbit": (I, [P, I, U64, P, P, P, P, P]),
         "spined_debug_ring_available": (I, [P]),

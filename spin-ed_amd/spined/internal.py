@@ -358,6 +358,15 @@ def orbit_method(op: Operator_) -> int:
     return int(_lib.load().spined_operator_orbit_method(op.ptr))
 
 
+def set_index_kind(basis: SpinBasis, kind: int) -> None:
+    """0 = bucket table + binary search, 1 = hash table, -1 = automatic."""
+    check_status(_lib.load().spined_basis_set_index_kind(basis.ptr, kind))
+
+
+def index_kind(basis: SpinBasis) -> int:
+    return int(_lib.load().spined_basis_get_index_kind(basis.ptr))
+
+
 def set_bucket_bits(basis: SpinBasis, bits: int) -> None:
     check_status(_lib.load().spined_basis_set_bucket_bits(basis.ptr, bits))
 

@@ -116,3 +116,45 @@ def test_rows_device_rejects_bad_width():
     yd = DeviceArray.zeros(D * 3, np.float64)
     with pytest.raises(Exception):
         spined.matvec_rows_device(op, xd, yd, F64, 3)
+
+
+# ---- block Lanczos phase (max_block_size >= 2) ---------------------------------
+RING24_E0 = -42.680058066148944  # Bethe ansatz, N = 24 Heisenberg ring
+
+
+@pytest.mark.parametrize("K", [2, 4, 8])
+def test_block_eigsh_fp64_ring24(K):
+    spec = ring(24)
+    basis, op = engine(spec)
+    r = spined.eigsh(op, k=1, tol=1e-10, max_block_size=K)
+    print("K", K, r.evals, r.rnorms, r.stats, flush=True)
+    assert r.status == 0
+    assert abs(r.evals[0] - RING24_E0) <= 1e-10 * abs(RING24_E0)
+    assert r.rnorms[0] <= 1e-10
+    o = oracle(spec)
+    ref = o.lanczos(k=1, tol=1e-11)
+    assert abs(np.dot(r.evecs[:, 0], ref["evecs"][:, 0])) >= 1 - 1e-8
+
+
+def test_block_eigsh_lowest4_ring24():
+    spec = ring(24)
+    basis, op = engine(spec)
+    r = spined.eigsh(op, k=4, tol=1e-10, max_block_size=4)
+    o = oracle(spec)
+    ref = o.lanczos(k=4, tol=1e-11, max_iter=400)
+    print(r.evals, r.rnorms, r.stats, flush=True)
+    assert r.status == 0
+    np.testing.assert_allclose(r.evals, ref["evals"], rtol=1e-10, atol=0)
+    assert np.all(r.rnorms <= 1e-10)
+
+
+def test_block_eigsh_mixed_ring26():
+    spec = ring(26, k=13, p=1, z=-1)   # N/2 odd: ground state at momentum pi
+    basis, op = engine(spec)
+    r = spined.eigsh(op, k=1, tol=1e-10, precision="mixed", max_block_size=8)
+    ref = spined.eigsh(op, k=1, tol=1e-10)
+    print(r.evals, r.rnorms, r.stats, ref.evals, flush=True)
+    assert r.status == 0 and ref.status == 0
+    assert abs(r.evals[0] - ref.evals[0]) <= 1e-10 * abs(ref.evals[0])
+    assert r.rnorms[0] <= 1e-10
+    assert r.stats["matvecs_mixed"] > 0

@@ -1,0 +1,72 @@
+"""Hash index of representatives (spined_basis_set_index_kind(basis, 1))
+against the bucket index: every lookup is exact in both forms, so matvecs
+(single, block, complex), off-diagonal counts and eigenpairs are
+bit-identical."""
+import numpy as np
+import pytest
+
+import spined
+from spined import lattice
+from spined.device import DeviceArray
+from helpers import Spec, engine, oracle, ring
+
+pytestmark = pytest.mark.gpu
+
+CASES = [
+    ring(10, k=5, p=1, z=-1, name="chain10_example"),
+    ring(20, k=0, p=0, z=1),
+    ring(20, k=3, p=None, z=None, name="ring20_k3_complex"),
+    ring(18, k=9, p=1, z=-1),
+    ring(14, k=None, p=None, z=1, name="xxz14_flip"),
+    Spec(16, 8, 1, lattice.square_symmetries(4), [(lattice.HEISENBERG, lattice.square_bonds(4))],
+         "square4x4"),
+]
+
+
+@pytest.mark.parametrize("spec", CASES, ids=lambda s: s.name)
+def test_hash_index_matvec_bit_identical(spec):
+    print("case", spec.name, flush=True)
+    basis, op = engine(spec)
+    o = oracle(spec)
+    D = o.dimension
+    rng = np.random.default_rng(21)
+    x = rng.uniform(-1, 1, D)
+    if not o.is_real:
+        x = x + 1j * rng.uniform(-1, 1, D)
+    assert spined.index_kind(basis) == 0
+    y0 = spined.apply(op, x)
+    c0 = spined.count_offdiag(op)
+    spined.set_index_kind(basis, 1)
+    assert spined.index_kind(basis) == 1
+    y1 = spined.apply(op, x)
+    c1 = spined.count_offdiag(op)
+    np.testing.assert_array_equal(y0, y1)
+    assert c0 == c1
+    yr = o.matvec(x)
+    assert np.max(np.abs(y1 - yr)) <= 1e-13 * np.max(np.abs(yr))
+    if o.is_real:
+        K = 4
+        X = rng.uniform(-1, 1, (D, K))
+        xd = DeviceArray.from_numpy(np.ascontiguousarray(X))
+        yd = DeviceArray.zeros(D * K, np.float64)
+        spined.matvec_rows_device(op, xd, yd, 1, K)
+        spined.synchronize()
+        Y = yd.numpy().reshape(D, K)
+        for c in range(K):
+            np.testing.assert_array_equal(Y[:, c], spined.apply(op, np.ascontiguousarray(X[:, c])))
+    spined.set_index_kind(basis, 0)
+    np.testing.assert_array_equal(spined.apply(op, x), y0)
+
+
+def test_hash_index_before_build_and_eigsh():
+    spec = ring(24)
+    syms = [spined.mk_symmetry(p, s) for p, s in spec.gens]
+    basis = spined.mk_basis(spined.mk_group(syms), spec.n, spec.w, spec.z)
+    spined.set_index_kind(basis, 1)
+    spined.build_basis(basis)
+    assert spined.index_kind(basis) == 1
+    op = spined.mk_operator(basis, [spined.mk_interaction(m, s) for m, s in spec.terms])
+    r = spined.eigsh(op, k=1, tol=1e-10)
+    assert r.status == 0
+    assert abs(r.evals[0] - (-42.680058066148944)) <= 1e-10 * 42.68
+    assert r.rnorms[0] <= 1e-10

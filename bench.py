@@ -72,8 +72,8 @@ def parse():
                    help="index of representatives (spined_basis_set_index_kind)")
     p.add_argument("--solve-block", type=int, default=0, choices=(0, 2, 4, 8),
                    help="block width of the solver's first phase (max_block_size); 0 = one-vector")
-    p.add_argument("--block-precision", default="fp64", choices=("fp64", "mixed"),
-                   help="matvecs-only block mode: FP64 X, or FP32 X with FP64 Y")
+    p.add_argument("--mv-precision", default="fp64", choices=("fp64", "mixed"),
+                   help="matvecs-only mode: FP64 x, or FP32 x with FP64 y (the FP32-storage phase)")
     return p.parse_args()
 
 
@@ -131,7 +131,7 @@ def main():
         x.free()
         y.free()
         K = a.block
-        mixed = a.block_precision == "mixed"
+        mixed = a.mv_precision == "mixed"
         xb = DeviceArray.from_numpy(np.ones(D * K, np.float32 if mixed else np.float64))
         yb = DeviceArray(D * K, np.float64)
         code = 0 if mixed else 1
@@ -143,19 +143,26 @@ def main():
         spined.synchronize()
         dt = (time.perf_counter() - t) / a.matvecs_only
         print(json.dumps({"mode": "matvecs-only", "config": desc, "dimension": D, "block": K, "index": a.index,
-                          "block_precision": a.block_precision, "launches": a.matvecs_only,
+                          "mv_precision": a.mv_precision, "launches": a.matvecs_only,
                           "ms_per_launch": dt * 1e3, "ms_per_vector": dt * 1e3 / K,
                           "offdiag_gathers_per_launch": e_off}))
         return 0
     if a.matvecs_only:
+        mixed = a.mv_precision == "mixed"
+        if mixed:
+            x.free()
+            x = DeviceArray.from_numpy(np.ones(D, np.float32))
+        code = 0 if mixed else 1
+        spined.matvec_device(op, x, y, code, y_is_fp64=mixed)
+        spined.synchronize()
         t = time.perf_counter()
         for _ in range(a.matvecs_only):
-            spined.matvec_device(op, x, y, 1)
+            spined.matvec_device(op, x, y, code, y_is_fp64=mixed)
         spined.synchronize()
         dt = (time.perf_counter() - t) / a.matvecs_only
         print(json.dumps({"mode": "matvecs-only", "config": desc, "dimension": D, "index": a.index,
-                          "matvecs": a.matvecs_only, "ms_per_matvec": dt * 1e3,
-                          "offdiag_gathers_per_launch": e_off}))
+                          "mv_precision": a.mv_precision, "matvecs": a.matvecs_only,
+                          "ms_per_matvec": dt * 1e3, "offdiag_gathers_per_launch": e_off}))
         return 0
     x.free()
     y.free()
@@ -210,9 +217,16 @@ def main():
     bytes_mixed = D * (8 + 4 + 4 + 8) + e_off * 4
     alg_bytes = (bytes_fp64 * fp64_mv + bytes_mixed * mixed_mv) / max(matvecs, 1)
     achieved = alg_bytes / mv_avg if mv_avg > 0 else 0.0
+    # HBM bytes per matvec launch from the committed rocprofv3 FETCH_SIZE /
+    # WRITE_SIZE passes (calibrated against the plain copy kernel), weighted
+    # by this run's FP64 / FP32-storage matvec counts
     traffic = None
-    if a.traffic_json and Path(a.traffic_json).exists():
-        traffic = json.loads(Path(a.traffic_json).read_text()).get("bytes_per_launch")
+    tj = Path(a.traffic_json) if a.traffic_json else ROOT / "profiles" / f"matvec_traffic_{a.config}.json"
+    if tj.exists():
+        tdat = json.loads(tj.read_text())
+        per = {k: v["fetch_bytes"] + v["write_bytes"] for k, v in tdat.get("per_launch", {}).items()}
+        if "fp64" in per and "mixed" in per:
+            traffic = (per["fp64"] * fp64_mv + per["mixed"] * mixed_mv) / max(matvecs, 1)
 
     compare = None
     if not a.no_compare and (a.precision != "fp64" or a.start != "random"):

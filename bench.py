@@ -70,6 +70,8 @@ def parse():
                    help="matvecs-only mode: row-interleaved block width")
     p.add_argument("--index", default="auto", choices=("auto", "bucket", "hash"),
                    help="index of representatives (spined_basis_set_index_kind)")
+    p.add_argument("--bucket-bits", type=int, default=-1,
+                   help="bucket table bits of the bucket index (-1: automatic)")
     p.add_argument("--solve-block", type=int, default=0, choices=(0, 2, 4, 8),
                    help="block width of the solver's first phase (max_block_size); 0 = one-vector")
     p.add_argument("--mv-precision", default="fp64", choices=("fp64", "mixed"),
@@ -106,6 +108,7 @@ def main():
     op = spined.mk_operator(basis, [spined.mk_interaction(lattice.HEISENBERG,
                                                           lattice.ring_bonds(n))])
     spined.set_index_kind(basis, {"auto": -1, "bucket": 0, "hash": 1}[a.index])
+    spined.set_bucket_bits(basis, a.bucket_bits)
     spined.build_basis(basis)
     device_synchronize()
     t_build = time.perf_counter() - t0
@@ -161,6 +164,7 @@ def main():
         spined.synchronize()
         dt = (time.perf_counter() - t) / a.matvecs_only
         print(json.dumps({"mode": "matvecs-only", "config": desc, "dimension": D, "index": a.index,
+                          "bucket_bits": spined.bucket_bits(basis),
                           "mv_precision": a.mv_precision, "matvecs": a.matvecs_only,
                           "ms_per_matvec": dt * 1e3, "offdiag_gathers_per_launch": e_off}))
         return 0

@@ -134,7 +134,9 @@ int spined_operator_count_offdiag(const ls_operator *op, uint64_t *out);
 int spined_operator_orbit_method(const ls_operator *op);
 /* Force an orbit method (for parity tests): -1 = automatic. */
 int spined_operator_set_orbit_method(ls_operator *op, int method);
-/* Tunables picked by the autotuner: bucket bits of the index table. */
+/* Bucket bits of the bucket index (2^bits buckets over the representatives'
+ * range, 1..34; -1 = automatic: ceil(log2 D) - 1).  Setting it on a built
+ * basis rebuilds the index. */
 int spined_basis_set_bucket_bits(ls_spin_basis *basis, int bits);
 int spined_basis_get_bucket_bits(const ls_spin_basis *basis);
 /* Index of representatives used by every lookup: 0 = bucket table on the

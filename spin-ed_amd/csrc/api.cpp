@@ -622,7 +622,7 @@ int spined_operator_set_orbit_method(ls_operator *op, int method) {
 
 int spined_basis_set_bucket_bits(ls_spin_basis *basis, int bits) {
   return guarded([&] {
-    if (!basis || bits < -1 || bits > 30) throw Error(LS_INVALID_ARGUMENT, "bad argument");
+    if (!basis || bits < -1 || bits > 34) throw Error(LS_INVALID_ARGUMENT, "bad argument");
     Basis &b = *basis->p;
     std::lock_guard<std::mutex> lk(b.mu);
     b.bucket_bits_request = bits;

@@ -24,6 +24,7 @@ namespace spined {
 namespace {
 
 constexpr int kBlock = 256;
+constexpr int kSegmentBits = 16; // level-1 segments of the bucket index
 
 struct EnumArgs {
   const uint64_t *__restrict__ binom; // 65 x 65
@@ -166,30 +167,70 @@ __global__ void __launch_bounds__(kBlock)
   stab[i] = static_cast<uint32_t>(r.count);
 }
 
-// buckets[b] = lower_bound(reps, b << shift) for b < nbuckets, D at b =
-// nbuckets: one thread per bucket (the top levels of the searches are shared
-// and stay in cache).
+// seg[s] = lower_bound(reps, s << shift) for s < nseg, D at s = nseg
+// (grid-stride: nseg + 1 may exceed one launch's work-item range).
 __global__ void __launch_bounds__(kBlock)
-    bucket_kernel(const uint64_t *__restrict__ reps, uint64_t D, int shift, uint64_t nbuckets,
-                  uint32_t *__restrict__ buckets) {
-  const uint64_t b = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
-  if (b > nbuckets) return;
-  if (b == nbuckets) {
-    buckets[b] = static_cast<uint32_t>(D);
-    return;
-  }
-  const uint64_t key = b << shift;
-  uint64_t lo = 0, n = D;
-  while (n > 0) {
-    const uint64_t half = n >> 1;
-    if (reps[lo + half] < key) {
-      lo += half + 1;
-      n -= half + 1;
-    } else {
-      n = half;
+    segment_start_kernel(const uint64_t *__restrict__ reps, uint64_t D, int shift, uint64_t nseg,
+                         uint32_t *__restrict__ seg) {
+  for (uint64_t s = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x; s <= nseg;
+       s += static_cast<uint64_t>(gridDim.x) * kBlock) {
+    if (s == nseg) {
+      seg[s] = static_cast<uint32_t>(D);
+      continue;
     }
+    const uint64_t key = s << shift;
+    uint64_t lo = 0, n = D;
+    while (n > 0) {
+      const uint64_t half = n >> 1;
+      if (reps[lo + half] < key) {
+        lo += half + 1;
+        n -= half + 1;
+      } else {
+        n = half;
+      }
+    }
+    seg[s] = static_cast<uint32_t>(lo);
   }
-  buckets[b] = static_cast<uint32_t>(lo);
+}
+
+// Level-2 table: buckets[b] = first i in segment s with reps[i] >= the
+// bucket's start value (s << shift) + (local << lsh_s); D at b = nbuckets.
+// The segment of b is the last one whose base is <= b (bases ascend).
+__global__ void __launch_bounds__(kBlock)
+    level2_kernel(const uint64_t *__restrict__ reps, uint64_t D, const uint2 *__restrict__ dir,
+                  const uint32_t *__restrict__ seg, uint64_t nseg, int shift, uint64_t nbuckets,
+                  uint32_t *__restrict__ buckets) {
+  for (uint64_t b = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x; b <= nbuckets;
+       b += static_cast<uint64_t>(gridDim.x) * kBlock) {
+    if (b == nbuckets) {
+      buckets[b] = static_cast<uint32_t>(D);
+      continue;
+    }
+    uint64_t lo_s = 0, n_s = nseg; // upper_bound of b over bases, minus one
+    while (n_s > 0) {
+      const uint64_t half = n_s >> 1;
+      if (dir[lo_s + half].x <= b) {
+        lo_s += half + 1;
+        n_s -= half + 1;
+      } else {
+        n_s = half;
+      }
+    }
+    const uint64_t sg = lo_s - 1;
+    const uint2 d = dir[sg];
+    const uint64_t key = (sg << shift) + ((b - d.x) << d.y);
+    uint64_t lo = seg[sg], n = seg[sg + 1] - seg[sg];
+    while (n > 0) {
+      const uint64_t half = n >> 1;
+      if (reps[lo + half] < key) {
+        lo += half + 1;
+        n -= half + 1;
+      } else {
+        n = half;
+      }
+    }
+    buckets[b] = static_cast<uint32_t>(lo);
+  }
 }
 
 __global__ void __launch_bounds__(kBlock) fill_slots_kernel(uint64_t *__restrict__ p, uint64_t n) {
@@ -271,7 +312,9 @@ void build_index(Basis &b) {
   b.index_kind = b.index_kind_request >= 0 ? b.index_kind_request : 0;
   if (b.index_kind == 1) {
     b.buckets.release();
+    b.dir.release();
     b.nbuckets = 0;
+    b.nseg = 0;
     b.shift = 0;
     b.bucket_bits = 0;
     build_hash_index(b, s);
@@ -279,22 +322,52 @@ void build_index(Basis &b) {
   }
   b.slots.release();
   b.hmask = 0;
-  int bits = b.bucket_bits_request;
-  if (bits < 0) {
-    int lg = 0;
-    while ((1ull << lg) < b.D) ++lg;
-    bits = std::max(1, std::min(30, lg - 1));
-  }
-  bits = std::max(1, std::min(30, bits));
+  // level-2 size: about 2^bits buckets, bits = ceil(log2 D) + 1 by default
+  // (two to four buckets per representative; about one binary-search step)
+  int lg = 0;
+  while ((1ull << lg) < b.D) ++lg;
+  int bits = b.bucket_bits_request >= 0 ? b.bucket_bits_request : lg + 1;
+  bits = std::max(1, std::min(34, bits));
   uint64_t last = 0;
   if (b.D) SPINED_HIP(hipMemcpy(&last, b.reps.ptr + (b.D - 1), 8, hipMemcpyDeviceToHost));
-  const int bitlen = last ? 64 - __builtin_clzll(last) : 0;
-  b.shift = std::max(0, bitlen - bits);
+  const int bitlen = last ? 64 - __builtin_clzll(last) : 1;
+  const int T = std::min(kSegmentBits, bitlen);
+  b.shift = bitlen - T;
+  b.nseg = b.D ? (last >> b.shift) + 1 : 0;
+  const auto cap = [](uint64_t n) {
+    return static_cast<unsigned>(std::max<uint64_t>(1, std::min<uint64_t>(grid_for(n), 1u << 20)));
+  };
+  DeviceBuffer<uint32_t> seg(b.nseg + 1);
+  segment_start_kernel<<<cap(b.nseg + 1), kBlock, 0, s>>>(b.reps.ptr, b.D, b.shift, b.nseg, seg.ptr);
+  SPINED_CHECK_LAUNCH();
+  std::vector<uint32_t> hseg(b.nseg + 1);
+  SPINED_HIP(hipMemcpyAsync(hseg.data(), seg.ptr, 4 * (b.nseg + 1), hipMemcpyDeviceToHost, s));
+  SPINED_HIP(hipStreamSynchronize(s));
+  // segment s gets 2^lb buckets, lb = ceil(log2(count_s * 2^bits / D)),
+  // clamped to [0, shift]; the total must fit the 32-bit bases
+  std::vector<uint2> hdir(std::max<uint64_t>(b.nseg, 1));
+  uint64_t total = 0;
+  for (;;) {
+    const double ratio = b.D ? std::ldexp(1.0, bits) / static_cast<double>(b.D) : 1.0;
+    total = 0;
+    for (uint64_t sg = 0; sg < b.nseg; ++sg) {
+      const double want = static_cast<double>(hseg[sg + 1] - hseg[sg]) * ratio;
+      int lb = 0;
+      while (lb < b.shift && std::ldexp(1.0, lb) < want) ++lb;
+      hdir[sg] = make_uint2(static_cast<uint32_t>(total), static_cast<uint32_t>(b.shift - lb));
+      total += 1ull << lb;
+    }
+    if (total < 0xffffffffull || bits <= 1) break;
+    --bits;
+  }
   b.bucket_bits = bits;
-  b.nbuckets = (last >> b.shift) + 1;
+  b.nbuckets = total;
+  b.dir.allocate(hdir.size());
+  SPINED_HIP(hipMemcpyAsync(b.dir.ptr, hdir.data(), sizeof(uint2) * hdir.size(),
+                            hipMemcpyHostToDevice, s));
   b.buckets.allocate(b.nbuckets + 1);
-  bucket_kernel<<<grid_for(b.nbuckets + 1), kBlock, 0, s>>>(b.reps.ptr, b.D, b.shift, b.nbuckets,
-                                                           b.buckets.ptr);
+  level2_kernel<<<cap(b.nbuckets + 1), kBlock, 0, s>>>(b.reps.ptr, b.D, b.dir.ptr, seg.ptr, b.nseg,
+                                                      b.shift, b.nbuckets, b.buckets.ptr);
   SPINED_CHECK_LAUNCH();
   SPINED_HIP(hipStreamSynchronize(s));
 }

@@ -89,10 +89,13 @@ struct Basis {
   uint64_t D = 0;
   DeviceBuffer<uint64_t> reps;
   DeviceBuffer<uint32_t> stab;
+  // two-level bucket index (see IndexArgs)
   DeviceBuffer<uint32_t> buckets;
+  DeviceBuffer<uint2> dir;
   int bucket_bits_request = -1; // -1: automatic
-  int bucket_bits = 0;
+  int bucket_bits = 0;          // about 2^bits level-2 buckets in total
   int shift = 0;
+  uint64_t nseg = 0;
   uint64_t nbuckets = 0;
   // index form: 0 = bucket table + binary search, 1 = hash table
   int index_kind_request = -1; // -1: automatic
@@ -103,7 +106,13 @@ struct Basis {
   double seconds_build = 0.0;
 
   IndexArgs index_args() const {
-    return IndexArgs{reps.ptr, buckets.ptr, nbuckets, shift, index_kind == 1 ? slots.ptr : nullptr,
+    return IndexArgs{reps.ptr,
+                     buckets.ptr,
+                     dir.ptr,
+                     nseg,
+                     shift >= 64 ? ~0ull : (1ull << shift) - 1,
+                     shift,
+                     index_kind == 1 ? slots.ptr : nullptr,
                      hmask};
   }
 };

@@ -279,10 +279,14 @@ __host__ __device__ inline uint64_t mix64(uint64_t z) {
 
 // ---------------------------------------------------------------------------
 // Index of a representative.  Two forms:
-//  * bucket table on the top bits plus a branchless binary search inside the
-//    bucket: buckets[b] = first i with reps[i] >> shift >= b, for b in
-//    [0, nbuckets]; buckets[nbuckets] = D.  Representatives cluster (orbit
-//    minima start with long zero runs), so buckets hold tens of entries.
+//  * two-level bucket table + branchless binary search inside the bucket.
+//    Representatives cluster (orbit minima start with long zero runs), so a
+//    table on the top bits alone leaves tens of entries per bucket.  Level 1
+//    splits the value range by its top bits into segments (v >> shift);
+//    segment s owns 2^lb_s level-2 buckets, lb_s chosen from its count, and
+//    dir[s] = (base_s, local shift).  Level 2: buckets[b] = first i with
+//    reps[i] >= start value of bucket b, in global order, so bucket b ends
+//    where b + 1 starts (also across segments); buckets[nbuckets] = D.
 //  * open-addressing hash table (slots != nullptr): 2^H 8-byte slots, H =
 //    ceil(log2 D) + 1 (load <= 1/2), linear probing from the low H bits of
 //    h = mix64(rep); a slot holds (h >> 32) << 32 | index, or kEmptySlot.  A
@@ -293,12 +297,23 @@ constexpr uint64_t kEmptySlot = ~0ull;
 
 struct IndexArgs {
   const uint64_t *__restrict__ reps;
-  const uint32_t *__restrict__ buckets;
-  uint64_t nbuckets;
-  int32_t shift;
+  const uint32_t *__restrict__ buckets; // level 2, nbuckets + 1 entries
+  const uint2 *__restrict__ dir;        // level 1: (base, local shift) per segment
+  uint64_t nseg;
+  uint64_t segmask;                     // (1 << shift) - 1
+  int32_t shift;                        // segment = v >> shift
   const uint64_t *__restrict__ slots;
   uint64_t hmask;
 };
+
+// Level-2 bucket of v, or false when v lies beyond the last segment.
+__host__ __device__ inline bool bucket_of(const IndexArgs &t, uint64_t v, uint64_t &b) {
+  const uint64_t s = v >> t.shift;
+  if (s >= t.nseg) return false;
+  const uint2 d = t.dir[s];
+  b = static_cast<uint64_t>(d.x) + ((v & t.segmask) >> d.y);
+  return true;
+}
 
 // First slot index whose fingerprint matches v (its stored index is a
 // candidate, to be confirmed with reps), or -1 when the probe meets an empty
@@ -333,8 +348,8 @@ __host__ __device__ inline int64_t hash_find(const IndexArgs &t, uint64_t v) {
 
 __host__ __device__ inline int64_t find_index(const IndexArgs &t, uint64_t v) {
   if (t.slots) return hash_find(t, v);
-  const uint64_t b = v >> t.shift;
-  if (b >= t.nbuckets) return -1;
+  uint64_t b = 0;
+  if (!bucket_of(t, v, b)) return -1;
   uint32_t lo = t.buckets[b];
   uint32_t n = t.buckets[b + 1] - lo;
   if (n == 0) return -1;
@@ -361,9 +376,9 @@ __host__ __device__ inline void find_index2(const IndexArgs &t, uint64_t va, boo
     if (bad_b) kb = hash_find(t, vb);
     return;
   }
-  const uint64_t ba = va >> t.shift, bb = vb >> t.shift;
-  use_a = use_a && ba < t.nbuckets;
-  use_b = use_b && bb < t.nbuckets;
+  uint64_t ba = 0, bb = 0;
+  use_a = use_a && bucket_of(t, va, ba);
+  use_b = use_b && bucket_of(t, vb, bb);
   uint32_t loa = 0, na = 0, lob = 0, nb = 0;
   if (use_a) {
     loa = t.buckets[ba];

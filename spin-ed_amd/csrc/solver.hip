@@ -115,6 +115,146 @@ __global__ void __launch_bounds__(kBlock)
   }
 }
 
+// Complex Hermitian Lanczos (vectors of double2, addressed as 2n doubles by
+// the componentwise kernels): only the reorthogonalisation needs the
+// conjugated dots and the complex update.
+// partial[(2 (8c + q) + {0: re, 1: im}) * G + b] = sum over this block's rows
+// of conj(V_i[r]) w[r].
+__global__ void __launch_bounds__(kBlock)
+    cmultidot_kernel(const double2 *const *__restrict__ V, int nvec,
+                     const double2 *__restrict__ w, uint64_t n, double *__restrict__ partial) {
+  __shared__ double lds[2 * kDotChunk][kBlock / 64];
+  const int c = blockIdx.y;
+  const int i0 = c * kDotChunk;
+  double re[kDotChunk], im[kDotChunk];
+#pragma unroll
+  for (int q = 0; q < kDotChunk; ++q) re[q] = im[q] = 0.0;
+  const double2 *vp[kDotChunk];
+#pragma unroll
+  for (int q = 0; q < kDotChunk; ++q) vp[q] = (i0 + q < nvec) ? V[i0 + q] : nullptr;
+  for (uint64_t r = blockIdx.x * static_cast<uint64_t>(kBlock) + threadIdx.x; r < n;
+       r += static_cast<uint64_t>(gridDim.x) * kBlock) {
+    const double2 wr = w[r];
+#pragma unroll
+    for (int q = 0; q < kDotChunk; ++q)
+      if (i0 + q < nvec) {
+        const double2 v = vp[q][r];
+        re[q] = fma(v.y, wr.y, fma(v.x, wr.x, re[q]));
+        im[q] = fma(-v.y, wr.x, fma(v.x, wr.y, im[q]));
+      }
+  }
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int q = 0; q < kDotChunk; ++q) {
+    double a = re[q], b = im[q];
+    for (int off = 32; off > 0; off >>= 1) {
+      a += __shfl_down(a, off, 64);
+      b += __shfl_down(b, off, 64);
+    }
+    if (lane == 0) {
+      lds[2 * q][wid] = a;
+      lds[2 * q + 1][wid] = b;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < 2 * kDotChunk && i0 + static_cast<int>(threadIdx.x / 2) < nvec) {
+    double sum = 0.0;
+    for (int k = 0; k < kBlock / 64; ++k) sum += lds[threadIdx.x][k];
+    partial[(2 * static_cast<uint64_t>(i0) + threadIdx.x) * gridDim.x + blockIdx.x] = sum;
+  }
+}
+
+// w[r] -= sum_i h_i V_i[r], h interleaved (re, im)
+__global__ void __launch_bounds__(kBlock)
+    cgemv_sub_kernel(const double2 *const *__restrict__ V, int nvec, const double *__restrict__ h,
+                     double2 *__restrict__ w, uint64_t n) {
+  for (uint64_t r = blockIdx.x * static_cast<uint64_t>(kBlock) + threadIdx.x; r < n;
+       r += static_cast<uint64_t>(gridDim.x) * kBlock) {
+    double ar = 0.0, ai = 0.0;
+    for (int i = 0; i < nvec; ++i) {
+      const double2 v = V[i][r];
+      const double hr = h[2 * i], hi = h[2 * i + 1];
+      ar = fma(-hi, v.y, fma(hr, v.x, ar));
+      ai = fma(hi, v.x, fma(hr, v.y, ai));
+    }
+    double2 x = w[r];
+    x.x -= ar;
+    x.y -= ai;
+    w[r] = x;
+  }
+}
+
+constexpr int kFuseMax = 32; // vectors held in registers by the fused passes
+
+// Block reduction of acc[q] (q < nq) into partial[q * gridDim.x + blockIdx.x]
+// with the multidot kernel's order (wave shuffles, then the 4 waves in
+// order), so fused and unfused passes give bit-identical sums.
+template <int NQ>
+__device__ inline void block_partials(const double (&acc)[NQ], int nq, double *__restrict__ partial) {
+  __shared__ double lds[NQ][kBlock / 64];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) {
+    if (q < nq) {
+      double v = acc[q];
+      for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+      if (lane == 0) lds[q][wid] = v;
+    }
+  }
+  __syncthreads();
+  if (static_cast<int>(threadIdx.x) < nq) {
+    double sum = 0.0;
+    for (int k = 0; k < kBlock / 64; ++k) sum += lds[threadIdx.x][k];
+    partial[static_cast<uint64_t>(threadIdx.x) * gridDim.x + blockIdx.x] = sum;
+  }
+}
+
+// Second Gram-Schmidt pass fused with the first update:
+//   w[r] -= sum_i h1[i] V_i[r];  partial dots of V_i with the new w.
+// Per row the arithmetic equals gemv_sub_kernel followed by multidot_kernel.
+template <class VT>
+__global__ void __launch_bounds__(kBlock)
+    sub_dot_kernel(const VT *const *__restrict__ V, int nvec, const double *__restrict__ h1,
+                   double *__restrict__ w, uint64_t n, double *__restrict__ partial) {
+  double acc[kFuseMax];
+#pragma unroll
+  for (int q = 0; q < kFuseMax; ++q) acc[q] = 0.0;
+  for (uint64_t r = blockIdx.x * static_cast<uint64_t>(kBlock) + threadIdx.x; r < n;
+       r += static_cast<uint64_t>(gridDim.x) * kBlock) {
+    double vals[kFuseMax];
+#pragma unroll
+    for (int q = 0; q < kFuseMax; ++q) vals[q] = q < nvec ? to_d(V[q][r]) : 0.0;
+    double sub = 0.0;
+#pragma unroll
+    for (int q = 0; q < kFuseMax; ++q)
+      if (q < nvec) sub = fma(h1[q], vals[q], sub);
+    const double wr = w[r] - sub;
+    w[r] = wr;
+#pragma unroll
+    for (int q = 0; q < kFuseMax; ++q)
+      if (q < nvec) acc[q] = fma(vals[q], wr, acc[q]);
+  }
+  block_partials<kFuseMax>(acc, nvec, partial);
+}
+
+// Last update fused with the norm: w[r] -= sum_i h2[i] V_i[r]; partial
+// sums of w[r]^2 (the norm_w multidot's arithmetic).
+template <class VT>
+__global__ void __launch_bounds__(kBlock)
+    sub_norm_kernel(const VT *const *__restrict__ V, int nvec, const double *__restrict__ h2,
+                    double *__restrict__ w, uint64_t n, double *__restrict__ partial) {
+  double acc[1] = {0.0};
+  for (uint64_t r = blockIdx.x * static_cast<uint64_t>(kBlock) + threadIdx.x; r < n;
+       r += static_cast<uint64_t>(gridDim.x) * kBlock) {
+    double sub = 0.0;
+    for (int i = 0; i < nvec; ++i) sub = fma(h2[i], to_d(V[i][r]), sub);
+    const double wr = w[r] - sub;
+    w[r] = wr;
+    acc[0] = fma(wr, wr, acc[0]);
+  }
+  block_partials<1>(acc, 1, partial);
+}
+
 // v[r] = w[r] / (*beta) (beta == 0 -> zero vector)
 template <class VT>
 __global__ void __launch_bounds__(kBlock)
@@ -145,20 +285,28 @@ __global__ void __launch_bounds__(256)
   jacobi_eigh_device(T, ldt, m, evals, S, max_sweeps);
 }
 
-// In place, row-local: V_i[r] <- sum_q S[i*m + q] V_q[r] for i < keep.
+constexpr int kMaxKeepOne = 32; // kept Ritz vectors of the one-vector restart
+
+// In place, row-local: V_i[r] <- sum_q S[i*m + q] V_q[r] for i < keep <= 32.
+// The outputs live in registers (unrolled, guarded), the inputs stream once.
 template <class VT>
 __global__ void __launch_bounds__(kBlock)
     restart_kernel(VT *const *__restrict__ V, int m, int keep, const double *__restrict__ S,
                    uint64_t n) {
   for (uint64_t r = blockIdx.x * static_cast<uint64_t>(kBlock) + threadIdx.x; r < n;
        r += static_cast<uint64_t>(gridDim.x) * kBlock) {
-    double row[kMaxBasis];
-    for (int q = 0; q < m; ++q) row[q] = to_d(V[q][r]);
-    for (int i = 0; i < keep; ++i) {
-      double acc = 0.0;
-      for (int q = 0; q < m; ++q) acc = fma(S[i * m + q], row[q], acc);
-      V[i][r] = static_cast<VT>(acc);
+    double out[kMaxKeepOne];
+#pragma unroll
+    for (int i = 0; i < kMaxKeepOne; ++i) out[i] = 0.0;
+    for (int q = 0; q < m; ++q) {
+      const double x = to_d(V[q][r]);
+#pragma unroll
+      for (int i = 0; i < kMaxKeepOne; ++i)
+        if (i < keep) out[i] = fma(S[i * m + q], x, out[i]);
     }
+#pragma unroll
+    for (int i = 0; i < kMaxKeepOne; ++i)
+      if (i < keep) V[i][r] = static_cast<VT>(out[i]);
   }
 }
 
@@ -193,11 +341,14 @@ __global__ void __launch_bounds__(kBlock) convert_kernel(const A *__restrict__ a
 }
 
 
+// stride 2: h1, h2 hold complex coefficients (re, im); T takes the real
+// parts (the projected matrix of a Hermitian operator is real symmetric in
+// this basis; the imaginary parts are rounding).
 __global__ void store_column2_kernel(const double *__restrict__ h1, const double *__restrict__ h2,
-                                     int j, double *__restrict__ T, int ldt) {
+                                     int j, double *__restrict__ T, int ldt, int stride) {
   const int i = threadIdx.x;
   if (i <= j) {
-    const double v = h1[i] + h2[i];
+    const double v = h1[stride * i] + h2[stride * i];
     T[i * ldt + j] = v;
     T[j * ldt + i] = v;
   }
@@ -224,9 +375,12 @@ struct PhaseResult {
 };
 
 // One thick-restart Lanczos phase with vector storage type VT.
-template <class VT> struct Phase {
+// C = true: complex Hermitian operator; VT = double and every vector holds
+// n double2 values, i.e. ne = 2n doubles for the componentwise kernels.
+template <class VT, bool C = false> struct Phase {
   Operator &op;
   uint64_t n;
+  uint64_t ne; // doubles (or floats) per vector
   int m, k, keep;
   hipStream_t s;
   std::vector<DeviceBuffer<VT>> vecs;
@@ -241,21 +395,22 @@ template <class VT> struct Phase {
   int mlast = 0; // size of the last projected problem (Ritz assembly)
 
   Phase(Operator &o, uint64_t n_, int m_, int k_, int keep_, hipStream_t s_)
-      : op(o), n(n_), m(m_), k(k_), keep(keep_), s(s_) {
+      : op(o), n(n_), ne(C ? 2 * n_ : n_), m(m_), k(k_), keep(keep_), s(s_) {
+    static_assert(!C || std::is_same<VT, double>::value, "complex vectors are stored in FP64");
     vecs.resize(m + 1);
-    for (auto &v : vecs) v.allocate(n);
+    for (auto &v : vecs) v.allocate(ne);
     std::vector<VT *> hp(m + 1);
     for (int i = 0; i <= m; ++i) hp[i] = vecs[i].ptr;
     vptr.allocate(m + 1);
     SPINED_HIP(hipMemcpy(vptr.ptr, hp.data(), sizeof(VT *) * (m + 1), hipMemcpyHostToDevice));
-    w.allocate(n);
+    w.allocate(ne);
     const double *wp = w.ptr;
     wptr.allocate(1);
     SPINED_HIP(hipMemcpy(wptr.ptr, &wp, sizeof(wp), hipMemcpyHostToDevice));
-    G = grid_for(n, 2048);
+    G = grid_for(ne, 2048);
     const int chunks = (m + 1 + kDotChunk - 1) / kDotChunk;
-    partial.allocate(static_cast<size_t>(chunks) * kDotChunk * G);
-    h.allocate(2 * kMaxBasis + 2);
+    partial.allocate(static_cast<size_t>(chunks) * kDotChunk * G * (C ? 2 : 1));
+    h.allocate(4 * kMaxBasis + 4);
     T.allocate(kMaxBasis * kMaxBasis);
     theta.allocate(kMaxBasis);
     S.allocate(kMaxBasis * kMaxBasis);
@@ -273,7 +428,7 @@ template <class VT> struct Phase {
     SPINED_HIP(hipEventCreate(&a));
     SPINED_HIP(hipEventCreate(&b));
     SPINED_HIP(hipEventRecord(a, s));
-    MatvecCall c{&op, xt, VecType::F64, x, y, n, s};
+    MatvecCall c{&op, xt, xt == VecType::C128 ? VecType::C128 : VecType::F64, x, y, n, s};
     launch_matvec(c);
     SPINED_HIP(hipEventRecord(b, s));
     ev.push_back(a);
@@ -292,9 +447,12 @@ template <class VT> struct Phase {
     ev.clear();
   }
 
-  template <class PT> void multidot(const PT *const *V, int nvec, const double *wv, double *hout) {
+  // real dots over `len` elements (default: the whole vector, ne)
+  template <class PT>
+  void multidot(const PT *const *V, int nvec, const double *wv, double *hout, uint64_t len = 0) {
     const int chunks = (nvec + kDotChunk - 1) / kDotChunk;
-    multidot_kernel<PT><<<dim3(G, chunks), kBlock, 0, s>>>(V, nvec, wv, n, partial.ptr);
+    multidot_kernel<PT><<<dim3(G, chunks), kBlock, 0, s>>>(V, nvec, wv, len ? len : ne,
+                                                           partial.ptr);
     SPINED_CHECK_LAUNCH();
     reduce_partials_kernel<<<nvec, kBlock, 0, s>>>(partial.ptr, nvec, G, hout, 0);
     SPINED_CHECK_LAUNCH();
@@ -306,8 +464,33 @@ template <class VT> struct Phase {
     SPINED_CHECK_LAUNCH();
   }
 
+  // complex conjugated dots h[2i], h[2i+1] = conj(V_i) . w
+  void cdots(int nvec, double *hout) {
+    const int chunks = (nvec + kDotChunk - 1) / kDotChunk;
+    const auto *V2 = reinterpret_cast<const double2 *const *>(vptr.ptr);
+    cmultidot_kernel<<<dim3(G, chunks), kBlock, 0, s>>>(V2, nvec,
+                                                        reinterpret_cast<const double2 *>(w.ptr), n,
+                                                        partial.ptr);
+    SPINED_CHECK_LAUNCH();
+    reduce_partials_kernel<<<2 * nvec, kBlock, 0, s>>>(partial.ptr, 2 * nvec, G, hout, 0);
+    SPINED_CHECK_LAUNCH();
+  }
+
+  void csub(int nvec, const double *hc) {
+    const auto *V2 = reinterpret_cast<const double2 *const *>(vptr.ptr);
+    cgemv_sub_kernel<<<G, kBlock, 0, s>>>(V2, nvec, hc, reinterpret_cast<double2 *>(w.ptr), n);
+    SPINED_CHECK_LAUNCH();
+  }
+
   // orthogonalise w against V[0..nvec) twice; h1 + h2 are the coefficients
   void reorth(int nvec, double *h1, double *h2) {
+    if constexpr (C) {
+      cdots(nvec, h1);
+      csub(nvec, h1);
+      cdots(nvec, h2);
+      csub(nvec, h2);
+      return;
+    }
     multidot<VT>(vptr.ptr, nvec, w.ptr, h1);
     gemv_sub_kernel<VT><<<G, kBlock, 0, s>>>(vptr.ptr, nvec, h1, w.ptr, n);
     SPINED_CHECK_LAUNCH();
@@ -319,17 +502,40 @@ template <class VT> struct Phase {
   // v_0 from an FP64 start vector in w (normalised here)
   void set_start_from_w() {
     norm_w(beta.ptr);
-    scale_store_kernel<VT><<<G, kBlock, 0, s>>>(w.ptr, beta.ptr, vecs[0].ptr, n);
+    scale_store_kernel<VT><<<G, kBlock, 0, s>>>(w.ptr, beta.ptr, vecs[0].ptr, ne);
+    SPINED_CHECK_LAUNCH();
+  }
+
+  // Twice classical Gram-Schmidt against V[0..nvec) and the norm of the
+  // result, in three passes over V: dots; update + dots; update + norm.
+  // Same arithmetic as reorth() followed by norm_w().
+  void reorth_norm_fused(int nvec, double *h1, double *h2, double *norm_out) {
+    multidot<VT>(vptr.ptr, nvec, w.ptr, h1);
+    sub_dot_kernel<VT><<<G, kBlock, 0, s>>>(vptr.ptr, nvec, h1, w.ptr, n, partial.ptr);
+    SPINED_CHECK_LAUNCH();
+    reduce_partials_kernel<<<nvec, kBlock, 0, s>>>(partial.ptr, nvec, G, h2, 0);
+    SPINED_CHECK_LAUNCH();
+    sub_norm_kernel<VT><<<G, kBlock, 0, s>>>(vptr.ptr, nvec, h2, w.ptr, n, partial.ptr);
+    SPINED_CHECK_LAUNCH();
+    reduce_partials_kernel<<<1, kBlock, 0, s>>>(partial.ptr, 1, G, norm_out, 0);
+    SPINED_CHECK_LAUNCH();
+    sqrt_inplace_kernel<<<1, 1, 0, s>>>(norm_out);
     SPINED_CHECK_LAUNCH();
   }
 
   void step(int j, uint64_t seed_breakdown) {
-    matvec(vecs[j].ptr, std::is_same<VT, float>::value ? VecType::F32 : VecType::F64, w.ptr);
-    double *h1 = h.ptr, *h2 = h.ptr + kMaxBasis + 1;
-    reorth(j + 1, h1, h2);
-    store_column2_kernel<<<1, kMaxBasis, 0, s>>>(h1, h2, j, T.ptr, kMaxBasis);
+    matvec(vecs[j].ptr,
+           C ? VecType::C128 : (std::is_same<VT, float>::value ? VecType::F32 : VecType::F64),
+           w.ptr);
+    double *h1 = h.ptr, *h2 = h.ptr + 2 * kMaxBasis + 2;
+    if (!C && j + 1 <= kFuseMax) {
+      reorth_norm_fused(j + 1, h1, h2, beta.ptr + j);
+    } else {
+      reorth(j + 1, h1, h2);
+      norm_w(beta.ptr + j);
+    }
+    store_column2_kernel<<<1, kMaxBasis, 0, s>>>(h1, h2, j, T.ptr, kMaxBasis, C ? 2 : 1);
     SPINED_CHECK_LAUNCH();
-    norm_w(beta.ptr + j);
     if (small) {
       // breakdown guard (invariant subspace): continue with a random vector
       double bj = 0.0, tj = 0.0;
@@ -337,14 +543,14 @@ template <class VT> struct Phase {
       SPINED_HIP(hipMemcpyAsync(&tj, T.ptr + j * kMaxBasis + j, 8, hipMemcpyDeviceToHost, s));
       SPINED_HIP(hipStreamSynchronize(s));
       if (bj <= 1e-12 * std::max(1.0, std::fabs(tj)) && j + 1 < static_cast<int>(n)) {
-        launch_random_fill(w.ptr, n, seed_breakdown + 7919u * (j + 1), s);
-        double *g1 = h.ptr, *g2 = h.ptr + kMaxBasis + 1; // scratch (column already stored)
+        launch_random_fill(w.ptr, ne, seed_breakdown + 7919u * (j + 1), s);
+        double *g1 = h.ptr, *g2 = h.ptr + 2 * kMaxBasis + 2; // scratch (column already stored)
         reorth(j + 1, g1, g2);
         double nb = 0.0;
         norm_w(beta.ptr + kMaxBasis);
         SPINED_HIP(hipMemcpyAsync(&nb, beta.ptr + kMaxBasis, 8, hipMemcpyDeviceToHost, s));
         SPINED_HIP(hipStreamSynchronize(s));
-        scale_store_kernel<VT><<<G, kBlock, 0, s>>>(w.ptr, beta.ptr + kMaxBasis, vecs[j + 1].ptr, n);
+        scale_store_kernel<VT><<<G, kBlock, 0, s>>>(w.ptr, beta.ptr + kMaxBasis, vecs[j + 1].ptr, ne);
         SPINED_CHECK_LAUNCH();
         const double zero = 0.0;
         SPINED_HIP(hipMemcpyAsync(beta.ptr + j, &zero, 8, hipMemcpyHostToDevice, s));
@@ -352,7 +558,7 @@ template <class VT> struct Phase {
         return;
       }
     }
-    scale_store_kernel<VT><<<G, kBlock, 0, s>>>(w.ptr, beta.ptr + j, vecs[j + 1].ptr, n);
+    scale_store_kernel<VT><<<G, kBlock, 0, s>>>(w.ptr, beta.ptr + j, vecs[j + 1].ptr, ne);
     SPINED_CHECK_LAUNCH();
   }
 
@@ -431,9 +637,9 @@ template <class VT> struct Phase {
         }
       }
       // thick restart: keep the lowest `keep` Ritz vectors and v_m
-      restart_kernel<VT><<<G, kBlock, 0, s>>>(vptr.ptr, m, keep, S.ptr, n);
+      restart_kernel<VT><<<G, kBlock, 0, s>>>(vptr.ptr, m, keep, S.ptr, ne);
       SPINED_CHECK_LAUNCH();
-      SPINED_HIP(hipMemcpyAsync(vecs[keep].ptr, vecs[m].ptr, n * sizeof(VT),
+      SPINED_HIP(hipMemcpyAsync(vecs[keep].ptr, vecs[m].ptr, ne * sizeof(VT),
                                 hipMemcpyDeviceToDevice, s));
       reset_projected_kernel<<<1, 256, 0, s>>>(T.ptr, kMaxBasis, theta.ptr, keep);
       SPINED_CHECK_LAUNCH();
@@ -444,7 +650,7 @@ template <class VT> struct Phase {
   // FP64 Ritz vector i of the last projected problem into out
   void ritz(int i, double *out) {
     ritz_kernel<VT><<<G, kBlock, 0, s>>>(vptr.ptr, mlast, S.ptr + static_cast<size_t>(i) * mlast,
-                                         out, n);
+                                         out, ne);
     SPINED_CHECK_LAUNCH();
   }
 };
@@ -467,6 +673,7 @@ void pick_sizes(uint64_t n, uint64_t k, const spined_eigsh_options &o, int &m, i
   int kp = o.min_restart_size > 0 ? o.min_restart_size : (m + static_cast<int>(k)) / 2;
   kp = std::max(kp, static_cast<int>(k));
   kp = std::min(kp, m - 1);
+  kp = std::min(kp, kMaxKeepOne);
   keep = std::max(kp, 0);
 }
 
@@ -572,6 +779,86 @@ void launch_copy_calibration(const void *src, void *dst, uint64_t bytes, hipStre
   SPINED_CHECK_LAUNCH();
 }
 
+// The FP64 phase that finishes every solve (real: C = false; complex
+// Hermitian: C = true, vectors of n double2), with the explicit residual
+// check of the Ritz vectors and the eigenvector copy-out.  start: the FP64
+// start vector (ne doubles); y, hy: scratch vectors of ne doubles.
+template <bool C>
+bool final_phase(Operator &op, uint64_t n, int m, int keep, uint64_t k, double tol, int dtype,
+                 const spined_eigsh_options &opts, DeviceBuffer<double> &start,
+                 DeviceBuffer<double> &y, DeviceBuffer<double> &hy, void *evecs, uint64_t ld,
+                 spined_monitor_fn monitor, void *ctx, spined_stats &st, std::vector<double> &theta,
+                 std::vector<double> &est, hipStream_t s) {
+  bool converged = false;
+  const uint64_t ne = C ? 2 * n : n;
+  const VecType vt = C ? VecType::C128 : VecType::F64;
+  Phase<double, C> p(op, n, m, static_cast<int>(k), keep, s);
+  SPINED_HIP(hipMemcpyAsync(p.w.ptr, start.ptr, ne * 8, hipMemcpyDeviceToDevice, s));
+  p.set_start_from_w();
+  start.release();
+  int64_t budget = opts.max_matvecs > 0 ? std::max<int64_t>(1, opts.max_matvecs - st.matvecs) : 0;
+  for (int attempt = 0; attempt < 8; ++attempt) {
+    PhaseResult r = p.run(tol, budget, false, 0, monitor, ctx, opts.verbose, opts.seed + 1);
+    st.restarts += r.cycles;
+    // explicit residuals of the Ritz vectors (FP64 matvec)
+    bool ok = true;
+    for (uint64_t i = 0; i < k; ++i) {
+      p.ritz(static_cast<int>(i), y.ptr);
+      p.matvec(y.ptr, vt, hy.ptr);
+      // Rayleigh quotient and residual norm
+      const double *yp = y.ptr;
+      DeviceBuffer<const double *> yptr(1);
+      SPINED_HIP(hipMemcpyAsync(yptr.ptr, &yp, sizeof(yp), hipMemcpyHostToDevice, s));
+      DeviceBuffer<double> sc(4);
+      p.template multidot<double>(yptr.ptr, 1, hy.ptr, sc.ptr);     // y.Hy
+      p.template multidot<double>(yptr.ptr, 1, y.ptr, sc.ptr + 1);  // y.y
+      double hsc[2];
+      SPINED_HIP(hipMemcpyAsync(hsc, sc.ptr, 16, hipMemcpyDeviceToHost, s));
+      SPINED_HIP(hipStreamSynchronize(s));
+      const double lam = hsc[0] / hsc[1];
+      const double ynorm = std::sqrt(hsc[1]);
+      residual_kernel<<<grid_for(ne), kBlock, 0, s>>>(hy.ptr, y.ptr, lam, hy.ptr, ne);
+      SPINED_CHECK_LAUNCH();
+      const double *hp = hy.ptr;
+      SPINED_HIP(hipMemcpyAsync(yptr.ptr, &hp, sizeof(hp), hipMemcpyHostToDevice, s));
+      p.template multidot<double>(yptr.ptr, 1, hy.ptr, sc.ptr + 2);
+      double rr = 0.0;
+      SPINED_HIP(hipMemcpyAsync(&rr, sc.ptr + 2, 8, hipMemcpyDeviceToHost, s));
+      SPINED_HIP(hipStreamSynchronize(s));
+      theta[i] = lam;
+      est[i] = std::sqrt(std::max(rr, 0.0)) / ynorm;
+      if (est[i] > tol) ok = false;
+      if (evecs) {
+        // normalise and copy out in the requested element type
+        const double inv = 1.0 / ynorm;
+        std::vector<double> hv(ne);
+        SPINED_HIP(hipMemcpyAsync(hv.data(), y.ptr, ne * 8, hipMemcpyDeviceToHost, s));
+        SPINED_HIP(hipStreamSynchronize(s));
+        // complex: ld counts complex elements, each two reals
+        const uint64_t per = C ? 2 : 1;
+        if (dtype == LS_FLOAT64 || dtype == LS_COMPLEX128) {
+          double *o = static_cast<double *>(evecs) + i * ld * per;
+          for (uint64_t q = 0; q < ne; ++q) o[q] = hv[q] * inv;
+        } else {
+          float *o = static_cast<float *>(evecs) + i * ld * per;
+          for (uint64_t q = 0; q < ne; ++q) o[q] = static_cast<float>(hv[q] * inv);
+        }
+      }
+    }
+    p.harvest_events();
+    if (ok || (budget > 0 && p.matvecs >= budget)) {
+      converged = ok;
+      break;
+    }
+    // estimates said converged but the explicit residual disagrees: keep
+    // iterating from the current basis with a tighter target
+    tol *= 0.5;
+  }
+  st.matvecs += p.matvecs;
+  st.seconds_matvec += p.matvec_seconds;
+  return converged;
+}
+
 int eigsh(Operator &op, int dtype, uint64_t k, double tol, const spined_eigsh_options &opts,
           double *evals, void *evecs, uint64_t ld, double *rnorms, spined_monitor_fn monitor,
           void *ctx) {
@@ -580,9 +867,12 @@ int eigsh(Operator &op, int dtype, uint64_t k, double tol, const spined_eigsh_op
   const uint64_t n = b.D;
   if (k == 0 || k > n) throw Error(LS_INVALID_ARGUMENT, "number of eigenpairs must be in [1, dim]");
   if (k > 16) throw Error(LS_NOT_IMPLEMENTED, "at most 16 eigenpairs per solve");
-  if (dtype != LS_FLOAT64 && dtype != LS_FLOAT32)
-    throw Error(LS_INVALID_DATATYPE, "spined_eigsh supports real float32/float64 output");
-  if (!op.tables.is_real) throw Error(LS_INVALID_DATATYPE, "spined_eigsh needs a real operator");
+  const bool cplx = !op.tables.is_real;
+  if (!cplx && dtype != LS_FLOAT64 && dtype != LS_FLOAT32)
+    throw Error(LS_INVALID_DATATYPE, "a real operator takes float32/float64 eigenvectors");
+  if (cplx && dtype != LS_COMPLEX128 && dtype != LS_COMPLEX64)
+    throw Error(LS_INVALID_DATATYPE, "a complex operator takes complex64/complex128 eigenvectors");
+  const uint64_t ne = cplx ? 2 * n : n;
   if (evecs && ld < n) throw Error(LS_INVALID_ARGUMENT, "leading dimension smaller than dim");
   if (!(tol > 0.0)) tol = 1e-10; // PRIMME's eps = 0 means "its default"; ours is 1e-10
   hipStream_t s = stream();
@@ -591,31 +881,39 @@ int eigsh(Operator &op, int dtype, uint64_t k, double tol, const spined_eigsh_op
   spined_stats st{};
 
   // ---- start vector (FP64, in a scratch buffer) --------------------------
-  DeviceBuffer<double> start(n);
+  DeviceBuffer<double> start(ne);
   const auto ts = std::chrono::steady_clock::now();
+  std::vector<double> host_start;
   if (opts.start == SPINED_START_VECTOR) {
     if (!opts.start_vector) throw Error(LS_INVALID_ARGUMENT, "start_vector is NULL");
-    SPINED_HIP(hipMemcpyAsync(start.ptr, opts.start_vector, n * 8, hipMemcpyHostToDevice, s));
+    if (cplx) {
+      // real start vector as the real parts of the complex one
+      host_start.assign(ne, 0.0);
+      for (uint64_t q = 0; q < n; ++q) host_start[2 * q] = opts.start_vector[q];
+      SPINED_HIP(hipMemcpyAsync(start.ptr, host_start.data(), ne * 8, hipMemcpyHostToDevice, s));
+    } else {
+      SPINED_HIP(hipMemcpyAsync(start.ptr, opts.start_vector, n * 8, hipMemcpyHostToDevice, s));
+    }
   } else if (opts.start == SPINED_START_LEARNED) {
     if (!opts.model) throw Error(LS_INVALID_ARGUMENT, "learned start needs a model");
+    if (cplx) throw Error(LS_INVALID_ARGUMENT, "the learned start covers real sectors only");
     cnn_start_vector(static_cast<const spined_cnn *>(opts.model), b, opts.start_sign_mask,
                      start.ptr);
   } else {
-    launch_random_fill(start.ptr, n, opts.seed, s);
+    launch_random_fill(start.ptr, ne, opts.seed, s);
   }
   SPINED_HIP(hipStreamSynchronize(s));
   st.seconds_start = seconds_since(ts);
 
   std::vector<double> theta(k), est(k);
-  bool converged = false;
-  DeviceBuffer<double> y(n), hy(n);
+  DeviceBuffer<double> y(ne), hy(ne);
 
   // Block phase (max_block_size >= 2): FP32 storage in mixed mode, FP64
   // otherwise; the one-vector FP64 phase below then finishes from the sum of
   // its Ritz vectors (and owns the explicit residual check).
   const int K = block_width(opts.max_block_size);
   bool block_done = false;
-  if (K > 1 && matvec_block_supported(op) && n >= (1u << 14)) {
+  if (!cplx && K > 1 && matvec_block_supported(op) && n >= (1u << 14)) {
     const bool mixed = opts.precision == SPINED_PRECISION_MIXED;
     const BlockOutcome bo =
         mixed ? run_block<float>(K, op, n, k, std::max(tol, 1e-5), true, 1, opts, start.ptr,
@@ -629,7 +927,9 @@ int eigsh(Operator &op, int dtype, uint64_t k, double tol, const spined_eigsh_op
     block_done = bo.ran;
   }
 
-  if (!block_done && opts.precision == SPINED_PRECISION_MIXED && n > static_cast<uint64_t>(m)) {
+  // complex sectors: FP64 throughout (the mixed phase stores real vectors)
+  if (!cplx && !block_done && opts.precision == SPINED_PRECISION_MIXED &&
+      n > static_cast<uint64_t>(m)) {
     // FP32-storage phase down to its floor, then hand the FP64 Ritz vector
     // (sum of the k lowest) to the FP64 phase.
     Phase<float> p(op, n, m, static_cast<int>(k), keep, s);
@@ -654,68 +954,11 @@ int eigsh(Operator &op, int dtype, uint64_t k, double tol, const spined_eigsh_op
     st.restarts += r.cycles;
   }
 
-  Phase<double> p(op, n, m, static_cast<int>(k), keep, s);
-  SPINED_HIP(hipMemcpyAsync(p.w.ptr, start.ptr, n * 8, hipMemcpyDeviceToDevice, s));
-  p.set_start_from_w();
-  start.release();
-  int64_t budget = opts.max_matvecs > 0 ? std::max<int64_t>(1, opts.max_matvecs - st.matvecs) : 0;
-  for (int attempt = 0; attempt < 8; ++attempt) {
-    PhaseResult r = p.run(tol, budget, false, 0, monitor, ctx, opts.verbose, opts.seed + 1);
-    st.restarts += r.cycles;
-    // explicit residuals of the Ritz vectors (FP64 matvec)
-    bool ok = true;
-    for (uint64_t i = 0; i < k; ++i) {
-      p.ritz(static_cast<int>(i), y.ptr);
-      p.matvec(y.ptr, VecType::F64, hy.ptr);
-      // Rayleigh quotient and residual norm
-      const double *yp = y.ptr;
-      DeviceBuffer<const double *> yptr(1);
-      SPINED_HIP(hipMemcpyAsync(yptr.ptr, &yp, sizeof(yp), hipMemcpyHostToDevice, s));
-      DeviceBuffer<double> sc(4);
-      p.template multidot<double>(yptr.ptr, 1, hy.ptr, sc.ptr);     // y.Hy
-      p.template multidot<double>(yptr.ptr, 1, y.ptr, sc.ptr + 1);  // y.y
-      double hsc[2];
-      SPINED_HIP(hipMemcpyAsync(hsc, sc.ptr, 16, hipMemcpyDeviceToHost, s));
-      SPINED_HIP(hipStreamSynchronize(s));
-      const double lam = hsc[0] / hsc[1];
-      const double ynorm = std::sqrt(hsc[1]);
-      residual_kernel<<<grid_for(n), kBlock, 0, s>>>(hy.ptr, y.ptr, lam, hy.ptr, n);
-      SPINED_CHECK_LAUNCH();
-      const double *hp = hy.ptr;
-      SPINED_HIP(hipMemcpyAsync(yptr.ptr, &hp, sizeof(hp), hipMemcpyHostToDevice, s));
-      p.template multidot<double>(yptr.ptr, 1, hy.ptr, sc.ptr + 2);
-      double rr = 0.0;
-      SPINED_HIP(hipMemcpyAsync(&rr, sc.ptr + 2, 8, hipMemcpyDeviceToHost, s));
-      SPINED_HIP(hipStreamSynchronize(s));
-      theta[i] = lam;
-      est[i] = std::sqrt(std::max(rr, 0.0)) / ynorm;
-      if (est[i] > tol) ok = false;
-      if (evecs) {
-        // normalise and copy out in the requested element type
-        const double inv = 1.0 / ynorm;
-        std::vector<double> hv(n);
-        SPINED_HIP(hipMemcpyAsync(hv.data(), y.ptr, n * 8, hipMemcpyDeviceToHost, s));
-        SPINED_HIP(hipStreamSynchronize(s));
-        if (dtype == LS_FLOAT64) {
-          double *o = static_cast<double *>(evecs) + i * ld;
-          for (uint64_t q = 0; q < n; ++q) o[q] = hv[q] * inv;
-        } else {
-          float *o = static_cast<float *>(evecs) + i * ld;
-          for (uint64_t q = 0; q < n; ++q) o[q] = static_cast<float>(hv[q] * inv);
-        }
-      }
-    }
-    p.harvest_events();
-    if (ok || (budget > 0 && p.matvecs >= budget)) {
-      converged = ok;
-      break;
-    }
-    // estimates said converged but the explicit residual disagrees: keep
-    // iterating from the current basis with a tighter target
-    tol *= 0.5;
-  }
-  st.matvecs += p.matvecs;
-  st.seconds_matvec += p.matvec_seconds;
+  const bool converged =
+      cplx ? final_phase<true>(op, n, m, keep, k, tol, dtype, opts, start, y, hy, evecs, ld,
+                               monitor, ctx, st, theta, est, s)
+           : final_phase<false>(op, n, m, keep, k, tol, dtype, opts, start, y, hy, evecs, ld,
+                                monitor, ctx, st, theta, est, s);
   for (uint64_t i = 0; i < k; ++i) {
     evals[i] = theta[i];
     if (rnorms) rnorms[i] = est[i];

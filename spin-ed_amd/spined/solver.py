@@ -56,7 +56,11 @@ def eigsh(op: Operator_, k: int = 1, tol: float = 1e-10, *, precision: str = "fp
     evals = np.zeros(k)
     rn = np.zeros(k)
     dtype = np.dtype(dtype)
-    code = {np.dtype(np.float64): 1, np.dtype(np.float32): 0}[dtype]
+    if not op.is_real and dtype.kind != "c":
+        # complex (momentum) sectors: Hermitian Lanczos, complex eigenvectors
+        dtype = np.dtype(np.complex128 if dtype == np.float64 else np.complex64)
+    code = {np.dtype(np.float64): 1, np.dtype(np.float32): 0,
+            np.dtype(np.complex64): 2, np.dtype(np.complex128): 3}[dtype]
     evecs = np.zeros((D, k), dtype=dtype, order="F") if want_vectors else None
 
     def _mon(info_p, _ctx):

@@ -70,3 +70,26 @@ def test_hash_index_before_build_and_eigsh():
     assert r.status == 0
     assert abs(r.evals[0] - (-42.680058066148944)) <= 1e-10 * 42.68
     assert r.rnorms[0] <= 1e-10
+
+
+@pytest.mark.parametrize("spec", CASES[:4], ids=lambda s: s.name)
+def test_bucket_sizes_bit_identical(spec):
+    """The two-level bucket index at very coarse, default and very fine level-2
+    sizes: every lookup is exact, so matvecs and gather counts do not change."""
+    print("case", spec.name, flush=True)
+    basis, op = engine(spec)
+    o = oracle(spec)
+    D = o.dimension
+    x = np.random.default_rng(5).uniform(-1, 1, D)
+    if not o.is_real:
+        x = x + 1j * np.random.default_rng(6).uniform(-1, 1, D)
+    y0 = spined.apply(op, x)
+    c0 = spined.count_offdiag(op)
+    default_bits = spined.bucket_bits(basis)
+    for bits in (1, 4, default_bits + 2, 34, -1):
+        spined.set_bucket_bits(basis, bits)
+        np.testing.assert_array_equal(spined.apply(op, x), y0, err_msg=f"bits {bits}")
+        assert spined.count_offdiag(op) == c0, bits
+    assert spined.bucket_bits(basis) == default_bits
+    yr = o.matvec(x)
+    assert np.max(np.abs(y0 - yr)) <= 1e-13 * np.max(np.abs(yr))

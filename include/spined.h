@@ -134,6 +134,13 @@ int spined_operator_count_offdiag(const ls_operator *op, uint64_t *out);
 int spined_operator_orbit_method(const ls_operator *op);
 /* Force an orbit method (for parity tests): -1 = automatic. */
 int spined_operator_set_orbit_method(ls_operator *op, int method);
+/* Row schedule of the real matvec kernels: within tiles of 4096 rows the
+ * rows are processed in the order of their number of active off-diagonal
+ * terms, so the 64 lanes of a wave carry equal work (the sums per row are
+ * unchanged).  mode: -1 automatic (on for bases of >= 4096 rows), 0 off,
+ * 1 on; built at the next matvec.  get: 1 if a schedule is in use. */
+int spined_operator_set_schedule(ls_operator *op, int mode);
+int spined_operator_get_schedule(const ls_operator *op);
 /* Bucket bits of the bucket index (2^bits buckets over the representatives'
  * range, 1..34; -1 = automatic: ceil(log2 D) - 1).  Setting it on a built
  * basis rebuilds the index. */

@@ -110,6 +110,7 @@ Operator &prepare_operator(const ls_operator *op) {
   require_built(b);
   ensure_basis_device(b);
   ensure_device(o.tables);
+  ensure_schedule(o);
   return o;
 }
 
@@ -618,6 +619,23 @@ int spined_operator_set_orbit_method(ls_operator *op, int method) {
     if (!op || method < -1 || method > 1) throw Error(LS_INVALID_ARGUMENT, "bad argument");
     op->p->orbit_method_override = method;
   });
+}
+
+int spined_operator_set_schedule(ls_operator *op, int mode) {
+  return guarded([&] {
+    if (!op || mode < -1 || mode > 1) throw Error(LS_INVALID_ARGUMENT, "bad argument");
+    Operator &o = *op->p;
+    {
+      std::lock_guard<std::mutex> lk(o.mu);
+      o.schedule_request = mode;
+      o.schedule_built = false;
+      o.sched.release();
+    }
+  });
+}
+
+int spined_operator_get_schedule(const ls_operator *op) {
+  return op ? (op->p->schedule_built ? 1 : 0) : -1;
 }
 
 int spined_basis_set_bucket_bits(ls_spin_basis *basis, int bits) {

@@ -169,7 +169,17 @@ struct Operator {
   std::mutex mu;
   spined_stats last_stats{};
   int orbit_method() const; // 0 generic, 1 ring
+  // Row schedule of the real matvec kernels: within every tile of
+  // kScheduleTile consecutive rows, the rows ordered by their number of
+  // active off-diagonal templates, so a wave's lanes carry equal work.
+  // sched[i] = offset within the tile of the row processed at slot i.
+  int schedule_request = -1; // -1 automatic, 0 off, 1 on
+  bool schedule_built = false;
+  DeviceBuffer<uint16_t> sched;
 };
+
+constexpr int kScheduleTile = 4096;
+void ensure_schedule(Operator &op); // matvec.hip
 
 void compile_operator(Operator &op);
 

@@ -29,14 +29,22 @@ namespace {
 constexpr int kBlock = 256;
 
 // ---- real rows -------------------------------------------------------------
+// Row of slot i: identity, or the tile's schedule (rows of equal work side
+// by side in a wave).
+__device__ inline uint64_t scheduled_row(const uint16_t *__restrict__ sched, uint64_t i) {
+  return sched ? (i & ~static_cast<uint64_t>(kScheduleTile - 1)) + sched[i] : i;
+}
+
 template <int METHOD, int NW, bool BONDS, class XT, class YT>
 __global__ void __launch_bounds__(kBlock)
     matvec_real_kernel(OpArgs o, BondArgs bd, RatioArgs rt, GenericOrbitArgs ga,
                        RingOrbitArgs ra, IndexArgs ix, const uint32_t *__restrict__ stab,
-                       uint64_t D, const XT *__restrict__ x, YT *__restrict__ y) {
+                       uint64_t D, const XT *__restrict__ x, YT *__restrict__ y,
+                       const uint16_t *__restrict__ sched) {
   const uint64_t lb = xcd_block(blockIdx.x, gridDim.x);
-  const uint64_t i = lb * kBlock + threadIdx.x;
-  if (i >= D) return;
+  const uint64_t slot = lb * kBlock + threadIdx.x;
+  if (slot >= D) return;
+  const uint64_t i = scheduled_row(sched, slot);
   st_re(y, i, gather_row_real<METHOD, NW, BONDS>(o, bd, rt, ga, ra, ix, stab, i, x));
 }
 
@@ -46,10 +54,12 @@ template <int METHOD, int NW, bool BONDS, int K, class XT, class YT>
 __global__ void __launch_bounds__(kBlock)
     matvec_block_kernel(OpArgs o, BondArgs bd, RatioArgs rt, GenericOrbitArgs ga,
                         RingOrbitArgs ra, IndexArgs ix, const uint32_t *__restrict__ stab,
-                        uint64_t D, const XT *__restrict__ x, YT *__restrict__ y) {
+                        uint64_t D, const XT *__restrict__ x, YT *__restrict__ y,
+                        const uint16_t *__restrict__ sched) {
   const uint64_t lb = xcd_block(blockIdx.x, gridDim.x);
-  const uint64_t i = lb * kBlock + threadIdx.x;
-  if (i >= D) return;
+  const uint64_t slot = lb * kBlock + threadIdx.x;
+  if (slot >= D) return;
+  const uint64_t i = scheduled_row(sched, slot);
   double acc[K];
   gather_row_real_block<METHOD, NW, BONDS, K>(o, bd, rt, ga, ra, ix, stab, i, x, acc);
   RowK<YT, K> out;
@@ -76,6 +86,41 @@ __global__ void __launch_bounds__(kBlock)
   const uint64_t i = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
   if (i >= D) return;
   for (int c = 0; c < K; ++c) y[c * stride + i] = static_cast<B>(yr[i * K + c]);
+}
+
+// Row schedule of one tile (one workgroup per tile): count the active
+// templates of each row (pass A), then a counting sort by count, stable in
+// the row order (thread 0 scatters, so the schedule is deterministic).
+template <int NW, bool BONDS>
+__global__ void __launch_bounds__(kBlock)
+    schedule_kernel(OpArgs o, BondArgs bd, const uint64_t *__restrict__ reps, uint64_t D,
+                    uint16_t *__restrict__ sched) {
+  constexpr int kBins = 64 * NW + 1;
+  __shared__ uint16_t count_of[kScheduleTile];
+  __shared__ uint32_t offs[kBins];
+  const uint64_t base = static_cast<uint64_t>(blockIdx.x) * kScheduleTile;
+  const int rows = static_cast<int>(D - base < kScheduleTile ? D - base : kScheduleTile);
+  for (int t = threadIdx.x; t < kBins; t += kBlock) offs[t] = 0;
+  __syncthreads();
+  for (int k = threadIdx.x; k < rows; k += kBlock) {
+    uint64_t act[NW];
+    (void)row_pass_a<NW, BONDS>(o, bd, reps[base + k], act);
+    int c = 0;
+#pragma unroll
+    for (int w = 0; w < NW; ++w) c += __popcll(act[w]);
+    count_of[k] = static_cast<uint16_t>(c);
+    atomicAdd(&offs[c], 1u);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t run = 0;
+    for (int c = 0; c < kBins; ++c) {
+      const uint32_t h = offs[c];
+      offs[c] = run;
+      run += h;
+    }
+    for (int k = 0; k < rows; ++k) sched[base + offs[count_of[k]]++] = static_cast<uint16_t>(k);
+  }
 }
 
 // Rows with more than 256 off-diagonal templates: per-tuple loop.
@@ -238,12 +283,13 @@ struct Ctx {
   const uint32_t *stab;
   uint64_t D;
   bool bonds;
+  const uint16_t *sched;
 };
 
 template <int METHOD, int NW, bool BONDS, class XT, class YT>
 void launch_real_nw(const Ctx &c, const XT *x, YT *y, hipStream_t s) {
   matvec_real_kernel<METHOD, NW, BONDS, XT, YT><<<grid_for(c.D), kBlock, 0, s>>>(
-      c.o, c.bd, c.rt, c.ga, c.ra, c.ix, c.stab, c.D, x, y);
+      c.o, c.bd, c.rt, c.ga, c.ra, c.ix, c.stab, c.D, x, y, c.sched);
 }
 
 template <int METHOD, class XT, class YT>
@@ -292,7 +338,7 @@ template <int METHOD> void dispatch(const MatvecCall &m, const Ctx &c) {
 template <int METHOD, int NW, bool BONDS, int K, class XT, class YT>
 void launch_block_nw(const Ctx &c, const XT *x, YT *y, hipStream_t s) {
   matvec_block_kernel<METHOD, NW, BONDS, K, XT, YT><<<grid_for(c.D), kBlock, 0, s>>>(
-      c.o, c.bd, c.rt, c.ga, c.ra, c.ix, c.stab, c.D, x, y);
+      c.o, c.bd, c.rt, c.ga, c.ra, c.ix, c.stab, c.D, x, y, c.sched);
 }
 
 template <int METHOD, int K, class XT, class YT>
@@ -351,6 +397,7 @@ Ctx make_ctx(const Operator &op, const double *pc, const double *ps) {
   c.ix = b.index_args();
   c.stab = b.stab.ptr;
   c.D = b.D;
+  c.sched = op.schedule_built ? op.sched.ptr : nullptr;
   return c;
 }
 
@@ -450,6 +497,39 @@ void launch_rows_to_cols(VecType t, const void *yr, uint64_t D, int K, void *y, 
 }
 
 int block_chunk(uint64_t remaining) { return remaining >= 8 ? 8 : remaining >= 4 ? 4 : remaining >= 2 ? 2 : 1; }
+
+void ensure_schedule(Operator &op) {
+  std::lock_guard<std::mutex> lk(op.mu);
+  const Basis &b = *op.basis;
+  const auto &t = op.tables;
+  const int units = t.bonds ? static_cast<int>(t.h_bond_sites.size()) : t.ntemplates;
+  // automatic: bases of at least one tile with the real row-gather kernels
+  const bool want = op.schedule_request == 1 ||
+                    (op.schedule_request < 0 && b.D >= static_cast<uint64_t>(kScheduleTile));
+  if (!want || !t.is_real || units > (t.bonds ? 128 : 256) || b.D == 0) {
+    op.schedule_built = false;
+    op.sched.release();
+    return;
+  }
+  if (op.schedule_built) return;
+  hipStream_t s = stream();
+  op.sched.allocate(b.D);
+  const Ctx c = make_ctx(op, nullptr, nullptr);
+  const unsigned tiles = static_cast<unsigned>((b.D + kScheduleTile - 1) / kScheduleTile);
+  if (c.bonds && units <= 64)
+    schedule_kernel<1, true><<<tiles, kBlock, 0, s>>>(c.o, c.bd, b.reps.ptr, b.D, op.sched.ptr);
+  else if (c.bonds)
+    schedule_kernel<2, true><<<tiles, kBlock, 0, s>>>(c.o, c.bd, b.reps.ptr, b.D, op.sched.ptr);
+  else if (units <= 64)
+    schedule_kernel<1, false><<<tiles, kBlock, 0, s>>>(c.o, c.bd, b.reps.ptr, b.D, op.sched.ptr);
+  else if (units <= 128)
+    schedule_kernel<2, false><<<tiles, kBlock, 0, s>>>(c.o, c.bd, b.reps.ptr, b.D, op.sched.ptr);
+  else
+    schedule_kernel<4, false><<<tiles, kBlock, 0, s>>>(c.o, c.bd, b.reps.ptr, b.D, op.sched.ptr);
+  SPINED_CHECK_LAUNCH();
+  SPINED_HIP(hipStreamSynchronize(s));
+  op.schedule_built = true;
+}
 
 uint64_t count_offdiag_device(const Operator &op) {
   const Basis &b = *op.basis;

@@ -32,7 +32,8 @@ SPINED_SYMBOLS = (
     "spined_synchronize", "spined_basis_device_states", "spined_basis_get_stabilizers",
     "spined_operator_matmat_device", "spined_operator_matmat_rows_device",
     "spined_operator_count_offdiag", "spined_operator_orbit_method",
-    "spined_operator_set_orbit_method", "spined_basis_set_bucket_bits",
+    "spined_operator_set_orbit_method", "spined_operator_set_schedule",
+    "spined_operator_get_schedule", "spined_basis_set_bucket_bits",
     "spined_basis_get_bucket_bits", "spined_basis_set_index_kind", "spined_basis_get_index_kind",
     "spined_basis_sector_dimension", "spined_debug_orbit",
     "spined_debug_ring_available", "spined_device_synchronize", "spined_device_malloc",
@@ -144,6 +145,8 @@ def load():
         "spined_operator_count_offdiag": (I, [P, C.POINTER(U64)]),
         "spined_operator_orbit_method": (I, [P]),
         "spined_operator_set_orbit_method": (I, [P, I]),
+        "spined_operator_set_schedule": (I, [P, I]),
+        "spined_operator_get_schedule": (I, [P]),
         "spined_basis_set_bucket_bits": (I, [P, I]),
         "spined_basis_get_bucket_bits": (I, [P]),
         "spined_basis_set_index_kind": (I, [P, I]),

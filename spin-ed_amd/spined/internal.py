@@ -358,6 +358,15 @@ def orbit_method(op: Operator_) -> int:
     return int(_lib.load().spined_operator_orbit_method(op.ptr))
 
 
+def set_schedule(op: Operator_, mode: int) -> None:
+    """Row schedule of the matvec: -1 automatic, 0 off, 1 on."""
+    check_status(_lib.load().spined_operator_set_schedule(op.ptr, mode))
+
+
+def schedule(op: Operator_) -> int:
+    return int(_lib.load().spined_operator_get_schedule(op.ptr))
+
+
 def set_index_kind(basis: SpinBasis, kind: int) -> None:
     """0 = bucket table + binary search, 1 = hash table, -1 = automatic."""
     check_status(_lib.load().spined_basis_set_index_kind(basis.ptr, kind))

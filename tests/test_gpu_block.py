@@ -158,3 +158,35 @@ def test_block_eigsh_mixed_ring26():
     assert abs(r.evals[0] - ref.evals[0]) <= 1e-10 * abs(ref.evals[0])
     assert r.rnorms[0] <= 1e-10
     assert r.stats["matvecs_mixed"] > 0
+
+
+# ---- row schedule (rows of equal work side by side) ----------------------------
+@pytest.mark.parametrize("spec", [ring(20, k=0, p=0, z=1), ring(24), ring(16, k=0, p=0, z=1),
+                                  ring(12, k=0, p=0, z=None, w=None, matrix=SIGMA_XX,
+                                       name="xx12_templates")],
+                         ids=lambda s: s.name)
+def test_schedule_bit_identical(spec):
+    """The scheduled kernels compute every row exactly as the unscheduled
+    ones: single (FP64 and FP32-storage) and block matvecs are bit-identical;
+    forced on for bases smaller than one tile (a single partial tile)."""
+    print("case", spec.name, flush=True)
+    basis, op = engine(spec)
+    D = spined.get_number_states(basis)
+    rng = np.random.default_rng(17)
+    x = rng.uniform(-1, 1, D)
+    X = rng.uniform(-1, 1, (D, 4))
+    outs = []
+    for mode in (0, 1):
+        spined.set_schedule(op, mode)
+        y = spined.apply(op, x)
+        assert spined.schedule(op) == mode
+        ym = _single_matvec(op, D, x.astype(np.float32), F32, True)
+        Y = _rows_matvec(op, D, X, F64)
+        outs.append((y, ym, Y))
+    for a, b in zip(outs[0], outs[1]):
+        np.testing.assert_array_equal(a, b)
+    yr = oracle(spec).matvec(x)
+    assert np.max(np.abs(outs[1][0] - yr)) <= 1e-13 * np.max(np.abs(yr))
+    spined.set_schedule(op, -1)
+    spined.apply(op, x)
+    assert spined.schedule(op) == (1 if D >= 4096 else 0)

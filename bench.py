@@ -50,7 +50,8 @@ def parse():
     p.add_argument("--warmup", type=int, default=None)
     p.add_argument("--config", default=os.environ.get("SPINED_BENCH_CONFIG", "ring40"),
                    choices=sorted(CONFIGS))
-    p.add_argument("--precision", default="mixed", choices=["fp64", "mixed"])
+    p.add_argument("--precision", default="mixed", choices=["fp64", "mixed"],
+                   help="headline storage mode (mixed = FP32-storage phase first)")
     p.add_argument("--start", default="learned", choices=["random", "learned"])
     p.add_argument("--train-sizes", default="16,18,20,22,24,26,28,30,32")
     p.add_argument("--train-steps", type=int, default=800)
@@ -232,19 +233,29 @@ def main():
         if "fp64" in per and "mixed" in per:
             traffic = (per["fp64"] * fp64_mv + per["mixed"] * mixed_mv) / max(matvecs, 1)
 
+    # comparison solves (outside the timed region): plain FP64 from a random
+    # start, and the other storage mode from the same learned start
     compare = None
     if not a.no_compare and (a.precision != "fp64" or a.start != "random"):
-        device_synchronize()
-        tc = time.perf_counter()
-        rc = solve("fp64", use_model=False)
-        device_synchronize()
-        t_ref = time.perf_counter() - tc
-        log(f"reference fp64/random: E0={rc.evals[0]:.15f} res={rc.rnorms[0]:.2e} "
-            f"matvecs={rc.stats['matvecs']} t={t_ref:.3f}s")
-        compare = {"fp64_random_s": t_ref, "fp64_random_matvecs": rc.stats["matvecs"],
-                   "fp64_random_E0": float(rc.evals[0]),
-                   "speedup_vs_fp64_random": t_ref / per_step,
-                   "E0_difference_vs_fp64": float(last.evals[0] - rc.evals[0])}
+        compare = {}
+        runs = [("fp64_random", "fp64", False)]
+        if model is not None:
+            other = "mixed" if a.precision == "fp64" else "fp64"
+            runs.append((f"{other}_learned", other, True))
+        for tag, prec, use_model in runs:
+            device_synchronize()
+            tc = time.perf_counter()
+            rc = solve(prec, use_model=use_model)
+            device_synchronize()
+            t_ref = time.perf_counter() - tc
+            log(f"comparison {tag}: E0={rc.evals[0]:.15f} res={rc.rnorms[0]:.2e} "
+                f"matvecs={rc.stats['matvecs']} t={t_ref:.3f}s")
+            compare[f"{tag}_s"] = t_ref
+            compare[f"{tag}_matvecs"] = rc.stats["matvecs"]
+            compare[f"{tag}_E0"] = float(rc.evals[0])
+            compare[f"{tag}_residual"] = float(rc.rnorms[0])
+            compare[f"speedup_vs_{tag}"] = t_ref / per_step
+        compare["E0_difference_vs_fp64_random"] = float(last.evals[0] - compare["fp64_random_E0"])
 
     cpu = None
     if not a.no_cpu:

@@ -41,15 +41,21 @@ constexpr int OFF_W1 = 0, OFF_B1 = OFF_W1 + CH * KW, OFF_W2 = OFF_B1 + CH,
               OFF_B3 = OFF_W3 + CH * CH * KW, OFF_U = OFF_B3 + CH, OFF_U0 = OFF_U + CH,
               OFF_V = OFF_U0 + 1, OFF_V0 = OFF_V + CH, NPARAMS = OFF_V0 + 1;
 constexpr int kMaxN = 64;
+constexpr int kInPlaceMinN = 8; // layer 2 in place for n >= 8 (see cnn_forward_kernel)
 
 constexpr float GELU_A = 0.7978845608028654f;
 constexpr float GELU_B = 0.044715f;
 
+// tanh(z) = 1 - 2 / (exp(2z) + 1): one native exponential; saturates to
+// +-1 through exp's overflow / underflow, absolute error ~1e-7 (the network
+// runs on BF16 operands).
+__device__ inline float tanh_fast(float z) { return 1.0f - 2.0f / (__expf(2.0f * z) + 1.0f); }
+
 __device__ inline float gelu_f(float x) {
-  return 0.5f * x * (1.0f + tanhf(GELU_A * (x + GELU_B * x * x * x)));
+  return 0.5f * x * (1.0f + tanh_fast(GELU_A * (x + GELU_B * x * x * x)));
 }
 __device__ inline float gelu_grad_f(float x) {
-  const float t = tanhf(GELU_A * (x + GELU_B * x * x * x));
+  const float t = tanh_fast(GELU_A * (x + GELU_B * x * x * x));
   return 0.5f * (1.0f + t) + 0.5f * x * (1.0f - t * t) * GELU_A * (1.0f + 3.0f * GELU_B * x * x);
 }
 
@@ -162,8 +168,11 @@ __global__ void __launch_bounds__(64)
                        float *__restrict__ out_t, float *__restrict__ z_store) {
   extern __shared__ __attribute__((aligned(16))) __bf16 lds[];
   const int n = c.n;
+  // n >= kInPlaceMinN: layer 2 overwrites layer 1's activations in place
+  // (one 16 x n x 16 buffer; twice the resident waves per CU)
+  const bool in_place = n >= kInPlaceMinN;
   __bf16 *h1 = lds;
-  __bf16 *h2 = lds + 16 * n * CH;
+  __bf16 *h2 = in_place ? lds : lds + 16 * n * CH;
   const int lane = threadIdx.x, s = lane & 15, g = lane >> 4;
   const uint64_t base = static_cast<uint64_t>(blockIdx.x) * 16;
   const uint64_t sample = base + s;
@@ -200,14 +209,40 @@ __global__ void __launch_bounds__(64)
     load_conv_weights(P + OFF_W2, false, a);
 #pragma unroll
     for (int r = 0; r < 4; ++r) bias[r] = P[OFF_B2 + 4 * g + r];
-    for (int pos = 0; pos < n; ++pos) {
-      floatx4 acc = conv_at(a, h1, n, pos);
+    if (in_place) {
+      // slot s of h1 is last read by output s + 2, except slots n-2, n-1, 0
+      // and 1, which the wrap-around of the last outputs still reads: output
+      // pos is written when output pos + 2 is done; outputs 0, 1, n-2, n-1
+      // are written after the loop.  (One wave per workgroup: its LDS
+      // accesses stay in program order.)
+      floatx4 o0 = {0.f, 0.f, 0.f, 0.f}, o1 = o0, prev1 = o0, prev2 = o0;
+      for (int pos = 0; pos < n; ++pos) {
+        floatx4 acc = conv_at(a, h1, n, pos);
 #pragma unroll
-      for (int r = 0; r < 4; ++r) acc[r] += bias[r];
-      if (z2) store4_f32(z2, n, pos, acc);
+        for (int r = 0; r < 4; ++r) acc[r] += bias[r];
+        if (z2) store4_f32(z2, n, pos, acc);
 #pragma unroll
-      for (int r = 0; r < 4; ++r) acc[r] = gelu_f(acc[r]);
-      store4(h2, n, pos, acc);
+        for (int r = 0; r < 4; ++r) acc[r] = gelu_f(acc[r]);
+        if (pos >= 4) store4(h1, n, pos - 2, prev2);
+        if (pos == 0) o0 = acc;
+        if (pos == 1) o1 = acc;
+        prev2 = prev1;
+        prev1 = acc;
+      }
+      store4(h1, n, n - 2, prev2);
+      store4(h1, n, n - 1, prev1);
+      store4(h1, n, 0, o0);
+      store4(h1, n, 1, o1);
+    } else {
+      for (int pos = 0; pos < n; ++pos) {
+        floatx4 acc = conv_at(a, h1, n, pos);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) acc[r] += bias[r];
+        if (z2) store4_f32(z2, n, pos, acc);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) acc[r] = gelu_f(acc[r]);
+        store4(h2, n, pos, acc);
+      }
     }
   }
   __syncthreads();
@@ -526,7 +561,9 @@ __global__ void mfma_probe_kernel(float *out) {
   for (int r = 0; r < 4; ++r) out[(4 * g + r) * 16 + row] = c[r]; // D[4g+r][col]
 }
 
-size_t lds_bytes_forward(int n) { return static_cast<size_t>(2) * 16 * n * CH * 2; }
+size_t lds_bytes_forward(int n) {
+  return static_cast<size_t>(n >= kInPlaceMinN ? 1 : 2) * 16 * n * CH * 2;
+}
 size_t lds_bytes_backward(int n) { return static_cast<size_t>(3) * 16 * n * CH * 2; }
 
 } // namespace

@@ -71,6 +71,8 @@ def parse():
                    help="matvecs-only mode: row-interleaved block width")
     p.add_argument("--index", default="auto", choices=("auto", "bucket", "hash"),
                    help="index of representatives (spined_basis_set_index_kind)")
+    p.add_argument("--no-autotune", action="store_true",
+                   help="skip the matvec autotuner (run before the timed region)")
     p.add_argument("--bucket-bits", type=int, default=-1,
                    help="bucket table bits of the bucket index (-1: automatic)")
     p.add_argument("--solve-block", type=int, default=0, choices=(0, 2, 4, 8),
@@ -117,6 +119,12 @@ def main():
     log(f"{a.config}: D={D} basis build {t_build:.3f} s")
     e_off = spined.count_offdiag(op)
     log(f"off-diagonal gathers per matvec: {e_off}")
+    tune = None
+    if not a.no_autotune and not a.matvecs_only and a.bucket_bits < 0 and a.index == "auto":
+        t = time.perf_counter()
+        tune = spined.autotune(op)
+        tune["seconds"] = round(time.perf_counter() - t, 2)
+        log(f"autotune: {tune}")
 
     if a.copy_calibration_bytes:
         src_b = DeviceArray.zeros(a.copy_calibration_bytes // 8, np.float64)
@@ -278,6 +286,7 @@ def main():
         "data": "synthetic (deterministic YAML-equivalent deck; random start vector seed 0x5EED)",
         "config": {"workload": desc, "dimension": D, "precision": a.precision,
                    "start": a.start, "tolerance": a.tol, "basis_build_s": round(t_build, 3),
+                   "autotune": tune,
                    "learned_start": train_info},
         "comparison": compare,
         "result": {"E0": float(last.evals[0]), "residual": float(last.rnorms[0]),

@@ -152,6 +152,24 @@ int spined_basis_get_bucket_bits(const ls_spin_basis *basis);
  * lookup); -1 = automatic.  Setting it on a built basis rebuilds the index. */
 int spined_basis_set_index_kind(ls_spin_basis *basis, int kind);
 int spined_basis_get_index_kind(const ls_spin_basis *basis);
+/* Autotuner: times the operator's real matvec on its own basis for every
+ * candidate setting and keeps the fastest.  Candidates: index kind (bucket
+ * table / hash), bucket-table size (bits ceil(log2 D) .. +2), row schedule
+ * on/off; then the per-vector time of the block matvec for K = 1, 2, 4, 8.
+ * Settings needing more device memory than is free (minus 10%) are
+ * skipped.  Each timing is the mean of `repeats` launches after one warm-up.
+ * The chosen setting stays on the basis / operator. */
+typedef struct spined_autotune_report {
+  int32_t index_kind;         /* chosen: 0 bucket table, 1 hash */
+  int32_t bucket_bits;        /* chosen (bucket table) */
+  int32_t schedule;           /* chosen: 1 row schedule on */
+  int32_t candidates;         /* settings timed */
+  double ms_per_matvec;       /* chosen setting, FP64 x */
+  double ms_per_matvec_first; /* the setting in force before tuning */
+  double ms_per_vector_block[4]; /* block matvec, K = 1, 2, 4, 8 (0: not timed) */
+  int32_t best_block;         /* K with the lowest per-vector time */
+} spined_autotune_report;
+int spined_operator_autotune(ls_operator *op, int32_t repeats, spined_autotune_report *report);
 /* Exact sector dimension by character counting (before ls_build). */
 int spined_basis_sector_dimension(const ls_spin_basis *basis, double *out);
 

@@ -602,6 +602,117 @@ int spined_operator_matmat_rows_device(const ls_operator *op, int dtype, uint64_
   });
 }
 
+namespace {
+// Mean device time of `repeats` FP64 matvecs (block width K) after one warm-up.
+double time_matvec(Operator &o, int K, int repeats, double *x, double *y) {
+  hipStream_t s = stream();
+  const uint64_t n = o.basis->D;
+  auto launch = [&] {
+    if (K == 1)
+      launch_matvec(MatvecCall{&o, VecType::F64, VecType::F64, x, y, n, s});
+    else
+      launch_matvec_block(MatvecBlockCall{&o, VecType::F64, VecType::F64, x, y, n, K, s});
+  };
+  launch();
+  hipEvent_t a, b;
+  SPINED_HIP(hipEventCreate(&a));
+  SPINED_HIP(hipEventCreate(&b));
+  SPINED_HIP(hipEventRecord(a, s));
+  for (int r = 0; r < repeats; ++r) launch();
+  SPINED_HIP(hipEventRecord(b, s));
+  SPINED_HIP(hipEventSynchronize(b));
+  float ms = 0.f;
+  SPINED_HIP(hipEventElapsedTime(&ms, a, b));
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  return ms / repeats;
+}
+
+uint64_t free_device_bytes() {
+  size_t f = 0, t = 0;
+  SPINED_HIP(hipMemGetInfo(&f, &t));
+  return f;
+}
+} // namespace
+
+int spined_operator_autotune(ls_operator *op, int32_t repeats, spined_autotune_report *report) {
+  return guarded([&] {
+    Operator &o = prepare_operator(op);
+    Basis &b = *o.basis;
+    if (!report) throw Error(LS_INVALID_ARGUMENT, "NULL report");
+    if (!o.tables.is_real) throw Error(LS_INVALID_DATATYPE, "the autotuner times real operators");
+    if (repeats < 1) repeats = 2;
+    *report = spined_autotune_report{};
+    const uint64_t n = b.D;
+    if (n == 0) return;
+    // block widths whose two FP64 blocks fit in half the free memory
+    int kmax = matvec_block_supported(o) ? 8 : 1;
+    while (kmax > 1 && 2 * n * kmax * 8 > free_device_bytes() / 2) kmax /= 2;
+    DeviceBuffer<double> x(n * kmax), y(n * kmax);
+    launch_random_fill(x.ptr, n * kmax, 0x7E57, stream());
+    int lg = 0;
+    while ((1ull << lg) < n) ++lg;
+    struct Setting {
+      int kind, bits, sched;
+    };
+    const Setting first{b.index_kind, b.index_kind == 0 ? b.bucket_bits : -1,
+                        o.schedule_built ? 1 : 0};
+    auto apply = [&](const Setting &st) {
+      {
+        std::lock_guard<std::mutex> lk(b.mu);
+        b.index_kind_request = st.kind;
+        b.bucket_bits_request = st.bits;
+        build_index(b);
+      }
+      {
+        std::lock_guard<std::mutex> lk(o.mu);
+        o.schedule_request = st.sched;
+        o.schedule_built = false;
+        o.sched.release();
+      }
+      ensure_schedule(o);
+    };
+    report->ms_per_matvec_first = time_matvec(o, 1, repeats, x.ptr, y.ptr);
+    std::vector<Setting> cands;
+    for (int sched = 0; sched <= 1; ++sched) {
+      for (int bits = lg; bits <= lg + 2; ++bits) cands.push_back({0, bits, sched});
+      cands.push_back({1, -1, sched});
+    }
+    Setting best = first;
+    double best_ms = report->ms_per_matvec_first;
+    for (const Setting &c : cands) {
+      // memory of the candidate index: ~2^(bits+1) * 4 B (bucket), 2^(lg+1) * 8 B (hash)
+      const uint64_t need = c.kind == 1 ? (2ull << lg) * 8 : (2ull << std::min(c.bits, 33)) * 4;
+      const uint64_t have = free_device_bytes() + (b.buckets.n + b.slots.n * 2) * 4;
+      if (need > have - have / 10) continue;
+      apply(c);
+      const double ms = time_matvec(o, 1, repeats, x.ptr, y.ptr);
+      ++report->candidates;
+      if (ms < best_ms) {
+        best_ms = ms;
+        best = Setting{c.kind, c.kind == 0 ? b.bucket_bits : -1, c.sched};
+      }
+    }
+    apply(best);
+    report->index_kind = b.index_kind;
+    report->bucket_bits = b.bucket_bits;
+    report->schedule = o.schedule_built ? 1 : 0;
+    report->ms_per_matvec = time_matvec(o, 1, repeats, x.ptr, y.ptr);
+    double best_pv = 0.0;
+    const int Ks[4] = {1, 2, 4, 8};
+    for (int q = 0; q < 4; ++q) {
+      if (Ks[q] > kmax) break;
+      const double pv =
+          Ks[q] == 1 ? report->ms_per_matvec : time_matvec(o, Ks[q], repeats, x.ptr, y.ptr) / Ks[q];
+      report->ms_per_vector_block[q] = pv;
+      if (best_pv == 0.0 || pv < best_pv) {
+        best_pv = pv;
+        report->best_block = Ks[q];
+      }
+    }
+  });
+}
+
 int spined_operator_count_offdiag(const ls_operator *op, uint64_t *out) {
   return guarded([&] {
     Operator &o = prepare_operator(op);

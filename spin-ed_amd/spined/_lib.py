@@ -41,8 +41,21 @@ SPINED_SYMBOLS = (
     "spined_cnn_create", "spined_cnn_destroy", "spined_cnn_get_params", "spined_cnn_set_params",
     "spined_cnn_forward_device", "spined_cnn_backward_device", "spined_cnn_adam_step",
     "spined_cnn_start_vector", "spined_debug_mfma_probe", "spined_debug_copy_device",
-    "spined_batch_eigsh",
+    "spined_batch_eigsh", "spined_operator_autotune",
 )
+
+
+class SpinedAutotuneReport(C.Structure):
+    _fields_ = [
+        ("index_kind", C.c_int32),
+        ("bucket_bits", C.c_int32),
+        ("schedule", C.c_int32),
+        ("candidates", C.c_int32),
+        ("ms_per_matvec", C.c_double),
+        ("ms_per_matvec_first", C.c_double),
+        ("ms_per_vector_block", C.c_double * 4),
+        ("best_block", C.c_int32),
+    ]
 
 
 class SpinedEigshOptions(C.Structure):
@@ -146,6 +159,7 @@ def load():
         "spined_operator_orbit_method": (I, [P]),
         "spined_operator_set_orbit_method": (I, [P, I]),
         "spined_operator_set_schedule": (I, [P, I]),
+        "spined_operator_autotune": (I, [P, C.c_int32, C.POINTER(SpinedAutotuneReport)]),
         "spined_operator_get_schedule": (I, [P]),
         "spined_basis_set_bucket_bits": (I, [P, I]),
         "spined_basis_get_bucket_bits": (I, [P]),

@@ -358,6 +358,16 @@ def orbit_method(op: Operator_) -> int:
     return int(_lib.load().spined_operator_orbit_method(op.ptr))
 
 
+def autotune(op: Operator_, repeats: int = 2) -> dict:
+    """Time the matvec over index kind, bucket-table size and row schedule
+    (the fastest stays in force) and the block matvec over K = 1, 2, 4, 8."""
+    rep = _lib.SpinedAutotuneReport()
+    check_status(_lib.load().spined_operator_autotune(op.ptr, repeats, C.byref(rep)))
+    out = {f: getattr(rep, f) for f, _ in _lib.SpinedAutotuneReport._fields_}
+    out["ms_per_vector_block"] = {k: v for k, v in zip((1, 2, 4, 8), rep.ms_per_vector_block) if v}
+    return out
+
+
 def set_schedule(op: Operator_, mode: int) -> None:
     """Row schedule of the matvec: -1 automatic, 0 off, 1 on."""
     check_status(_lib.load().spined_operator_set_schedule(op.ptr, mode))

@@ -93,3 +93,19 @@ def test_bucket_sizes_bit_identical(spec):
     assert spined.bucket_bits(basis) == default_bits
     yr = o.matvec(x)
     assert np.max(np.abs(y0 - yr)) <= 1e-13 * np.max(np.abs(yr))
+
+
+def test_autotune_keeps_results_exact():
+    spec = ring(24)
+    basis, op = engine(spec)
+    D = spined.get_number_states(basis)
+    x = np.random.default_rng(8).uniform(-1, 1, D)
+    y0 = spined.apply(op, x)
+    rep = spined.autotune(op, repeats=2)
+    print(rep, flush=True)
+    assert rep["candidates"] >= 4
+    assert rep["ms_per_matvec"] > 0 and rep["ms_per_matvec_first"] > 0
+    assert rep["best_block"] in (1, 2, 4, 8)
+    assert set(rep["ms_per_vector_block"]) == {1, 2, 4, 8}
+    assert spined.index_kind(basis) == rep["index_kind"]
+    np.testing.assert_array_equal(spined.apply(op, x), y0)

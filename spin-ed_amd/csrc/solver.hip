@@ -216,6 +216,12 @@ template <class VT>
 __global__ void __launch_bounds__(kBlock)
     sub_dot_kernel(const VT *const *__restrict__ V, int nvec, const double *__restrict__ h1,
                    double *__restrict__ w, uint64_t n, double *__restrict__ partial) {
+  // pointers hoisted out of the row loop; slots past nvec read V[0] (a valid
+  // address) so every load is unconditional and the loads of a row are all
+  // in flight together; their values are masked out of the arithmetic
+  const VT *vp[kFuseMax];
+#pragma unroll
+  for (int q = 0; q < kFuseMax; ++q) vp[q] = V[q < nvec ? q : 0];
   double acc[kFuseMax];
 #pragma unroll
   for (int q = 0; q < kFuseMax; ++q) acc[q] = 0.0;
@@ -223,16 +229,14 @@ __global__ void __launch_bounds__(kBlock)
        r += static_cast<uint64_t>(gridDim.x) * kBlock) {
     double vals[kFuseMax];
 #pragma unroll
-    for (int q = 0; q < kFuseMax; ++q) vals[q] = q < nvec ? to_d(V[q][r]) : 0.0;
+    for (int q = 0; q < kFuseMax; ++q) vals[q] = to_d(vp[q][r]);
     double sub = 0.0;
 #pragma unroll
-    for (int q = 0; q < kFuseMax; ++q)
-      if (q < nvec) sub = fma(h1[q], vals[q], sub);
+    for (int q = 0; q < kFuseMax; ++q) sub = q < nvec ? fma(h1[q], vals[q], sub) : sub;
     const double wr = w[r] - sub;
     w[r] = wr;
 #pragma unroll
-    for (int q = 0; q < kFuseMax; ++q)
-      if (q < nvec) acc[q] = fma(vals[q], wr, acc[q]);
+    for (int q = 0; q < kFuseMax; ++q) acc[q] = q < nvec ? fma(vals[q], wr, acc[q]) : acc[q];
   }
   block_partials<kFuseMax>(acc, nvec, partial);
 }
@@ -243,11 +247,18 @@ template <class VT>
 __global__ void __launch_bounds__(kBlock)
     sub_norm_kernel(const VT *const *__restrict__ V, int nvec, const double *__restrict__ h2,
                     double *__restrict__ w, uint64_t n, double *__restrict__ partial) {
+  const VT *vp[kFuseMax];
+#pragma unroll
+  for (int q = 0; q < kFuseMax; ++q) vp[q] = V[q < nvec ? q : 0];
   double acc[1] = {0.0};
   for (uint64_t r = blockIdx.x * static_cast<uint64_t>(kBlock) + threadIdx.x; r < n;
        r += static_cast<uint64_t>(gridDim.x) * kBlock) {
+    double vals[kFuseMax];
+#pragma unroll
+    for (int q = 0; q < kFuseMax; ++q) vals[q] = to_d(vp[q][r]);
     double sub = 0.0;
-    for (int i = 0; i < nvec; ++i) sub = fma(h2[i], to_d(V[i][r]), sub);
+#pragma unroll
+    for (int q = 0; q < kFuseMax; ++q) sub = q < nvec ? fma(h2[q], vals[q], sub) : sub;
     const double wr = w[r] - sub;
     w[r] = wr;
     acc[0] = fma(wr, wr, acc[0]);

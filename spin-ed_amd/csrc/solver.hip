@@ -870,6 +870,58 @@ bool final_phase(Operator &op, uint64_t n, int m, int keep, uint64_t k, double t
   return converged;
 }
 
+// Real operators on at most kMaxBasis states: H column by column from unit
+// vectors (D matvecs), then the one-workgroup Jacobi eigensolver on the
+// symmetrised dense matrix.  Exact, including degenerate levels (which a
+// single-vector recurrence returns once per eigenspace).
+void dense_small_eigsh(Operator &op, uint64_t n, uint64_t k, int dtype, double *evals,
+                       void *evecs, uint64_t ld, double *rnorms, spined_stats &st,
+                       hipStream_t s) {
+  std::vector<double> H(n * n), e(n, 0.0);
+  DeviceBuffer<double> dx(n), dy(n);
+  for (uint64_t j = 0; j < n; ++j) {
+    std::fill(e.begin(), e.end(), 0.0);
+    e[j] = 1.0;
+    SPINED_HIP(hipMemcpyAsync(dx.ptr, e.data(), 8 * n, hipMemcpyHostToDevice, s));
+    launch_matvec(MatvecCall{&op, VecType::F64, VecType::F64, dx.ptr, dy.ptr, n, s});
+    SPINED_HIP(hipMemcpyAsync(H.data() + j * n, dy.ptr, 8 * n, hipMemcpyDeviceToHost, s));
+    SPINED_HIP(hipStreamSynchronize(s));
+  }
+  st.matvecs += static_cast<int64_t>(n);
+  // T = (H + H^T) / 2 in the projected-matrix layout (ld = kMaxBasis)
+  std::vector<double> T(kMaxBasis * kMaxBasis, 0.0);
+  for (uint64_t a = 0; a < n; ++a)
+    for (uint64_t b = 0; b < n; ++b) T[a * kMaxBasis + b] = 0.5 * (H[a * n + b] + H[b * n + a]);
+  DeviceBuffer<double> dT(kMaxBasis * kMaxBasis), dth(kMaxBasis), dS(kMaxBasis * kMaxBasis);
+  SPINED_HIP(hipMemcpyAsync(dT.ptr, T.data(), 8 * T.size(), hipMemcpyHostToDevice, s));
+  jacobi_eigh_kernel<<<1, 256, 0, s>>>(dT.ptr, kMaxBasis, static_cast<int>(n), dth.ptr, dS.ptr, 60);
+  SPINED_CHECK_LAUNCH();
+  std::vector<double> th(n), S(n * n);
+  SPINED_HIP(hipMemcpyAsync(th.data(), dth.ptr, 8 * n, hipMemcpyDeviceToHost, s));
+  SPINED_HIP(hipMemcpyAsync(S.data(), dS.ptr, 8 * n * n, hipMemcpyDeviceToHost, s));
+  SPINED_HIP(hipStreamSynchronize(s));
+  for (uint64_t i = 0; i < k; ++i) {
+    const double *v = S.data() + i * n; // eigenvector i (column i of S)
+    double r2 = 0.0;
+    for (uint64_t a = 0; a < n; ++a) {
+      double hv = 0.0;
+      for (uint64_t b = 0; b < n; ++b) hv += H[b * n + a] * v[b];
+      r2 += (hv - th[i] * v[a]) * (hv - th[i] * v[a]);
+    }
+    evals[i] = th[i];
+    if (rnorms) rnorms[i] = std::sqrt(r2);
+    if (evecs) {
+      if (dtype == LS_FLOAT64) {
+        double *o = static_cast<double *>(evecs) + i * ld;
+        for (uint64_t a = 0; a < n; ++a) o[a] = v[a];
+      } else {
+        float *o = static_cast<float *>(evecs) + i * ld;
+        for (uint64_t a = 0; a < n; ++a) o[a] = static_cast<float>(v[a]);
+      }
+    }
+  }
+}
+
 int eigsh(Operator &op, int dtype, uint64_t k, double tol, const spined_eigsh_options &opts,
           double *evals, void *evecs, uint64_t ld, double *rnorms, spined_monitor_fn monitor,
           void *ctx) {
@@ -890,6 +942,13 @@ int eigsh(Operator &op, int dtype, uint64_t k, double tol, const spined_eigsh_op
   int m = 0, keep = 0;
   pick_sizes(n, k, opts, m, keep);
   spined_stats st{};
+  if (!cplx && n <= static_cast<uint64_t>(kMaxBasis)) {
+    dense_small_eigsh(op, n, k, dtype, evals, evecs, ld, rnorms, st, s);
+    st.seconds_total = seconds_since(t0);
+    std::lock_guard<std::mutex> lk(op.mu);
+    op.last_stats = st;
+    return LS_SUCCESS;
+  }
 
   // ---- start vector (FP64, in a scratch buffer) --------------------------
   DeviceBuffer<double> start(ne);

@@ -125,8 +125,6 @@ def diagonalize(cfg: UserConfig, dtype, model=None):
     """diagonalize (SpinED.hs:370-411): lowest number_vectors eigenpairs,
     written to /hamiltonian/{eigenvalues, eigenvectors, residuals}."""
     dtype = np.dtype(dtype)
-    if dtype.kind == "c":
-        raise NotImplementedError("complex Hamiltonians are not yet supported by spined_eigsh")
     log.debug("Using max_primme_basis_size=%d, max_primme_block_size=%d, "
               "min_primme_restart_size=%d", cfg.max_basis_size, cfg.max_block_size,
               cfg.min_restart_size)
@@ -149,7 +147,7 @@ def diagonalize(cfg: UserConfig, dtype, model=None):
     if res.status != 0:
         log.warning("eigensolver stopped before reaching the tolerance: residuals %s", res.rnorms)
     log.info("Obtained eigenvalues %s", res.evals)
-    real_t = np.float32 if dtype == np.float32 else np.float64
+    real_t = np.float32 if dtype in (np.float32, np.complex64) else np.float64
     evals = res.evals.astype(real_t)
     rnorms = res.rnorms.astype(real_t)
     with hdf5.File(cfg.output, "a") as f:
@@ -164,7 +162,11 @@ def compute_expectations(cfg: UserConfig, evecs: np.ndarray) -> None:
     """computeExpectations / computeExpectation (SpinED.hs:296-313)."""
     for o in cfg.observables:
         log.info("Computing expectation values of %s...", o.name)
-        m = I.expectation(o.obj, np.asfortranarray(evecs))
+        x = evecs
+        if not I.is_operator_real(o.obj) and not np.iscomplexobj(evecs):
+            # a complex observable in a real sector: promote the eigenvectors
+            x = evecs.astype(np.complex64 if evecs.dtype == np.float32 else np.complex128)
+        m = I.expectation(o.obj, np.asfortranarray(x))
         log.info("Obtained expectation values %s", m)
         with hdf5.File(cfg.output, "a") as f:
             f.write(OBSERVABLES_PATH + "/" + o.name, m.astype(np.complex128))

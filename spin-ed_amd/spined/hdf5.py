@@ -176,6 +176,9 @@ class File:
         """writeDataset (SpinED.hs:354-360): overwrite with a warning."""
         L = lib()
         data = np.ascontiguousarray(data)
+        parent = path.rsplit("/", 1)[0]
+        if parent:
+            self.create_group(parent)
         if self.exists(path):
             log.warning("Overwriting %s...", path)
             L.H5Ldelete(self.fid, path.encode(), H5P_DEFAULT)

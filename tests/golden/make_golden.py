@@ -1,0 +1,110 @@
+"""Writes the golden fixtures of tests/golden from the CPU oracle.
+
+Inputs: the example decks that ship with SpinED (example/*.yaml), parsed by
+this package's own YAML front end (spined.config, the mirror of the
+reference's parser).  The reference's arithmetic lives in libraries absent
+from this image, so the expected outputs come from the oracle, whose every
+function is pinned independently (tests/test_oracle_pins.py).
+
+Each fixture <deck>.npz holds only arrays (numpy.load with allow_pickle=False):
+  deck          JSON text of the parsed inputs (spins, weight, inversion,
+                symmetries, terms as [re, im] matrices and site lists)
+  dimension, reps_xor, reps_sum      representatives: count and checksums
+  sample_idx, sample_rep, sample_stab   sampled rows (all rows when D <= 4096)
+  x_seeds       counter-hash seeds of x (pyoracle.random_vector): x = re (+ i im)
+  y_idx, y_re, y_im, y_sum_re, y_sum_im, y_norm2   sampled rows and sums of H x
+  evals         lowest eigenvalues (dense LAPACK for D <= 6000, else the
+                oracle's Lanczos for real sectors; empty when neither applies)
+
+Run from the repository root:  python tests/golden/make_golden.py
+"""
+import json
+import sys
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parents[2]
+sys.path[:0] = [str(ROOT / "oracle"), str(ROOT / "spin-ed_amd")]
+import pyoracle  # noqa: E402
+from spined import config  # noqa: E402
+
+EXAMPLES = Path("/root/reference/example")
+DECKS = ["heisenberg_chain_4", "heisenberg_chain_10", "heisenberg_kagome_12",
+         "heisenberg_square_4x4", "heisenberg_triangular_19", "xxz_triangular_19",
+         "heisenberg_square_5x5"]
+OUT = Path(__file__).resolve().parent
+SAMPLE = 4096
+SEEDS = (20261202, 777)
+
+
+def deck_inputs(spec):
+    b = spec.basis
+    return {
+        "number_spins": b.number_spins,
+        "hamming_weight": b.hamming_weight,
+        "spin_inversion": b.spin_inversion,
+        "symmetries": [[s.permutation, s.sector] for s in b.symmetries],
+        "terms": [{"matrix": [[[z.real, z.imag] for z in row] for row in t.matrix],
+                   "sites": t.sites} for t in spec.hamiltonian.terms],
+    }
+
+
+def oracle_system(inp):
+    o = pyoracle.System(inp["number_spins"], inp["hamming_weight"], inp["spin_inversion"],
+                        [(p, s) for p, s in inp["symmetries"]])
+    for t in inp["terms"]:
+        m = np.array([[complex(re, im) for re, im in row] for row in t["matrix"]])
+        o.add_interaction(m, t["sites"])
+    return o
+
+
+def dense(o):
+    D = o.dimension
+    H = np.empty((D, D), dtype=np.complex128)
+    for j in range(D):
+        e = np.zeros(D, dtype=np.complex128)
+        e[j] = 1.0
+        H[:, j] = o.matvec(e)
+    return H
+
+
+def make(name):
+    spec = config.read_config(str(EXAMPLES / f"{name}.yaml"))
+    inp = deck_inputs(spec)
+    o = oracle_system(inp)
+    o.build()
+    reps, stab = o.basis()
+    D = len(reps)
+    rng = np.random.default_rng(D)
+    idx = np.arange(D) if D <= SAMPLE else np.sort(rng.choice(D, SAMPLE, replace=False))
+    x = pyoracle.random_vector(SEEDS[0], D)
+    if not o.is_real:
+        x = x + 1j * pyoracle.random_vector(SEEDS[1], D)
+    y = o.matvec(x)
+    yi = y[idx]
+    if D <= 6000:
+        evals = np.linalg.eigvalsh(dense(o))[:min(4, D)]
+    elif o.is_real:
+        evals = o.lanczos(k=1, tol=1e-12, max_iter=600)["evals"]
+    else:
+        evals = np.empty(0)
+    np.savez_compressed(
+        OUT / f"{name}.npz",
+        deck=np.array(json.dumps(inp)),
+        dimension=np.array(D, dtype=np.int64),
+        reps_xor=np.array(np.bitwise_xor.reduce(reps) if D else 0, dtype=np.uint64),
+        reps_sum=np.array(np.sum(reps, dtype=np.uint64), dtype=np.uint64),
+        sample_idx=idx.astype(np.int64), sample_rep=reps[idx], sample_stab=stab[idx],
+        x_seeds=np.array(SEEDS, dtype=np.uint64),
+        y_idx=idx.astype(np.int64), y_re=np.real(yi).astype(np.float64),
+        y_im=np.imag(yi).astype(np.float64),
+        y_sum_re=np.array(np.sum(np.real(y))), y_sum_im=np.array(np.sum(np.imag(y))),
+        y_norm2=np.array(np.vdot(y, y).real),
+        evals=np.asarray(evals, dtype=np.float64))
+    print(f"{name}: D={D} real={o.is_real} evals={np.asarray(evals)[:2]}")
+
+
+if __name__ == "__main__":
+    for d in (sys.argv[1:] or DECKS):
+        make(d)

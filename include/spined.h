@@ -230,6 +230,13 @@ int spined_debug_mfma_probe(float *out_host);
 int spined_debug_orbit(const ls_spin_basis *basis, int method, uint64_t count,
                        const uint64_t *states, uint64_t *reps, int32_t *phases, int32_t *counts,
                        uint8_t *valid);
+/* Ring orbit of states[q] with its spins at sites[q] and sites[q] + 1
+ * (mod n) exchanged (they must differ), computed from the run data of
+ * states[q] -- the path of the nearest-neighbour row gather -- on the host.
+ * For CPU-side verification against the full orbit scan. */
+int spined_debug_orbit_hop(const ls_spin_basis *basis, uint64_t count, const uint64_t *states,
+                           const int32_t *sites, uint64_t *reps, int32_t *phases, int32_t *counts,
+                           uint8_t *valid);
 /* 1 if the basis' group is a ring (rotation/reflection) group. */
 int spined_debug_ring_available(const ls_spin_basis *basis);
 /* Plain device copy of `bytes` (multiple of 16): the calibration kernel for

@@ -810,6 +810,32 @@ int spined_debug_orbit(const ls_spin_basis *basis, int method, uint64_t count,
   });
 }
 
+int spined_debug_orbit_hop(const ls_spin_basis *basis, uint64_t count, const uint64_t *states,
+                           const int32_t *sites, uint64_t *reps, int32_t *phases, int32_t *counts,
+                           uint8_t *valid) {
+  return guarded([&] {
+    if (!basis || (count && (!states || !sites || !reps))) throw Error(LS_INVALID_ARGUMENT, "NULL");
+    const Basis &b = *basis->p;
+    if (!b.tables.ring_available) throw Error(LS_INVALID_ARGUMENT, "ring method unavailable");
+    const auto ra = b.tables.ring_args(false);
+    const int n = ra.n;
+    for (uint64_t q = 0; q < count; ++q) {
+      const uint64_t r = states[q];
+      const int i = sites[q];
+      const int j = i + 1 == n ? 0 : i + 1;
+      if (i < 0 || i >= n || (((r >> i) ^ (r >> j)) & 1) == 0)
+        throw Error(LS_INVALID_ARGUMENT, "sites i and i+1 must hold different spins");
+      const RunInfo ri0 = run_info(~r & ra.mask, n, ra.mask);
+      const RunInfo ri1 = run_info(r, n, ra.mask);
+      const OrbitResult o = orbit_ring_hop(ra, ri0, ri1, r, i);
+      reps[q] = o.rep;
+      if (phases) phases[q] = o.phase;
+      if (counts) counts[q] = o.count;
+      if (valid) valid[q] = o.valid ? 1 : 0;
+    }
+  });
+}
+
 int spined_debug_copy_device(void *dst, const void *src, uint64_t bytes) {
   return guarded([&] {
     if (!dst || !src || bytes % 16) throw Error(LS_INVALID_ARGUMENT, "bytes must be a multiple of 16");

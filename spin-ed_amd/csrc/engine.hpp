@@ -153,6 +153,10 @@ struct OperatorTables {
   std::vector<uint32_t> h_bond_sites;        // s0 | s1 << 8
   std::vector<double> h_bond_dp, h_bond_da, h_bond_o;
   std::vector<uint64_t> h_bond_flip;
+  // every bond joins ring neighbours (i, i+1 mod n): its lower site i
+  bool nn_ring = false;
+  std::vector<uint8_t> h_bond_lo;
+  DeviceBuffer<uint8_t> bond_lo;
   DeviceBuffer<uint32_t> bond_sites;
   DeviceBuffer<double> bond_dp, bond_da, bond_o;
   DeviceBuffer<uint64_t> bond_flip;

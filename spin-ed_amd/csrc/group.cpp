@@ -423,6 +423,21 @@ void compile_operator(Operator &op) {
     t.h_bond_o.clear();
     t.h_bond_flip.clear();
   }
+  // nearest-neighbour ring bonds: the row gather may take the hop orbit path
+  t.nn_ring = t.bonds && !t.h_bond_sites.empty();
+  t.h_bond_lo.clear();
+  const int n = op.basis ? op.basis->n : 0;
+  for (uint32_t bs : t.h_bond_sites) {
+    if (!t.nn_ring) break;
+    const int s0 = static_cast<int>(bs & 0xffu), s1 = static_cast<int>(bs >> 8);
+    if (n >= 3 && s1 == (s0 + 1) % n)
+      t.h_bond_lo.push_back(static_cast<uint8_t>(s0));
+    else if (n >= 3 && s0 == (s1 + 1) % n)
+      t.h_bond_lo.push_back(static_cast<uint8_t>(s1));
+    else
+      t.nn_ring = false;
+  }
+  if (!t.nn_ring) t.h_bond_lo.clear();
 }
 
 void ensure_device(OperatorTables &t) {
@@ -441,6 +456,7 @@ void ensure_device(OperatorTables &t) {
   upload(t.bond_da, t.h_bond_da);
   upload(t.bond_o, t.h_bond_o);
   upload(t.bond_flip, t.h_bond_flip);
+  upload(t.bond_lo, t.h_bond_lo);
   t.on_device = true;
 }
 

@@ -390,7 +390,8 @@ Ctx make_ctx(const Operator &op, const double *pc, const double *ps) {
   c.o = op_args(op, pc, ps);
   c.bonds = t.bonds;
   c.bd = BondArgs{t.bond_sites.ptr, t.bond_dp.ptr, t.bond_da.ptr, t.bond_o.ptr, t.bond_flip.ptr,
-                  static_cast<int32_t>(t.h_bond_sites.size())};
+                  static_cast<int32_t>(t.h_bond_sites.size()),
+                  t.nn_ring && b.tables.ring_available ? t.bond_lo.ptr : nullptr};
   c.rt = RatioArgs{b.tables.div_idx.ptr, b.tables.ratio.ptr, b.tables.nd};
   c.ga = b.tables.generic_args(true);
   c.ra = b.tables.ring_args(true);

@@ -252,13 +252,11 @@ __host__ __device__ inline void ring_candidates(const RingOrbitArgs &a, uint64_t
   }
 }
 
-__host__ __device__ inline OrbitResult orbit_ring(const RingOrbitArgs &a, uint64_t x) {
-  const int n = a.n;
+// Candidate stage of the ring orbit, given the tops and lengths of the
+// longest zero runs (t0, l0) and, with spin inversion, one runs (t1, l1).
+__host__ __device__ inline OrbitResult orbit_ring_tops(const RingOrbitArgs &a, uint64_t x,
+                                                       uint64_t t0, int l0, uint64_t t1, int l1) {
   const uint64_t xn = ~x & a.mask;
-  int l0 = 0, l1 = -1;
-  const uint64_t t0 = longest_run_tops(xn, n, a.mask, l0); // zero runs of x
-  uint64_t t1 = 0;
-  if (a.has_flip) t1 = longest_run_tops(x, n, a.mask, l1); // one runs of x
   const int lm = l0 > l1 ? l0 : l1;
   uint64_t best = ~0ull;
   int32_t ph = 0, cnt = 0;
@@ -266,6 +264,143 @@ __host__ __device__ inline OrbitResult orbit_ring(const RingOrbitArgs &a, uint64
   if (l0 == lm) ring_candidates(a, x, t0, l0, 0, best, ph, cnt, ok);
   if (a.has_flip && l1 == lm) ring_candidates(a, xn, t1, l1, a.flip_phase, best, ph, cnt, ok);
   return OrbitResult{best, ph, cnt, ok};
+}
+
+__host__ __device__ inline OrbitResult orbit_ring(const RingOrbitArgs &a, uint64_t x) {
+  const int n = a.n;
+  const uint64_t xn = ~x & a.mask;
+  int l0 = 0, l1 = -1;
+  const uint64_t t0 = longest_run_tops(xn, n, a.mask, l0); // zero runs of x
+  uint64_t t1 = 0;
+  if (a.has_flip) t1 = longest_run_tops(x, n, a.mask, l1); // one runs of x
+  return orbit_ring_tops(a, x, t0, l0, t1, l1);
+}
+
+// ---------------------------------------------------------------------------
+// Ring orbits of hop states.  In the row gather of a nearest-neighbour
+// exchange Hamiltonian, every off-diagonal state is the row's representative
+// r with one particle moved across a bond (i, i+1 mod n).  A move changes
+// only the two runs next to the bond, so the longest-run tops of the moved
+// state follow from the row's run data: the tops of r's runs of exact length
+// L (the longest) and L-1.  Every other run is unchanged.
+// ---------------------------------------------------------------------------
+struct RunInfo {
+  uint64_t EL;  // tops of runs of ones of length exactly L
+  uint64_t EL1; // tops of runs of ones of length exactly L - 1 (L >= 2)
+  int32_t L;    // longest run length
+  bool usable;  // 0 < popcount < n (else the moved state takes the full scan)
+};
+
+// Tops of windows of k >= 1 ones (q with ones at q, q-1, .., q-k+1), from
+// the doubling levels lv[s] = windows of 2^s ones.
+__host__ __device__ inline uint64_t window_tops(const uint64_t (&lv)[6], int k, int n,
+                                                uint64_t mask) {
+  uint64_t cur = mask;
+  int len = 0;
+#pragma unroll
+  for (int s = 5; s >= 0; --s) {
+    if (k & (1 << s)) {
+      cur &= rotl_n(lv[s], len, n, mask);
+      len += 1 << s;
+    }
+  }
+  return cur;
+}
+
+__host__ __device__ inline RunInfo run_info(uint64_t z, int n, uint64_t mask) {
+  RunInfo ri{0, 0, 0, false};
+  if (z == 0 || z == mask) return ri;
+  int L = 0;
+  ri.EL = longest_run_tops_bf(z, n, mask, L);
+  ri.L = L;
+  ri.usable = true;
+  if (L >= 2) {
+    uint64_t lv[6];
+    lv[0] = z;
+#pragma unroll
+    for (int s = 1; s < 6; ++s)
+      lv[s] = (1 << s) <= n - 1 ? lv[s - 1] & rotl_n(lv[s - 1], 1 << (s - 1), n, mask) : 0;
+    const int k = L - 1;
+    // exact length k: window of k ones ending at q, zero above q and below q - k + 1
+    ri.EL1 = window_tops(lv, k, n, mask) & ~rotr_n(z, 1, n, mask) & ~rotl_n(z, k, n, mask);
+  }
+  return ri;
+}
+
+// Consecutive ones of z at positions p, p-1, ... (down) or p, p+1, ... (up),
+// cyclically; z != mask.
+__host__ __device__ inline int ones_down(uint64_t z, int p, int n, uint64_t mask) {
+  const uint64_t y = ~rotl_n(z, n - 1 - p, n, mask) & mask; // bit p at n - 1, inverted
+  return y ? __builtin_clzll(y << (64 - n)) : n;
+}
+__host__ __device__ inline int ones_up(uint64_t z, int p, int n, uint64_t mask) {
+  const uint64_t y = ~rotr_n(z, p, n, mask) & mask; // bit p at 0, inverted
+  return y ? ctz64(y) : n;
+}
+
+__host__ __device__ inline uint64_t bit_at(int q) { return 1ull << q; }
+
+// Tops and length of the longest runs of ones of z' = z with its bits i and
+// j = (i + 1) mod n exchanged (they differ).  False: the bond's two runs are
+// one run around the ring, or z has no usable run data; take the full scan.
+__host__ __device__ inline bool hop_tops(const RunInfo &ri, uint64_t z, int i, int j, int n,
+                                         uint64_t mask, uint64_t &tops, int &len) {
+  if (!ri.usable) return false;
+  const int im1 = i == 0 ? n - 1 : i - 1;
+  int a, b, topR1, newTopR1, topR2, newTopR2;
+  if ((z >> i) & 1) { // the one at i moves up to j
+    a = ones_down(z, i, n, mask);
+    const int jp1 = j + 1 == n ? 0 : j + 1;
+    b = ones_up(z, jp1, n, mask);
+    topR1 = i;
+    newTopR1 = im1;
+    const int t = j + b;
+    topR2 = t >= n ? t - n : t;
+    newTopR2 = topR2;
+  } else { // the one at j moves down to i
+    a = ones_up(z, j, n, mask);
+    b = ones_down(z, im1, n, mask);
+    const int t = j + a - 1;
+    topR1 = t >= n ? t - n : t;
+    newTopR1 = topR1;
+    topR2 = im1;
+    newTopR2 = i;
+  }
+  if (a + b >= n - 1) return false;
+  const int L = ri.L;
+  uint64_t el = ri.EL, el1 = ri.EL1;
+  if (a == L) el &= ~bit_at(topR1);
+  if (a == L - 1) el1 &= ~bit_at(topR1);
+  if (b > 0 && b == L) el &= ~bit_at(topR2);
+  if (b > 0 && b == L - 1) el1 &= ~bit_at(topR2);
+  int lp = b + 1;
+  if (a - 1 > lp) lp = a - 1;
+  if (el1 && L - 1 > lp) lp = L - 1;
+  if (el && L > lp) lp = L;
+  uint64_t t = 0;
+  if (L == lp) t |= el;
+  if (L - 1 == lp) t |= el1;
+  if (a >= 2 && a - 1 == lp) t |= bit_at(newTopR1);
+  if (b + 1 == lp) t |= bit_at(newTopR2);
+  tops = t;
+  len = lp;
+  return true;
+}
+
+// Orbit of r with the bits at i and j = (i + 1) mod n exchanged (they
+// differ), from the row's run data ri0 (zero runs of r) and ri1 (one runs).
+// Same result as orbit_ring(a, r ^ (bit i | bit j)).
+__host__ __device__ inline OrbitResult orbit_ring_hop(const RingOrbitArgs &a, const RunInfo &ri0,
+                                                      const RunInfo &ri1, uint64_t r, int i) {
+  const int n = a.n;
+  const int j = i + 1 == n ? 0 : i + 1;
+  const uint64_t x = r ^ (bit_at(i) | bit_at(j));
+  uint64_t t0 = 0, t1 = 0;
+  int l0 = 0, l1 = -1;
+  bool ok = hop_tops(ri0, ~r & a.mask, i, j, n, a.mask, t0, l0);
+  if (a.has_flip) ok = ok && hop_tops(ri1, r, i, j, n, a.mask, t1, l1);
+  if (!ok) return orbit_ring(a, x);
+  return orbit_ring_tops(a, x, t0, l0, t1, l1);
 }
 
 // SplitMix64 finaliser (Steele, Lea & Flood 2014): the counter hash of the

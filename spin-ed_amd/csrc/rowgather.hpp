@@ -113,6 +113,9 @@ struct BondArgs {
   const double *__restrict__ o;
   const uint64_t *__restrict__ flip;
   int32_t nbonds;
+  // non-null when every bond joins ring neighbours: lower site of each bond
+  // (the ring-orbit gather then takes the hop path, orbit_ring_hop)
+  const uint8_t *__restrict__ nn_lo;
 };
 
 // sqrt(|Stab r'| / |Stab r|) from a table over the divisors of |G|
@@ -120,6 +123,28 @@ struct RatioArgs {
   const uint16_t *__restrict__ div_idx;
   const double *__restrict__ ratio;
   int32_t nd;
+};
+
+// Orbit of the row state r with template j applied.  Ring orbits of
+// nearest-neighbour bond moves take the hop path from the row's run data
+// (computed once per row); everything else the full scan.  Same results.
+template <int METHOD, bool BONDS> struct RowOrbit {
+  RunInfo ri0{0, 0, 0, false}, ri1{0, 0, 0, false};
+  __device__ RowOrbit(const RingOrbitArgs &ra, const BondArgs &bd, uint64_t r) {
+    if constexpr (METHOD == 1 && BONDS) {
+      if (bd.nn_lo) {
+        ri0 = run_info(~r & ra.mask, ra.n, ra.mask);
+        if (ra.has_flip) ri1 = run_info(r, ra.n, ra.mask);
+      }
+    }
+  }
+  __device__ OrbitResult orbit(const OpArgs &o, const BondArgs &bd, const GenericOrbitArgs &ga,
+                               const RingOrbitArgs &ra, uint64_t r, int j) const {
+    if constexpr (METHOD == 1 && BONDS) {
+      if (bd.nn_lo) return orbit_ring_hop(ra, ri0, ri1, r, bd.nn_lo[j]);
+    }
+    return do_orbit<METHOD>(ga, ra, r ^ (BONDS ? bd.flip[j] : o.flip[j]));
+  }
 };
 
 // Index lookups of two orbit representatives and the gathers of their x
@@ -204,6 +229,7 @@ __device__ inline double gather_row_real(const OpArgs &o, const BondArgs &bd, co
   const uint64_t r = ix.reps[i];
   uint64_t act[NW];
   const double diag = row_pass_a<NW, BONDS>(o, bd, r, act);
+  const RowOrbit<METHOD, BONDS> rob(ra, bd, r);
   double acc = diag * ld_re(x, i);
   const uint32_t si = stab[i];
   const double sr = static_cast<double>(si);
@@ -219,9 +245,9 @@ __device__ inline double gather_row_real(const OpArgs &o, const BondArgs &bd, co
   while (any_set<NW>(act)) {
     const int ja = pop_first<NW>(act);
     const int jb = pop_first<NW>(act);
-    const OrbitResult oa = do_orbit<METHOD>(ga, ra, r ^ (BONDS ? bd.flip[ja] : o.flip[ja]));
+    const OrbitResult oa = rob.orbit(o, bd, ga, ra, r, ja);
     OrbitResult ob{0, 0, 0, false};
-    if (jb >= 0) ob = do_orbit<METHOD>(ga, ra, r ^ (BONDS ? bd.flip[jb] : o.flip[jb]));
+    if (jb >= 0) ob = rob.orbit(o, bd, ga, ra, r, jb);
     int64_t ka, kb;
     double xa, xb;
     lookup_gather2(ix, oa, ob, [&](int64_t q) { return ld_re(x, static_cast<uint64_t>(q)); }, ka,
@@ -254,6 +280,7 @@ __device__ inline void gather_row_real_block(const OpArgs &o, const BondArgs &bd
   const uint64_t r = ix.reps[i];
   uint64_t act[NW];
   const double diag = row_pass_a<NW, BONDS>(o, bd, r, act);
+  const RowOrbit<METHOD, BONDS> rob(ra, bd, r);
   {
     const Row xi = xr[i];
 #pragma unroll
@@ -271,9 +298,9 @@ __device__ inline void gather_row_real_block(const OpArgs &o, const BondArgs &bd
   while (any_set<NW>(act)) {
     const int ja = pop_first<NW>(act);
     const int jb = pop_first<NW>(act);
-    const OrbitResult oa = do_orbit<METHOD>(ga, ra, r ^ (BONDS ? bd.flip[ja] : o.flip[ja]));
+    const OrbitResult oa = rob.orbit(o, bd, ga, ra, r, ja);
     OrbitResult ob{0, 0, 0, false};
-    if (jb >= 0) ob = do_orbit<METHOD>(ga, ra, r ^ (BONDS ? bd.flip[jb] : o.flip[jb]));
+    if (jb >= 0) ob = rob.orbit(o, bd, ga, ra, r, jb);
     int64_t ka, kb;
     Row xa, xb;
     lookup_gather2(ix, oa, ob, [&](int64_t q) { return xr[q]; }, ka, kb, xa, xb);

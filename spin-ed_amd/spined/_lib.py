@@ -36,7 +36,8 @@ SPINED_SYMBOLS = (
     "spined_operator_get_schedule", "spined_basis_set_bucket_bits",
     "spined_basis_get_bucket_bits", "spined_basis_set_index_kind", "spined_basis_get_index_kind",
     "spined_basis_sector_dimension", "spined_debug_orbit",
-    "spined_debug_ring_available", "spined_device_synchronize", "spined_device_malloc",
+    "spined_debug_ring_available", "spined_debug_orbit_hop", "spined_device_synchronize",
+    "spined_device_malloc",
     "spined_device_free", "spined_memcpy", "spined_memset", "spined_cnn_num_params",
     "spined_cnn_create", "spined_cnn_destroy", "spined_cnn_get_params", "spined_cnn_set_params",
     "spined_cnn_forward_device", "spined_cnn_backward_device", "spined_cnn_adam_step",
@@ -168,6 +169,7 @@ def load():
         "spined_basis_sector_dimension": (I, [P, C.POINTER(D)]),
         "spined_debug_orbit": (I, [P, I, U64, P, P, P, P, P]),
         "spined_debug_ring_available": (I, [P]),
+        "spined_debug_orbit_hop": (I, [P, U64, P, P, P, P, P, P]),
         "spined_device_synchronize": (I, []),
         "spined_device_malloc": (I, [PP, U64]),
         "spined_device_free": (I, [P]),

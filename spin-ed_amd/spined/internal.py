@@ -408,5 +408,21 @@ def debug_orbit(basis: SpinBasis, states, method: int):
     return reps, ph, cnt, val.astype(bool)
 
 
+def debug_orbit_hop(basis: SpinBasis, states, sites):
+    """Host run of the nearest-neighbour hop orbit (test hook): the orbit of
+    states[q] with its spins at sites[q] and sites[q] + 1 (mod n) exchanged."""
+    s = np.ascontiguousarray(states, dtype=np.uint64)
+    i = np.ascontiguousarray(sites, dtype=np.int32)
+    n = len(s)
+    reps = np.empty(n, np.uint64)
+    ph = np.empty(n, np.int32)
+    cnt = np.empty(n, np.int32)
+    val = np.empty(n, np.uint8)
+    check_status(_lib.load().spined_debug_orbit_hop(basis.ptr, n, s.ctypes.data, i.ctypes.data,
+                                                    reps.ctypes.data, ph.ctypes.data,
+                                                    cnt.ctypes.data, val.ctypes.data))
+    return reps, ph, cnt, val.astype(bool)
+
+
 def ring_available(basis: SpinBasis) -> bool:
     return bool(_lib.load().spined_debug_ring_available(basis.ptr))

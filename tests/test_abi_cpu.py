@@ -168,3 +168,54 @@ def test_build_unsafe_rejects_unsorted_input():
     with pytest.raises(LatticeSymmetriesException) as e:
         spined.build_basis(b, np.array([7, 11, 13, 12], dtype=np.uint64))
     assert e.value.code == 9
+
+
+@pytest.mark.parametrize("n,w,z,gens", [
+    (4, None, 1, lattice.ring_symmetries(4, 0, 0)),
+    (5, None, None, lattice.ring_symmetries(5, 2, None)),
+    (6, None, -1, lattice.ring_symmetries(6, 3, 1)),
+    (8, None, 1, lattice.ring_symmetries(8, 0, None)),
+    (13, None, None, lattice.ring_symmetries(13, 0, 0)),
+    (13, None, 1, lattice.ring_symmetries(13, 4, None)),
+    (16, None, 1, lattice.ring_symmetries(16, 8, 1)),
+    (24, None, -1, lattice.ring_symmetries(24, 12, 1)),
+    (31, None, None, lattice.ring_symmetries(31, 7, None)),
+    (40, None, 1, lattice.ring_symmetries(40, 0, 0)),
+    (64, None, 1, lattice.ring_symmetries(64, 0, 0)),
+])
+def test_hop_orbit_matches_full_scan(n, w, z, gens):
+    """The nearest-neighbour hop path (run data of the unmoved state, updated
+    for the two runs at the bond) gives exactly the full ring scan's
+    (representative, phase, count, valid), for every bond of states of every
+    weight, including the one-run cases that fall back to the full scan."""
+    b = spined.mk_basis(spined.mk_group([spined.mk_symmetry(p, s) for p, s in gens]), n, w, z)
+    assert spined.ring_available(b)
+    rng = np.random.default_rng(n)
+    mask = (1 << n) - 1
+    states, sites = [], []
+    if n <= 13:
+        pool = range(1 << n)
+    else:
+        pool = []
+        for wt in list(range(1, 4)) + list(range(n // 2 - 2, n // 2 + 3)) + list(range(n - 3, n)):
+            for _ in range(60):
+                idx = rng.choice(n, wt, replace=False)
+                pool.append(sum(1 << int(q) for q in idx))
+    for r in pool:
+        for i in range(n):
+            j = (i + 1) % n
+            if ((r >> i) ^ (r >> j)) & 1:
+                states.append(r)
+                sites.append(i)
+    states = np.array(states, dtype=np.uint64)
+    sites = np.array(sites, dtype=np.int32)
+    moved = states ^ ((np.uint64(1) << sites.astype(np.uint64)) |
+                      (np.uint64(1) << ((sites.astype(np.uint64) + np.uint64(1)) % np.uint64(n))))
+    h = spined.debug_orbit_hop(b, states, sites)
+    f = spined.debug_orbit(b, moved, 1)
+    np.testing.assert_array_equal(h[3], f[3])
+    np.testing.assert_array_equal(h[0], f[0])
+    np.testing.assert_array_equal(h[2], f[2])
+    v = f[3]
+    np.testing.assert_array_equal(h[1][v], f[1][v])
+    assert len(states) > 0

@@ -141,8 +141,9 @@ int spined_operator_set_orbit_method(ls_operator *op, int method);
  * 1 on; built at the next matvec.  get: 1 if a schedule is in use. */
 int spined_operator_set_schedule(ls_operator *op, int mode);
 int spined_operator_get_schedule(const ls_operator *op);
-/* Bucket bits of the bucket index (2^bits buckets over the representatives'
- * range, 1..34; -1 = automatic: ceil(log2 D) - 1).  Setting it on a built
+/* Bucket bits of the bucket index (about 2^bits level-2 buckets of 4 bytes,
+ * 1..36; -1 = automatic: ceil(log2 D) + 2).  The build lowers it until the
+ * table takes at most half the free device memory.  Setting it on a built
  * basis rebuilds the index. */
 int spined_basis_set_bucket_bits(ls_spin_basis *basis, int bits);
 int spined_basis_get_bucket_bits(const ls_spin_basis *basis);
@@ -152,12 +153,21 @@ int spined_basis_get_bucket_bits(const ls_spin_basis *basis);
  * lookup); -1 = automatic.  Setting it on a built basis rebuilds the index. */
 int spined_basis_set_index_kind(ls_spin_basis *basis, int kind);
 int spined_basis_get_index_kind(const ls_spin_basis *basis);
+/* 1 when the built basis holds every valid orbit minimum of its sector (an
+ * ls_build basis, or an ls_build_unsafe list of valid orbit minima as long as
+ * the sector dimension), 0 otherwise, -1 before a build.  On a complete
+ * basis with an operator that keeps the Hamming weight the bucket search
+ * skips its confirming load of the representative: every valid image is
+ * known to be present. */
+int spined_basis_is_complete(const ls_spin_basis *basis);
 /* Autotuner: times the operator's real matvec on its own basis for every
- * candidate setting and keeps the fastest.  Candidates: index kind (bucket
- * table / hash), bucket-table size (bits ceil(log2 D) .. +2), row schedule
- * on/off; then the per-vector time of the block matvec for K = 1, 2, 4, 8.
- * Settings needing more device memory than is free (minus 10%) are
- * skipped.  Each timing is the mean of `repeats` launches after one warm-up.
+ * candidate setting and keeps the fastest.  Candidates: bucket-table sizes
+ * (bits ceil(log2 D) + 1 .. + 4) with the row schedule, the default size
+ * without it, and the hash table when the bucket search must confirm its
+ * hits (spined_basis_is_complete); then the per-vector time of the block
+ * matvec for K = 1, 2, 4, 8.  Index tables larger than a quarter of the
+ * free device memory are skipped, leaving room for the eigensolver's
+ * vectors.  Each timing is the mean of `repeats` launches after one warm-up.
  * The chosen setting stays on the basis / operator. */
 typedef struct spined_autotune_report {
   int32_t index_kind;         /* chosen: 0 bucket table, 1 hash */

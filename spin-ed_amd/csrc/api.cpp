@@ -673,18 +673,26 @@ int spined_operator_autotune(ls_operator *op, int32_t repeats, spined_autotune_r
       ensure_schedule(o);
     };
     report->ms_per_matvec_first = time_matvec(o, 1, repeats, x.ptr, y.ptr);
+    // bucket searches that need no confirming load beat the hash table
+    const bool exact = b.complete && (b.w < 0 || o.tables.keeps_weight);
     std::vector<Setting> cands;
-    for (int sched = 0; sched <= 1; ++sched) {
-      for (int bits = lg; bits <= lg + 2; ++bits) cands.push_back({0, bits, sched});
-      cands.push_back({1, -1, sched});
-    }
+    for (int bits = lg + 1; bits <= std::min(lg + 4, kMaxBucketBits); ++bits)
+      cands.push_back({0, bits, 1});
+    cands.push_back({0, lg + 2, 0});
+    if (!exact) cands.push_back({1, -1, 1});
+    // an index may take a quarter of the memory free outside the tuner's
+    // own vectors and the current index: the rest is the eigensolver's
+    const uint64_t budget =
+        (free_device_bytes() + 16 * n * kmax + (b.buckets.n + b.slots.n * 2) * 4) / 4;
     Setting best = first;
     double best_ms = report->ms_per_matvec_first;
     for (const Setting &c : cands) {
-      // memory of the candidate index: ~2^(bits+1) * 4 B (bucket), 2^(lg+1) * 8 B (hash)
-      const uint64_t need = c.kind == 1 ? (2ull << lg) * 8 : (2ull << std::min(c.bits, 33)) * 4;
-      const uint64_t have = free_device_bytes() + (b.buckets.n + b.slots.n * 2) * 4;
-      if (need > have - have / 10) continue;
+      uint64_t need = (2ull << lg) * 8;                                // hash slots
+      if (c.kind == 0) {
+        const uint64_t nb = level2_size(b, c.bits, nullptr);
+        need = 4 * (nb ? nb + 1 : (2ull << c.bits));
+      }
+      if (need > budget) continue;
       apply(c);
       const double ms = time_matvec(o, 1, repeats, x.ptr, y.ptr);
       ++report->candidates;
@@ -751,7 +759,8 @@ int spined_operator_get_schedule(const ls_operator *op) {
 
 int spined_basis_set_bucket_bits(ls_spin_basis *basis, int bits) {
   return guarded([&] {
-    if (!basis || bits < -1 || bits > 34) throw Error(LS_INVALID_ARGUMENT, "bad argument");
+    if (!basis || bits < -1 || bits > kMaxBucketBits)
+      throw Error(LS_INVALID_ARGUMENT, "bad argument");
     Basis &b = *basis->p;
     std::lock_guard<std::mutex> lk(b.mu);
     b.bucket_bits_request = bits;
@@ -777,6 +786,12 @@ int spined_basis_get_index_kind(const ls_spin_basis *basis) {
   if (!basis) return -1;
   const Basis &b = *basis->p;
   return b.built ? b.index_kind : (b.index_kind_request >= 0 ? b.index_kind_request : 0);
+}
+
+int spined_basis_is_complete(const ls_spin_basis *basis) {
+  if (!basis) return -1;
+  const Basis &b = *basis->p;
+  return b.built ? (b.complete ? 1 : 0) : -1;
 }
 
 int spined_basis_sector_dimension(const ls_spin_basis *basis, double *out) {

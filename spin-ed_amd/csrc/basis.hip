@@ -153,18 +153,22 @@ __global__ void __launch_bounds__(kBlock)
   }
 }
 
+// |Stab r| per representative; *bad becomes nonzero when an entry is not a
+// valid orbit minimum of the sector's Hamming weight (w < 0: any weight).
 template <int METHOD>
 __global__ void __launch_bounds__(kBlock)
     stabilizer_kernel(GenericOrbitArgs ga, RingOrbitArgs ra, const uint64_t *__restrict__ reps,
-                      uint64_t D, uint32_t *__restrict__ stab) {
+                      uint64_t D, int w, uint32_t *__restrict__ stab, uint32_t *bad) {
   const uint64_t i = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
   if (i >= D) return;
+  const uint64_t v = reps[i];
   OrbitResult r;
   if constexpr (METHOD == 1)
-    r = orbit_ring(ra, reps[i]);
+    r = orbit_ring(ra, v);
   else
-    r = orbit_generic(ga, reps[i]);
+    r = orbit_generic(ga, v);
   stab[i] = static_cast<uint32_t>(r.count);
+  if (!r.valid || r.rep != v || (w >= 0 && __popcll(v) != w)) atomicOr(bad, 1u);
 }
 
 // seg[s] = lower_bound(reps, s << shift) for s < nseg, D at s = nseg
@@ -209,7 +213,7 @@ __global__ void __launch_bounds__(kBlock)
     uint64_t lo_s = 0, n_s = nseg; // upper_bound of b over bases, minus one
     while (n_s > 0) {
       const uint64_t half = n_s >> 1;
-      if (dir[lo_s + half].x <= b) {
+      if (dir_base(dir[lo_s + half]) <= b) {
         lo_s += half + 1;
         n_s -= half + 1;
       } else {
@@ -218,7 +222,7 @@ __global__ void __launch_bounds__(kBlock)
     }
     const uint64_t sg = lo_s - 1;
     const uint2 d = dir[sg];
-    const uint64_t key = (sg << shift) + ((b - d.x) << d.y);
+    const uint64_t key = (sg << shift) + ((b - dir_base(d)) << dir_lsh(d));
     uint64_t lo = seg[sg], n = seg[sg + 1] - seg[sg];
     while (n > 0) {
       const uint64_t half = n >> 1;
@@ -274,15 +278,25 @@ uint64_t binomial(int n, int k) {
 
 unsigned grid_for(uint64_t n) { return static_cast<unsigned>((n + kBlock - 1) / kBlock); }
 
-void compute_stabilizers(Basis &b, hipStream_t s) {
-  if (b.D == 0) return;
+// Fills b.stab; returns true when every entry is a valid orbit minimum of
+// the sector (synchronises the stream).
+bool compute_stabilizers(Basis &b, hipStream_t s) {
+  if (b.D == 0) return true;
   const auto ga = b.tables.generic_args(true);
   const auto ra = b.tables.ring_args(true);
+  DeviceBuffer<uint32_t> bad(1);
+  SPINED_HIP(hipMemsetAsync(bad.ptr, 0, 4, s));
   if (b.tables.ring_available)
-    stabilizer_kernel<1><<<grid_for(b.D), kBlock, 0, s>>>(ga, ra, b.reps.ptr, b.D, b.stab.ptr);
+    stabilizer_kernel<1><<<grid_for(b.D), kBlock, 0, s>>>(ga, ra, b.reps.ptr, b.D, b.w, b.stab.ptr,
+                                                        bad.ptr);
   else
-    stabilizer_kernel<0><<<grid_for(b.D), kBlock, 0, s>>>(ga, ra, b.reps.ptr, b.D, b.stab.ptr);
+    stabilizer_kernel<0><<<grid_for(b.D), kBlock, 0, s>>>(ga, ra, b.reps.ptr, b.D, b.w, b.stab.ptr,
+                                                        bad.ptr);
   SPINED_CHECK_LAUNCH();
+  uint32_t h = 1;
+  SPINED_HIP(hipMemcpyAsync(&h, bad.ptr, 4, hipMemcpyDeviceToHost, s));
+  SPINED_HIP(hipStreamSynchronize(s));
+  return h == 0;
 }
 
 } // namespace
@@ -322,12 +336,17 @@ void build_index(Basis &b) {
   }
   b.slots.release();
   b.hmask = 0;
-  // level-2 size: about 2^bits buckets, bits = ceil(log2 D) + 1 by default
-  // (two to four buckets per representative; about one binary-search step)
+  b.buckets.release();
+  b.dir.release();
+  // level-2 size: about 2^bits buckets, bits = ceil(log2 D) + 2 by default
+  // (four to eight buckets per representative: most lookups land in a
+  // bucket that holds only their target)
   int lg = 0;
   while ((1ull << lg) < b.D) ++lg;
-  int bits = b.bucket_bits_request >= 0 ? b.bucket_bits_request : lg + 1;
-  bits = std::max(1, std::min(34, bits));
+  int bits = b.bucket_bits_request >= 0 ? b.bucket_bits_request : lg + 2;
+  bits = std::max(1, std::min(kMaxBucketBits, bits));
+  size_t free_bytes = 0, total_bytes = 0;
+  SPINED_HIP(hipMemGetInfo(&free_bytes, &total_bytes));
   uint64_t last = 0;
   if (b.D) SPINED_HIP(hipMemcpy(&last, b.reps.ptr + (b.D - 1), 8, hipMemcpyDeviceToHost));
   const int bitlen = last ? 64 - __builtin_clzll(last) : 1;
@@ -340,26 +359,14 @@ void build_index(Basis &b) {
   DeviceBuffer<uint32_t> seg(b.nseg + 1);
   segment_start_kernel<<<cap(b.nseg + 1), kBlock, 0, s>>>(b.reps.ptr, b.D, b.shift, b.nseg, seg.ptr);
   SPINED_CHECK_LAUNCH();
-  std::vector<uint32_t> hseg(b.nseg + 1);
-  SPINED_HIP(hipMemcpyAsync(hseg.data(), seg.ptr, 4 * (b.nseg + 1), hipMemcpyDeviceToHost, s));
+  b.h_segstart.assign(b.nseg + 1, 0);
+  SPINED_HIP(hipMemcpyAsync(b.h_segstart.data(), seg.ptr, 4 * (b.nseg + 1), hipMemcpyDeviceToHost,
+                            s));
   SPINED_HIP(hipStreamSynchronize(s));
-  // segment s gets 2^lb buckets, lb = ceil(log2(count_s * 2^bits / D)),
-  // clamped to [0, shift]; the total must fit the 32-bit bases
+  // the table may take at most half the free memory
   std::vector<uint2> hdir(std::max<uint64_t>(b.nseg, 1));
-  uint64_t total = 0;
-  for (;;) {
-    const double ratio = b.D ? std::ldexp(1.0, bits) / static_cast<double>(b.D) : 1.0;
-    total = 0;
-    for (uint64_t sg = 0; sg < b.nseg; ++sg) {
-      const double want = static_cast<double>(hseg[sg + 1] - hseg[sg]) * ratio;
-      int lb = 0;
-      while (lb < b.shift && std::ldexp(1.0, lb) < want) ++lb;
-      hdir[sg] = make_uint2(static_cast<uint32_t>(total), static_cast<uint32_t>(b.shift - lb));
-      total += 1ull << lb;
-    }
-    if (total < 0xffffffffull || bits <= 1) break;
-    --bits;
-  }
+  uint64_t total = level2_size(b, bits, hdir.data());
+  while (4 * (total + 1) > free_bytes / 2 && bits > 1) total = level2_size(b, --bits, hdir.data());
   b.bucket_bits = bits;
   b.nbuckets = total;
   b.dir.allocate(hdir.size());
@@ -370,6 +377,23 @@ void build_index(Basis &b) {
                                                       b.shift, b.nbuckets, b.buckets.ptr);
   SPINED_CHECK_LAUNCH();
   SPINED_HIP(hipStreamSynchronize(s));
+}
+
+// Level-2 buckets for about 2^bits in total: segment s gets 2^lb_s,
+// lb_s = ceil(log2(count_s * 2^bits / D)) clamped to [0, shift].  Uses the
+// segment starts of the last bucket-table build; fills dir when given.
+uint64_t level2_size(const Basis &b, int bits, uint2 *dir) {
+  if (b.nseg == 0 || b.h_segstart.size() != b.nseg + 1) return 0;
+  const double ratio = b.D ? std::ldexp(1.0, bits) / static_cast<double>(b.D) : 1.0;
+  uint64_t total = 0;
+  for (uint64_t sg = 0; sg < b.nseg; ++sg) {
+    const double want = static_cast<double>(b.h_segstart[sg + 1] - b.h_segstart[sg]) * ratio;
+    int lb = 0;
+    while (lb < b.shift && std::ldexp(1.0, lb) < want) ++lb;
+    if (dir) dir[sg] = dir_entry(total, static_cast<uint32_t>(b.shift - lb));
+    total += 1ull << lb;
+  }
+  return total;
 }
 
 void build_basis(Basis &b) {
@@ -434,8 +458,7 @@ void build_basis(Basis &b) {
                                        std::to_string(capacity));
   b.D = emitted;
   b.stab.allocate(std::max<uint64_t>(b.D, 1));
-  compute_stabilizers(b, s);
-  SPINED_HIP(hipStreamSynchronize(s));
+  b.complete = compute_stabilizers(b, s);
   build_index(b);
   b.host_reps.reset();
   b.built = true;
@@ -464,8 +487,12 @@ void adopt_basis(Basis &b, uint64_t n, const uint64_t *reps) {
   if (n) SPINED_HIP(hipMemcpy(b.reps.ptr, reps, n * 8, hipMemcpyHostToDevice));
   b.D = n;
   b.stab.allocate(std::max<uint64_t>(n, 1));
-  compute_stabilizers(b, s);
-  SPINED_HIP(hipStreamSynchronize(s));
+  // the list is the whole sector when each entry is a valid orbit minimum
+  // (ascending, so distinct) and their number is the sector dimension
+  const bool minima = compute_stabilizers(b, s);
+  double im = 0.0;
+  const double dim = sector_dimension(b, &im);
+  b.complete = minima && std::fabs(im) < 0.5 && std::llround(dim) == static_cast<long long>(n);
   build_index(b);
   b.host_reps = std::make_shared<std::vector<uint64_t>>(reps, reps + n);
   b.built = true;

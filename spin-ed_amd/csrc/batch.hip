@@ -399,7 +399,7 @@ int batch_eigsh(const std::vector<Operator *> &ops, double tol, int m_req, uint6
     I.rt = RatioArgs{b.tables.div_idx.ptr, b.tables.ratio.ptr, b.tables.nd};
     I.ga = b.tables.generic_args(true);
     I.ra = b.tables.ring_args(true);
-    I.ix = b.index_args();
+    I.ix = b.index_args(t.keeps_weight);
     I.stab = b.stab.ptr;
     I.D = b.D;
     I.V = Vpool.ptr + voff[i] * static_cast<uint64_t>(m + 1);

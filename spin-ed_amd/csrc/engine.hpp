@@ -79,6 +79,9 @@ struct OrbitTables {
 void build_orbit_tables(const Group &g, int n, int spin_inversion, OrbitTables &t);
 void ensure_device(OrbitTables &t);
 
+// Largest level-2 size of the bucket index: about 2^36 buckets (256 GiB).
+constexpr int kMaxBucketBits = 36;
+
 struct Basis {
   std::shared_ptr<const Group> group;
   int n = 0, w = -1, z = 0;
@@ -105,7 +108,15 @@ struct Basis {
   std::shared_ptr<std::vector<uint64_t>> host_reps; // lazily downloaded
   double seconds_build = 0.0;
 
-  IndexArgs index_args() const {
+  // reps holds every valid orbit minimum of the sector (set by the
+  // enumerating build, or by an adopted list that passed the check)
+  bool complete = false;
+  // segment starts of the last bucket-table build (nseg + 1 entries)
+  std::vector<uint32_t> h_segstart;
+
+  // images_keep_weight: the operator maps weight-w states to weight-w states
+  IndexArgs index_args(bool images_keep_weight) const {
+    const bool exact = index_kind == 0 && complete && (w < 0 || images_keep_weight);
     return IndexArgs{reps.ptr,
                      buckets.ptr,
                      dir.ptr,
@@ -113,7 +124,8 @@ struct Basis {
                      shift >= 64 ? ~0ull : (1ull << shift) - 1,
                      shift,
                      index_kind == 1 ? slots.ptr : nullptr,
-                     hmask};
+                     hmask,
+                     exact ? 1 : 0};
   }
 };
 
@@ -122,6 +134,8 @@ void build_basis(Basis &b);                                  // basis.hip
 void adopt_basis(Basis &b, uint64_t n, const uint64_t *reps); // basis.hip (ls_build_unsafe)
 void validate_representatives(const Basis &b, uint64_t n, const uint64_t *reps);
 void build_index(Basis &b);                                   // basis.hip
+// level-2 bucket count for about 2^bits buckets (fills dir when given)
+uint64_t level2_size(const Basis &b, int bits, uint2 *dir);   // basis.hip
 
 struct Interaction {
   int n = 0;
@@ -138,6 +152,7 @@ struct OperatorTables {
   int ntemplates = 0;
   int max_local = 0; // max 2^n over tuples
   bool is_real = true;
+  bool keeps_weight = true; // every template preserves the number of up spins
   // per tuple: packed sites (4 x 8 bits), n, offset into diag values and
   // into the (a -> template range) table
   std::vector<uint32_t> h_tuple_sites, h_tuple_n, h_tuple_off;

@@ -360,6 +360,9 @@ void compile_operator(Operator &op) {
           if (b == a) continue;
           const cplx m = term->matrix[b * dim + a];
           if (m == cplx(0.0, 0.0)) continue;
+          if (__builtin_popcount(static_cast<unsigned>(a)) !=
+              __builtin_popcount(static_cast<unsigned>(b)))
+            t.keeps_weight = false;
           uint64_t fl = 0;
           for (int k = 0; k < n; ++k) {
             const int bit = n - 1 - k; // first site = most significant bit

@@ -395,7 +395,7 @@ Ctx make_ctx(const Operator &op, const double *pc, const double *ps) {
   c.rt = RatioArgs{b.tables.div_idx.ptr, b.tables.ratio.ptr, b.tables.nd};
   c.ga = b.tables.generic_args(true);
   c.ra = b.tables.ring_args(true);
-  c.ix = b.index_args();
+  c.ix = b.index_args(t.keeps_weight);
   c.stab = b.stab.ptr;
   c.D = b.D;
   c.sched = op.schedule_built ? op.sched.ptr : nullptr;
@@ -539,7 +539,7 @@ uint64_t count_offdiag_device(const Operator &op) {
   const OpArgs o = op_args(op, nullptr, nullptr);
   const auto ga = b.tables.generic_args(true);
   const auto ra = b.tables.ring_args(true);
-  const IndexArgs ix = b.index_args();
+  const IndexArgs ix = b.index_args(op.tables.keeps_weight);
   const unsigned g = grid_for(b.D);
   DeviceBuffer<unsigned long long> part(g);
   if (op.orbit_method() == 1)

@@ -419,7 +419,7 @@ __host__ __device__ inline uint64_t mix64(uint64_t z) {
 //    table on the top bits alone leaves tens of entries per bucket.  Level 1
 //    splits the value range by its top bits into segments (v >> shift);
 //    segment s owns 2^lb_s level-2 buckets, lb_s chosen from its count, and
-//    dir[s] = (base_s, local shift).  Level 2: buckets[b] = first i with
+//    dir[s] = (base_s, local shift) (dir_entry).  Level 2: buckets[b] = first i with
 //    reps[i] >= start value of bucket b, in global order, so bucket b ends
 //    where b + 1 starts (also across segments); buckets[nbuckets] = D.
 //  * open-addressing hash table (slots != nullptr): 2^H 8-byte slots, H =
@@ -430,6 +430,19 @@ __host__ __device__ inline uint64_t mix64(uint64_t z) {
 // ---------------------------------------------------------------------------
 constexpr uint64_t kEmptySlot = ~0ull;
 
+// Level-1 entry: the segment's first level-2 bucket (below 2^56) and its
+// local shift, packed as x = base bits 0..31, y = base bits 32..55 << 8 | lsh.
+__host__ __device__ inline uint2 dir_entry(uint64_t base, uint32_t lsh) {
+  uint2 d;
+  d.x = static_cast<uint32_t>(base);
+  d.y = static_cast<uint32_t>(base >> 32) << 8 | lsh;
+  return d;
+}
+__host__ __device__ inline uint64_t dir_base(uint2 d) {
+  return static_cast<uint64_t>(d.y >> 8) << 32 | d.x;
+}
+__host__ __device__ inline uint32_t dir_lsh(uint2 d) { return d.y & 0xffu; }
+
 struct IndexArgs {
   const uint64_t *__restrict__ reps;
   const uint32_t *__restrict__ buckets; // level 2, nbuckets + 1 entries
@@ -439,6 +452,11 @@ struct IndexArgs {
   int32_t shift;                        // segment = v >> shift
   const uint64_t *__restrict__ slots;
   uint64_t hmask;
+  // 1 when every valid orbit minimum the operator can reach is in reps (a
+  // basis enumerated or verified complete, an operator that keeps the
+  // Hamming weight): a bucket search then narrows to the entry without the
+  // confirming load of reps.
+  int32_t exact;
 };
 
 // Level-2 bucket of v, or false when v lies beyond the last segment.
@@ -446,7 +464,7 @@ __host__ __device__ inline bool bucket_of(const IndexArgs &t, uint64_t v, uint64
   const uint64_t s = v >> t.shift;
   if (s >= t.nseg) return false;
   const uint2 d = t.dir[s];
-  b = static_cast<uint64_t>(d.x) + ((v & t.segmask) >> d.y);
+  b = dir_base(d) + ((v & t.segmask) >> dir_lsh(d));
   return true;
 }
 
@@ -493,6 +511,7 @@ __host__ __device__ inline int64_t find_index(const IndexArgs &t, uint64_t v) {
     lo = (t.reps[lo + half] <= v) ? lo + half : lo;
     n -= half;
   }
+  if (t.exact) return static_cast<int64_t>(lo);
   return t.reps[lo] == v ? static_cast<int64_t>(lo) : -1;
 }
 
@@ -534,6 +553,11 @@ __host__ __device__ inline void find_index2(const IndexArgs &t, uint64_t va, boo
       lob = (t.reps[lob + h] <= vb) ? lob + h : lob;
       nb -= h;
     }
+  }
+  if (t.exact) {
+    ka = na == 1 ? static_cast<int64_t>(loa) : -1;
+    kb = nb == 1 ? static_cast<int64_t>(lob) : -1;
+    return;
   }
   ka = (na == 1 && t.reps[loa] == va) ? static_cast<int64_t>(loa) : -1;
   kb = (nb == 1 && t.reps[lob] == vb) ? static_cast<int64_t>(lob) : -1;

@@ -35,6 +35,7 @@ SPINED_SYMBOLS = (
     "spined_operator_set_orbit_method", "spined_operator_set_schedule",
     "spined_operator_get_schedule", "spined_basis_set_bucket_bits",
     "spined_basis_get_bucket_bits", "spined_basis_set_index_kind", "spined_basis_get_index_kind",
+    "spined_basis_is_complete",
     "spined_basis_sector_dimension", "spined_debug_orbit",
     "spined_debug_ring_available", "spined_debug_orbit_hop", "spined_device_synchronize",
     "spined_device_malloc",
@@ -166,6 +167,7 @@ def load():
         "spined_basis_get_bucket_bits": (I, [P]),
         "spined_basis_set_index_kind": (I, [P, I]),
         "spined_basis_get_index_kind": (I, [P]),
+        "spined_basis_is_complete": (I, [P]),
         "spined_basis_sector_dimension": (I, [P, C.POINTER(D)]),
         "spined_debug_orbit": (I, [P, I, U64, P, P, P, P, P]),
         "spined_debug_ring_available": (I, [P]),

@@ -386,6 +386,12 @@ def index_kind(basis: SpinBasis) -> int:
     return int(_lib.load().spined_basis_get_index_kind(basis.ptr))
 
 
+def is_complete(basis: SpinBasis) -> int:
+    """1: the built basis holds every valid orbit minimum of its sector;
+    0: it does not (a partial ls_build_unsafe list); -1: not built."""
+    return int(_lib.load().spined_basis_is_complete(basis.ptr))
+
+
 def set_bucket_bits(basis: SpinBasis, bits: int) -> None:
     check_status(_lib.load().spined_basis_set_bucket_bits(basis.ptr, bits))
 

@@ -86,7 +86,7 @@ def test_bucket_sizes_bit_identical(spec):
     y0 = spined.apply(op, x)
     c0 = spined.count_offdiag(op)
     default_bits = spined.bucket_bits(basis)
-    for bits in (1, 4, default_bits + 2, 34, -1):
+    for bits in (1, 4, default_bits + 2, 36, -1):
         spined.set_bucket_bits(basis, bits)
         np.testing.assert_array_equal(spined.apply(op, x), y0, err_msg=f"bits {bits}")
         assert spined.count_offdiag(op) == c0, bits
@@ -109,3 +109,71 @@ def test_autotune_keeps_results_exact():
     assert set(rep["ms_per_vector_block"]) == {1, 2, 4, 8}
     assert spined.index_kind(basis) == rep["index_kind"]
     np.testing.assert_array_equal(spined.apply(op, x), y0)
+
+
+SX = [[0, 1], [1, 0]]
+
+
+def test_complete_flag_and_partial_lists():
+    """The bucket search skips its confirming load of reps only on a basis
+    that holds every valid orbit minimum of its sector (ls_build, or an
+    ls_build_unsafe list that passes the check); a partial list keeps the
+    load, so the images of the dropped states are not found."""
+    spec = ring(16, k=0, p=1, z=1)
+    b1, op1 = engine(spec)
+    assert spined.is_complete(b1) == 1
+    reps = spined.basis_get_states(b1)
+    D = len(reps)
+    x = np.random.default_rng(5).uniform(-1, 1, D)
+    y1 = spined.apply(op1, x)
+    b2, op2 = engine(spec, build=False)
+    assert spined.is_complete(b2) == -1
+    spined.build_basis(b2, reps)
+    assert spined.is_complete(b2) == 1
+    np.testing.assert_array_equal(spined.apply(op2, x), y1)
+    keep = np.ones(D, dtype=bool)
+    keep[::3] = False
+    part = reps[keep]
+    b3, op3 = engine(spec, build=False)
+    spined.build_basis(b3, part)
+    assert spined.is_complete(b3) == 0
+    o = oracle(spec, build=False)
+    o.set_basis(part)
+    xp = x[keep]
+    yo = o.matvec(xp)
+    for bits in (1, 6, 36, -1):
+        spined.set_bucket_bits(b3, bits)
+        y3 = spined.apply(op3, xp)
+        assert np.max(np.abs(y3 - yo)) <= 1e-13 * np.max(np.abs(yo)), bits
+    # a list as long as the sector but holding a state that is not an orbit
+    # minimum is not complete either
+    bad = reps.copy()
+    r0 = int(bad[0])
+    alt = ((r0 << 1) | (r0 >> 15)) & 0xFFFF       # a rotation of the first minimum
+    assert alt not in set(reps.tolist()) and o.orbit(alt)[0] == r0
+    bad[0] = alt
+    bad = np.sort(bad)
+    b4, _ = engine(spec, build=False)
+    spined.build_basis(b4, bad)
+    assert spined.is_complete(b4) == 0
+
+
+@pytest.mark.parametrize("w", ["half", None], ids=["weight8", "all_weights"])
+def test_weight_changing_operator(w):
+    """An operator with a transverse field leaves the Hamming-weight sector:
+    its images must be dropped on a weight-restricted basis (lookups keep
+    the confirming load) and found on a basis of all weights."""
+    spec = ring(16, k=0, p=1, z=1, w=w, name=f"ring16_sx_{w}")
+    spec.terms.append((SX, [[i] for i in range(16)]))
+    basis, op = engine(spec)
+    assert spined.is_complete(basis) == 1
+    o = oracle(spec)
+    x = np.random.default_rng(6).uniform(-1, 1, o.dimension)
+    yo = o.matvec(x)
+    y = spined.apply(op, x)
+    assert np.max(np.abs(y - yo)) <= 1e-13 * np.max(np.abs(yo))
+    # the hash index always confirms: same gathers, same sums
+    c = spined.count_offdiag(op)
+    spined.set_index_kind(basis, 1)
+    np.testing.assert_array_equal(spined.apply(op, x), y)
+    assert spined.count_offdiag(op) == c

@@ -100,6 +100,12 @@ int spined_device_synchronize(void);
  * runtime), and copies / fills on the library stream (synchronous). */
 int spined_device_malloc(void **out, uint64_t bytes);
 int spined_device_free(void *ptr);
+/* The eigensolver keeps its vector blocks (those of 256 MiB or more) after a
+ * solve and reuses them for the next solve or phase of the same size on the
+ * device; this returns them to the driver (hipFree).  The library also does
+ * so by itself before any allocation the driver would refuse.  Call it
+ * before allocating large device memory outside this library. */
+int spined_device_release_pool(void);
 /* kind: 0 host->device, 1 device->host, 2 device->device */
 int spined_memcpy(void *dst, const void *src, uint64_t bytes, int kind);
 int spined_memset(void *dst, int value, uint64_t bytes);
@@ -142,7 +148,7 @@ int spined_operator_set_orbit_method(ls_operator *op, int method);
 int spined_operator_set_schedule(ls_operator *op, int mode);
 int spined_operator_get_schedule(const ls_operator *op);
 /* Bucket bits of the bucket index (about 2^bits level-2 buckets of 4 bytes,
- * 1..36; -1 = automatic: ceil(log2 D) + 2).  The build lowers it until the
+ * 1..36; -1 = automatic: ceil(log2 D) + 3).  The build lowers it until the
  * table takes at most half the free device memory.  Setting it on a built
  * basis rebuilds the index. */
 int spined_basis_set_bucket_bits(ls_spin_basis *basis, int bits);

@@ -39,6 +39,73 @@ int current_device() {
   return d;
 }
 
+namespace {
+struct PoolBlock {
+  int device;
+  size_t bytes;
+  void *ptr;
+};
+std::mutex g_pool_mu;
+std::vector<PoolBlock> g_pool;
+
+void release_blocks(int device) {
+  std::vector<void *> drop;
+  {
+    std::lock_guard<std::mutex> lk(g_pool_mu);
+    for (size_t i = 0; i < g_pool.size();) {
+      if (g_pool[i].device == device) {
+        drop.push_back(g_pool[i].ptr);
+        g_pool[i] = g_pool.back();
+        g_pool.pop_back();
+      } else {
+        ++i;
+      }
+    }
+  }
+  for (void *p : drop) (void)hipFree(p);
+}
+} // namespace
+
+void *device_alloc(size_t bytes, bool pooled) {
+  const int d = current_device();
+  if (pooled) {
+    std::lock_guard<std::mutex> lk(g_pool_mu);
+    for (size_t i = 0; i < g_pool.size(); ++i) {
+      if (g_pool[i].device == d && g_pool[i].bytes == bytes) {
+        void *p = g_pool[i].ptr;
+        g_pool[i] = g_pool.back();
+        g_pool.pop_back();
+        return p;
+      }
+    }
+  }
+  void *p = nullptr;
+  hipError_t e = hipMalloc(&p, bytes);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    release_blocks(d);
+    e = hipMalloc(&p, bytes);
+  }
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    throw Error(LS_OUT_OF_MEMORY, "hipMalloc of " + std::to_string(bytes) +
+                                      " bytes failed: " + hipGetErrorString(e));
+  }
+  return p;
+}
+
+void device_free(void *p, size_t bytes, bool pooled, int device) {
+  if (!p) return;
+  if (pooled) {
+    std::lock_guard<std::mutex> lk(g_pool_mu);
+    g_pool.push_back(PoolBlock{device, bytes, p});
+    return;
+  }
+  (void)hipFree(p);
+}
+
+void pool_release_device() { release_blocks(current_device()); }
+
 hipStream_t stream() {
   static std::mutex mu;
   static std::vector<hipStream_t> streams;
@@ -482,14 +549,12 @@ int spined_device_synchronize(void) {
 int spined_device_malloc(void **out, uint64_t bytes) {
   return guarded([&] {
     if (!out) throw Error(LS_INVALID_ARGUMENT, "NULL argument");
-    void *p = nullptr;
-    const hipError_t e = hipMalloc(&p, bytes ? bytes : 1);
-    if (e != hipSuccess) {
-      (void)hipGetLastError();
-      throw Error(LS_OUT_OF_MEMORY, "hipMalloc failed");
-    }
-    *out = p;
+    *out = device_alloc(bytes ? bytes : 1, false);
   });
+}
+
+int spined_device_release_pool(void) {
+  return guarded([&] { pool_release_device(); });
 }
 
 int spined_device_free(void *ptr) {
@@ -678,7 +743,7 @@ int spined_operator_autotune(ls_operator *op, int32_t repeats, spined_autotune_r
     std::vector<Setting> cands;
     for (int bits = lg + 1; bits <= std::min(lg + 4, kMaxBucketBits); ++bits)
       cands.push_back({0, bits, 1});
-    cands.push_back({0, lg + 2, 0});
+    cands.push_back({0, lg + 3, 0});
     if (!exact) cands.push_back({1, -1, 1});
     // an index may take a quarter of the memory free outside the tuner's
     // own vectors and the current index: the rest is the eigensolver's

@@ -338,12 +338,12 @@ void build_index(Basis &b) {
   b.hmask = 0;
   b.buckets.release();
   b.dir.release();
-  // level-2 size: about 2^bits buckets, bits = ceil(log2 D) + 2 by default
-  // (four to eight buckets per representative: most lookups land in a
+  // level-2 size: about 2^bits buckets, bits = ceil(log2 D) + 3 by default
+  // (eight to sixteen buckets per representative: most lookups land in a
   // bucket that holds only their target)
   int lg = 0;
   while ((1ull << lg) < b.D) ++lg;
-  int bits = b.bucket_bits_request >= 0 ? b.bucket_bits_request : lg + 2;
+  int bits = b.bucket_bits_request >= 0 ? b.bucket_bits_request : lg + 3;
   bits = std::max(1, std::min(kMaxBucketBits, bits));
   size_t free_bytes = 0, total_bytes = 0;
   SPINED_HIP(hipMemGetInfo(&free_bytes, &total_bytes));

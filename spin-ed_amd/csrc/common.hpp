@@ -5,6 +5,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <atomic>
 #include <cstdint>
 #include <cstdio>
@@ -40,48 +41,73 @@ void log_line(const char *fmt, ...) __attribute__((format(printf, 1, 2)));
 hipStream_t stream();
 int current_device();
 
-// Device buffer with RAII release (any thread).
+// Pool of large device blocks (at least kPoolMinBytes).  The eigensolver's
+// vectors (6.9 GB each at D = 8.6e8, dozens per solve) go back to the pool
+// instead of to hipFree, which unmaps them page by page, and the next
+// allocation of the same size on the device takes them again.  Every library
+// kernel runs on the device's library stream, so a reused block is only
+// touched after the work that used it before.  An allocation that the
+// driver refuses returns the pooled blocks of the device and retries once.
+constexpr size_t kPoolMinBytes = size_t(256) << 20;
+void *device_alloc(size_t bytes, bool pooled); // throws Error(LS_OUT_OF_MEMORY)
+void device_free(void *p, size_t bytes, bool pooled, int device);
+void pool_release_device();                    // hipFree the current device's pooled blocks
+
+// Device buffer with RAII release (any thread).  allocate_pooled(count,
+// bytes) takes a block of max(count * sizeof(T), bytes) bytes from the pool.
 template <class T> struct DeviceBuffer {
   T *ptr = nullptr;
   size_t n = 0;
+  size_t cap = 0;      // bytes of the block
+  bool pooled = false;
+  int dev = 0;         // device of a pooled block
   DeviceBuffer() = default;
   explicit DeviceBuffer(size_t count) { allocate(count); }
   DeviceBuffer(const DeviceBuffer &) = delete;
   DeviceBuffer &operator=(const DeviceBuffer &) = delete;
-  DeviceBuffer(DeviceBuffer &&o) noexcept : ptr(o.ptr), n(o.n) {
+  DeviceBuffer(DeviceBuffer &&o) noexcept
+      : ptr(o.ptr), n(o.n), cap(o.cap), pooled(o.pooled), dev(o.dev) {
     o.ptr = nullptr;
     o.n = 0;
+    o.cap = 0;
   }
   DeviceBuffer &operator=(DeviceBuffer &&o) noexcept {
     if (this != &o) {
       release();
       ptr = o.ptr;
       n = o.n;
+      cap = o.cap;
+      pooled = o.pooled;
+      dev = o.dev;
       o.ptr = nullptr;
       o.n = 0;
+      o.cap = 0;
     }
     return *this;
   }
   ~DeviceBuffer() { release(); }
-  void allocate(size_t count) {
+  void allocate(size_t count) { take(count, 0, false); }
+  void allocate_pooled(size_t count, size_t bytes = 0) { take(count, bytes, true); }
+  void release() {
+    if (ptr) device_free(ptr, cap, pooled, dev);
+    ptr = nullptr;
+    n = 0;
+    cap = 0;
+  }
+  size_t bytes() const { return n * sizeof(T); }
+
+private:
+  void take(size_t count, size_t bytes, bool pool) {
     release();
-    if (count) {
-      hipError_t e = hipMalloc(reinterpret_cast<void **>(&ptr), count * sizeof(T));
-      if (e != hipSuccess) {
-        (void)hipGetLastError();
-        ptr = nullptr;
-        throw Error(LS_OUT_OF_MEMORY, "hipMalloc of " + std::to_string(count * sizeof(T)) +
-                                          " bytes failed: " + hipGetErrorString(e));
-      }
+    const size_t b = std::max(count * sizeof(T), bytes);
+    if (b) {
+      pooled = pool && b >= kPoolMinBytes;
+      if (pooled) dev = current_device();
+      ptr = static_cast<T *>(device_alloc(b, pooled));
+      cap = b;
     }
     n = count;
   }
-  void release() {
-    if (ptr) (void)hipFree(ptr);
-    ptr = nullptr;
-    n = 0;
-  }
-  size_t bytes() const { return n * sizeof(T); }
 };
 
 // Converts exceptions into status codes at the C boundary.

@@ -327,64 +327,79 @@ __host__ __device__ inline RunInfo run_info(uint64_t z, int n, uint64_t mask) {
   return ri;
 }
 
-// Consecutive ones of z at positions p, p-1, ... (down) or p, p+1, ... (up),
-// cyclically; z != mask.
-__host__ __device__ inline int ones_down(uint64_t z, int p, int n, uint64_t mask) {
-  const uint64_t y = ~rotl_n(z, n - 1 - p, n, mask) & mask; // bit p at n - 1, inverted
-  return y ? __builtin_clzll(y << (64 - n)) : n;
-}
-__host__ __device__ inline int ones_up(uint64_t z, int p, int n, uint64_t mask) {
-  const uint64_t y = ~rotr_n(z, p, n, mask) & mask; // bit p at 0, inverted
-  return y ? ctz64(y) : n;
-}
-
 __host__ __device__ inline uint64_t bit_at(int q) { return 1ull << q; }
 
-// Tops and length of the longest runs of ones of z' = z with its bits i and
-// j = (i + 1) mod n exchanged (they differ).  False: the bond's two runs are
-// one run around the ring, or z has no usable run data; take the full scan.
-__host__ __device__ inline bool hop_tops(const RunInfo &ri, uint64_t z, int i, int j, int n,
-                                         uint64_t mask, uint64_t &tops, int &len) {
-  if (!ri.usable) return false;
-  const int im1 = i == 0 ? n - 1 : i - 1;
-  int a, b, topR1, newTopR1, topR2, newTopR2;
-  if ((z >> i) & 1) { // the one at i moves up to j
-    a = ones_down(z, i, n, mask);
-    const int jp1 = j + 1 == n ? 0 : j + 1;
-    b = ones_up(z, jp1, n, mask);
-    topR1 = i;
-    newTopR1 = im1;
-    const int t = j + b;
-    topR2 = t >= n ? t - n : t;
-    newTopR2 = topR2;
-  } else { // the one at j moves down to i
-    a = ones_up(z, j, n, mask);
-    b = ones_down(z, im1, n, mask);
-    const int t = j + a - 1;
-    topR1 = t >= n ? t - n : t;
-    newTopR1 = topR1;
-    topR2 = im1;
-    newTopR2 = i;
-  }
-  if (a + b >= n - 1) return false;
+// Tops and length of the longest runs of ones of a word after a move at the
+// bond (i, j = i + 1 mod n), from the word's run data ri and the two runs
+// next to the bond: run R1 (length a, top topR1) loses a one, which joins
+// run R2 (length b >= 0, top topR2); newTop* are their tops afterwards.
+// Every other run is unchanged.
+__host__ __device__ inline void hop_tops_finish(const RunInfo &ri, int a, int b, int topR1,
+                                                int newTopR1, int topR2, int newTopR2,
+                                                uint64_t &tops, int &len) {
   const int L = ri.L;
   uint64_t el = ri.EL, el1 = ri.EL1;
-  if (a == L) el &= ~bit_at(topR1);
-  if (a == L - 1) el1 &= ~bit_at(topR1);
-  if (b > 0 && b == L) el &= ~bit_at(topR2);
-  if (b > 0 && b == L - 1) el1 &= ~bit_at(topR2);
+  el &= a == L ? ~bit_at(topR1) : ~0ull;
+  el1 &= a == L - 1 ? ~bit_at(topR1) : ~0ull;
+  el &= (b > 0 && b == L) ? ~bit_at(topR2) : ~0ull;
+  el1 &= (b > 0 && b == L - 1) ? ~bit_at(topR2) : ~0ull;
   int lp = b + 1;
-  if (a - 1 > lp) lp = a - 1;
-  if (el1 && L - 1 > lp) lp = L - 1;
-  if (el && L > lp) lp = L;
+  lp = a - 1 > lp ? a - 1 : lp;
+  lp = (el1 && L - 1 > lp) ? L - 1 : lp;
+  lp = (el && L > lp) ? L : lp;
   uint64_t t = 0;
-  if (L == lp) t |= el;
-  if (L - 1 == lp) t |= el1;
-  if (a >= 2 && a - 1 == lp) t |= bit_at(newTopR1);
-  if (b + 1 == lp) t |= bit_at(newTopR2);
+  t |= L == lp ? el : 0;
+  t |= L - 1 == lp ? el1 : 0;
+  t |= (a >= 2 && a - 1 == lp) ? bit_at(newTopR1) : 0;
+  t |= b + 1 == lp ? bit_at(newTopR2) : 0;
   tops = t;
   len = lp;
-  return true;
+}
+
+// Longest-run tops of both words of x = r ^ (bit i | bit j), r_i != r_j:
+// zero runs (t0, l0; run data ri0 of ~r) and one runs (t1, l1; ri1 of r).
+// P, the word of the two that holds the one at i, sees it move up to j;
+// Q = ~P sees its one at j move down to i.  The four run lengths next to
+// the bond come from one rotation w of P (bit j at 0, bit i at n - 1):
+//   aP = ones of P down from i     = leading ones of w,
+//   bP = ones of P up from j + 1   = trailing ones of w >> 1 when aQ == 1,
+//   aQ = zeros of P up from j      = trailing zeros of w,
+//   bQ = zeros of P down from i-1  = leading zeros below bit n - 1 when aP == 1.
+// False: the word needed (both with flip, else the zero-run word) has its
+// two runs next to the bond joined around the ring, or no run data.
+__host__ __device__ inline bool hop_tops_pair(const RunInfo &ri0, const RunInfo &ri1, uint64_t r,
+                                              int i, int n, uint64_t mask, bool flip,
+                                              uint64_t &t0, int &l0, uint64_t &t1, int &l1) {
+  const int j = i + 1 == n ? 0 : i + 1;
+  const int im1 = i == 0 ? n - 1 : i - 1;
+  const bool ri = (r >> i) & 1;
+  const uint64_t P = ri ? r : (~r & mask);
+  const uint64_t w = rotr_n(P, j, n, mask);
+  const int aQ = ctz64(w);
+  const int aP = __builtin_clzll(~(w << (64 - n)));
+  const int bP = aQ == 1 ? ctz64(~(w >> 1)) : 0;
+  const uint64_t yq = w << (65 - n);
+  const int bQ = aP == 1 ? (yq ? __builtin_clzll(yq) : n) : 0;
+  const RunInfo &riP = ri ? ri1 : ri0;
+  const RunInfo &riQ = ri ? ri0 : ri1;
+  const bool okP = riP.usable && aP + bP < n - 1;
+  const bool okQ = riQ.usable && aQ + bQ < n - 1;
+  int tR2 = j + bP;
+  tR2 = tR2 >= n ? tR2 - n : tR2;
+  int tR1 = j + aQ - 1;
+  tR1 = tR1 >= n ? tR1 - n : tR1;
+  uint64_t tP = 0, tQ = 0;
+  int lP = 0, lQ = 0;
+  hop_tops_finish(riP, aP, bP, i, im1, tR2, tR2, tP, lP);
+  hop_tops_finish(riQ, aQ, bQ, tR1, tR1, im1, i, tQ, lQ);
+  t0 = ri ? tQ : tP;
+  l0 = ri ? lQ : lP;
+  t1 = ri ? tP : tQ;
+  l1 = ri ? lP : lQ;
+  const bool ok0 = ri ? okQ : okP;
+  const bool ok1 = ri ? okP : okQ;
+  if (!flip) l1 = -1;
+  return ok0 && (!flip || ok1);
 }
 
 // Orbit of r with the bits at i and j = (i + 1) mod n exchanged (they
@@ -397,8 +412,7 @@ __host__ __device__ inline OrbitResult orbit_ring_hop(const RingOrbitArgs &a, co
   const uint64_t x = r ^ (bit_at(i) | bit_at(j));
   uint64_t t0 = 0, t1 = 0;
   int l0 = 0, l1 = -1;
-  bool ok = hop_tops(ri0, ~r & a.mask, i, j, n, a.mask, t0, l0);
-  if (a.has_flip) ok = ok && hop_tops(ri1, r, i, j, n, a.mask, t1, l1);
+  const bool ok = hop_tops_pair(ri0, ri1, r, i, n, a.mask, a.has_flip, t0, l0, t1, l1);
   if (!ok) return orbit_ring(a, x);
   return orbit_ring_tops(a, x, t0, l0, t1, l1);
 }

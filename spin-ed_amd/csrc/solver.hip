@@ -408,13 +408,15 @@ template <class VT, bool C = false> struct Phase {
   Phase(Operator &o, uint64_t n_, int m_, int k_, int keep_, hipStream_t s_)
       : op(o), n(n_), ne(C ? 2 * n_ : n_), m(m_), k(k_), keep(keep_), s(s_) {
     static_assert(!C || std::is_same<VT, double>::value, "complex vectors are stored in FP64");
+    // FP64-sized blocks for either storage type: the FP32 phase of a mixed
+    // solve hands them on to the FP64 phase through the pool
     vecs.resize(m + 1);
-    for (auto &v : vecs) v.allocate(ne);
+    for (auto &v : vecs) v.allocate_pooled(ne, ne * sizeof(double));
     std::vector<VT *> hp(m + 1);
     for (int i = 0; i <= m; ++i) hp[i] = vecs[i].ptr;
     vptr.allocate(m + 1);
     SPINED_HIP(hipMemcpy(vptr.ptr, hp.data(), sizeof(VT *) * (m + 1), hipMemcpyHostToDevice));
-    w.allocate(ne);
+    w.allocate_pooled(ne);
     const double *wp = w.ptr;
     wptr.allocate(1);
     SPINED_HIP(hipMemcpy(wptr.ptr, &wp, sizeof(wp), hipMemcpyHostToDevice));
@@ -951,7 +953,8 @@ int eigsh(Operator &op, int dtype, uint64_t k, double tol, const spined_eigsh_op
   }
 
   // ---- start vector (FP64, in a scratch buffer) --------------------------
-  DeviceBuffer<double> start(ne);
+  DeviceBuffer<double> start;
+  start.allocate_pooled(ne);
   const auto ts = std::chrono::steady_clock::now();
   std::vector<double> host_start;
   if (opts.start == SPINED_START_VECTOR) {
@@ -976,7 +979,9 @@ int eigsh(Operator &op, int dtype, uint64_t k, double tol, const spined_eigsh_op
   st.seconds_start = seconds_since(ts);
 
   std::vector<double> theta(k), est(k);
-  DeviceBuffer<double> y(ne), hy(ne);
+  DeviceBuffer<double> y, hy;
+  y.allocate_pooled(ne);
+  hy.allocate_pooled(ne);
 
   // Block phase (max_block_size >= 2): FP32 storage in mixed mode, FP64
   // otherwise; the one-vector FP64 phase below then finishes from the sum of

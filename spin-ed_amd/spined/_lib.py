@@ -38,7 +38,7 @@ SPINED_SYMBOLS = (
     "spined_basis_is_complete",
     "spined_basis_sector_dimension", "spined_debug_orbit",
     "spined_debug_ring_available", "spined_debug_orbit_hop", "spined_device_synchronize",
-    "spined_device_malloc",
+    "spined_device_malloc", "spined_device_release_pool",
     "spined_device_free", "spined_memcpy", "spined_memset", "spined_cnn_num_params",
     "spined_cnn_create", "spined_cnn_destroy", "spined_cnn_get_params", "spined_cnn_set_params",
     "spined_cnn_forward_device", "spined_cnn_backward_device", "spined_cnn_adam_step",
@@ -175,6 +175,7 @@ def load():
         "spined_device_synchronize": (I, []),
         "spined_device_malloc": (I, [PP, U64]),
         "spined_device_free": (I, [P]),
+        "spined_device_release_pool": (I, []),
         "spined_memcpy": (I, [P, P, U64, I]),
         "spined_memset": (I, [P, I, U64]),
         "spined_cnn_num_params": (I, []),

@@ -340,6 +340,12 @@ def synchronize() -> None:
     check_status(_lib.load().spined_synchronize())
 
 
+def release_device_pool() -> None:
+    """Return the eigensolver's pooled vector blocks to the driver (they are
+    otherwise kept for the next solve of the same size)."""
+    check_status(_lib.load().spined_device_release_pool())
+
+
 def stream_handle() -> int:
     return int(_lib.load().spined_stream() or 0)
 

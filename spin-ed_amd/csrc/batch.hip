@@ -395,7 +395,7 @@ int batch_eigsh(const std::vector<Operator *> &ops, double tol, int m_req, uint6
     BatchInst &I = hinst[i];
     I.o = OpArgs{};
     I.bd = BondArgs{t.bond_sites.ptr, t.bond_dp.ptr, t.bond_da.ptr, t.bond_o.ptr, t.bond_flip.ptr,
-                    static_cast<int32_t>(t.h_bond_sites.size()), nullptr};
+                    static_cast<int32_t>(t.h_bond_sites.size()), nullptr, 0};
     I.rt = RatioArgs{b.tables.div_idx.ptr, b.tables.ratio.ptr, b.tables.nd};
     I.ga = b.tables.generic_args(true);
     I.ra = b.tables.ring_args(true);

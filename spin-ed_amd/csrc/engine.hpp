@@ -170,6 +170,7 @@ struct OperatorTables {
   std::vector<uint64_t> h_bond_flip;
   // every bond joins ring neighbours (i, i+1 mod n): its lower site i
   bool nn_ring = false;
+  bool nn_identity = false; // bond k joins sites k and k + 1 for every k
   std::vector<uint8_t> h_bond_lo;
   DeviceBuffer<uint8_t> bond_lo;
   DeviceBuffer<uint32_t> bond_sites;

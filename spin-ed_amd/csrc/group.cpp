@@ -441,6 +441,9 @@ void compile_operator(Operator &op) {
       t.nn_ring = false;
   }
   if (!t.nn_ring) t.h_bond_lo.clear();
+  t.nn_identity = t.nn_ring;
+  for (size_t k = 0; k < t.h_bond_lo.size(); ++k)
+    t.nn_identity = t.nn_identity && t.h_bond_lo[k] == k;
 }
 
 void ensure_device(OperatorTables &t) {

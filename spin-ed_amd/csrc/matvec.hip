@@ -35,17 +35,30 @@ __device__ inline uint64_t scheduled_row(const uint16_t *__restrict__ sched, uin
   return sched ? (i & ~static_cast<uint64_t>(kScheduleTile - 1)) + sched[i] : i;
 }
 
+// One wave per workgroup: rows take unequal time, and a one-wave group frees
+// its slot as soon as its own rows are done, which keeps more waves resident
+// than four-wave groups that hold their slots until the slowest wave ends.
+constexpr int kRowBlock = 64;
+
 template <int METHOD, int NW, bool BONDS, class XT, class YT>
-__global__ void __launch_bounds__(kBlock)
+__global__ void __launch_bounds__(kRowBlock)
     matvec_real_kernel(OpArgs o, BondArgs bd, RatioArgs rt, GenericOrbitArgs ga,
                        RingOrbitArgs ra, IndexArgs ix, const uint32_t *__restrict__ stab,
                        uint64_t D, const XT *__restrict__ x, YT *__restrict__ y,
                        const uint16_t *__restrict__ sched) {
+  // bond coefficients in LDS: a per-term global load would have to wait for
+  // every gather issued before it (vector loads complete in order)
+  __shared__ double so[BONDS ? 64 * NW : 1];
+  if constexpr (BONDS) {
+    for (int k = threadIdx.x; k < bd.nbonds; k += kRowBlock) so[k] = bd.o[k];
+    __syncthreads();
+  }
   const uint64_t lb = xcd_block(blockIdx.x, gridDim.x);
-  const uint64_t slot = lb * kBlock + threadIdx.x;
+  const uint64_t slot = lb * kRowBlock + threadIdx.x;
   if (slot >= D) return;
   const uint64_t i = scheduled_row(sched, slot);
-  st_re(y, i, gather_row_real<METHOD, NW, BONDS>(o, bd, rt, ga, ra, ix, stab, i, x));
+  st_re(y, i,
+        gather_row_real<METHOD, NW, BONDS>(o, bd, rt, ga, ra, ix, stab, i, x, BONDS ? so : nullptr));
 }
 
 // Row-interleaved block of K real vectors (x[row * K + c]); per column the
@@ -288,7 +301,8 @@ struct Ctx {
 
 template <int METHOD, int NW, bool BONDS, class XT, class YT>
 void launch_real_nw(const Ctx &c, const XT *x, YT *y, hipStream_t s) {
-  matvec_real_kernel<METHOD, NW, BONDS, XT, YT><<<grid_for(c.D), kBlock, 0, s>>>(
+  const unsigned g = static_cast<unsigned>((c.D + kRowBlock - 1) / kRowBlock);
+  matvec_real_kernel<METHOD, NW, BONDS, XT, YT><<<g, kRowBlock, 0, s>>>(
       c.o, c.bd, c.rt, c.ga, c.ra, c.ix, c.stab, c.D, x, y, c.sched);
 }
 
@@ -391,7 +405,8 @@ Ctx make_ctx(const Operator &op, const double *pc, const double *ps) {
   c.bonds = t.bonds;
   c.bd = BondArgs{t.bond_sites.ptr, t.bond_dp.ptr, t.bond_da.ptr, t.bond_o.ptr, t.bond_flip.ptr,
                   static_cast<int32_t>(t.h_bond_sites.size()),
-                  t.nn_ring && b.tables.ring_available ? t.bond_lo.ptr : nullptr};
+                  t.nn_ring && b.tables.ring_available ? t.bond_lo.ptr : nullptr,
+                  t.nn_identity ? 1 : 0};
   c.rt = RatioArgs{b.tables.div_idx.ptr, b.tables.ratio.ptr, b.tables.nd};
   c.ga = b.tables.generic_args(true);
   c.ra = b.tables.ring_args(true);

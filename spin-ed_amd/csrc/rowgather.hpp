@@ -116,6 +116,9 @@ struct BondArgs {
   // non-null when every bond joins ring neighbours: lower site of each bond
   // (the ring-orbit gather then takes the hop path, orbit_ring_hop)
   const uint8_t *__restrict__ nn_lo;
+  // 1 when bond j joins sites j and j + 1 (nn_lo[j] == j): the lower site
+  // is the bond index and the table is not read
+  int32_t nn_identity;
 };
 
 // sqrt(|Stab r'| / |Stab r|) from a table over the divisors of |G|
@@ -141,7 +144,7 @@ template <int METHOD, bool BONDS> struct RowOrbit {
   __device__ OrbitResult orbit(const OpArgs &o, const BondArgs &bd, const GenericOrbitArgs &ga,
                                const RingOrbitArgs &ra, uint64_t r, int j) const {
     if constexpr (METHOD == 1 && BONDS) {
-      if (bd.nn_lo) return orbit_ring_hop(ra, ri0, ri1, r, bd.nn_lo[j]);
+      if (bd.nn_lo) return orbit_ring_hop(ra, ri0, ri1, r, bd.nn_identity ? j : bd.nn_lo[j]);
     }
     return do_orbit<METHOD>(ga, ra, r ^ (BONDS ? bd.flip[j] : o.flip[j]));
   }
@@ -225,7 +228,9 @@ template <int METHOD, int NW, bool BONDS, class XT>
 __device__ inline double gather_row_real(const OpArgs &o, const BondArgs &bd, const RatioArgs &rt,
                                          const GenericOrbitArgs &ga, const RingOrbitArgs &ra,
                                          const IndexArgs &ix, const uint32_t *__restrict__ stab,
-                                         uint64_t i, const XT *__restrict__ x) {
+                                         uint64_t i, const XT *__restrict__ x,
+                                         const double *__restrict__ bond_o = nullptr) {
+  if (!bond_o) bond_o = bd.o;
   const uint64_t r = ix.reps[i];
   uint64_t act[NW];
   const double diag = row_pass_a<NW, BONDS>(o, bd, r, act);
@@ -236,10 +241,14 @@ __device__ inline double gather_row_real(const OpArgs &o, const BondArgs &bd, co
   const int row_di = rt.nd ? static_cast<int>(rt.div_idx[si]) : 0;
   // two templates per iteration: two orbit scans, then both searches and
   // gathers in flight together; accumulation stays in template order
+  // sqrt(|Stab r'| / |Stab r|) is 1 when the stabilisers have equal size
+  // (almost always; no table read then)
   auto coef_of = [&](int j, const OrbitResult &orb) {
-    const double f = rt.nd ? rt.ratio[rt.div_idx[orb.count] * rt.nd + row_di]
-                           : sqrt(static_cast<double>(orb.count) / sr);
-    double c = (BONDS ? bd.o[j] : o.coef_re[j]) * f;
+    const double f = orb.count == static_cast<int32_t>(si)
+                         ? 1.0
+                         : (rt.nd ? rt.ratio[rt.div_idx[orb.count] * rt.nd + row_di]
+                                  : sqrt(static_cast<double>(orb.count) / sr));
+    double c = (BONDS ? bond_o[j] : o.coef_re[j]) * f;
     return orb.phase != 0 ? -c : c;
   };
   while (any_set<NW>(act)) {
@@ -289,9 +298,13 @@ __device__ inline void gather_row_real_block(const OpArgs &o, const BondArgs &bd
   const uint32_t si = stab[i];
   const double sr = static_cast<double>(si);
   const int row_di = rt.nd ? static_cast<int>(rt.div_idx[si]) : 0;
+  // sqrt(|Stab r'| / |Stab r|) is 1 when the stabilisers have equal size
+  // (almost always; no table read then)
   auto coef_of = [&](int j, const OrbitResult &orb) {
-    const double f = rt.nd ? rt.ratio[rt.div_idx[orb.count] * rt.nd + row_di]
-                           : sqrt(static_cast<double>(orb.count) / sr);
+    const double f = orb.count == static_cast<int32_t>(si)
+                         ? 1.0
+                         : (rt.nd ? rt.ratio[rt.div_idx[orb.count] * rt.nd + row_di]
+                                  : sqrt(static_cast<double>(orb.count) / sr));
     double c = (BONDS ? bd.o[j] : o.coef_re[j]) * f;
     return orb.phase != 0 ? -c : c;
   };

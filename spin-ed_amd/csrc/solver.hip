@@ -86,9 +86,12 @@ __global__ void __launch_bounds__(kBlock)
   }
 }
 
-// h[i] (+)= sum_b partial[i*G + b], fixed order (deterministic).
+// h[i] (+)= sum_b partial[i*G + b], fixed order (deterministic).  Does
+// nothing when skip is given and *skip != 0.
 __global__ void reduce_partials_kernel(const double *__restrict__ partial, int nvec, int G,
-                                       double *__restrict__ h, int accumulate) {
+                                       double *__restrict__ h, int accumulate,
+                                       const int *__restrict__ skip = nullptr) {
+  if (skip && *skip) return;
   const int i = blockIdx.x;
   __shared__ double lds[kBlock];
   double s = 0.0;
@@ -185,6 +188,9 @@ __global__ void __launch_bounds__(kBlock)
 }
 
 constexpr int kFuseMax = 32; // vectors held in registers by the fused passes
+// FP64 phase: the second Gram-Schmidt pass runs only when the first update
+// keeps less than this fraction of |w| (second_pass_test_kernel).
+constexpr double kSecondPassEta = 1e-2;
 
 // Block reduction of acc[q] (q < nq) into partial[q * gridDim.x + blockIdx.x]
 // with the multidot kernel's order (wave shuffles, then the 4 waves in
@@ -212,10 +218,13 @@ __device__ inline void block_partials(const double (&acc)[NQ], int nq, double *_
 // Second Gram-Schmidt pass fused with the first update:
 //   w[r] -= sum_i h1[i] V_i[r];  partial dots of V_i with the new w.
 // Per row the arithmetic equals gemv_sub_kernel followed by multidot_kernel.
+// Does nothing when skip is given and *skip != 0.
 template <class VT>
 __global__ void __launch_bounds__(kBlock)
     sub_dot_kernel(const VT *const *__restrict__ V, int nvec, const double *__restrict__ h1,
-                   double *__restrict__ w, uint64_t n, double *__restrict__ partial) {
+                   double *__restrict__ w, uint64_t n, double *__restrict__ partial,
+                   const int *__restrict__ skip = nullptr) {
+  if (skip && *skip) return;
   // pointers hoisted out of the row loop; slots past nvec read V[0] (a valid
   // address) so every load is unconditional and the loads of a row are all
   // in flight together; their values are masked out of the arithmetic
@@ -241,12 +250,15 @@ __global__ void __launch_bounds__(kBlock)
   block_partials<kFuseMax>(acc, nvec, partial);
 }
 
-// Last update fused with the norm: w[r] -= sum_i h2[i] V_i[r]; partial
-// sums of w[r]^2 (the norm_w multidot's arithmetic).
+// Last update fused with the norm: w[r] -= sum_i c[i] V_i[r]; partial
+// sums of w[r]^2 (the norm_w multidot's arithmetic).  c = h2, or h1 when
+// skip is given and *skip != 0 (the first pass was the only one).
 template <class VT>
 __global__ void __launch_bounds__(kBlock)
     sub_norm_kernel(const VT *const *__restrict__ V, int nvec, const double *__restrict__ h2,
-                    double *__restrict__ w, uint64_t n, double *__restrict__ partial) {
+                    double *__restrict__ w, uint64_t n, double *__restrict__ partial,
+                    const double *__restrict__ h1 = nullptr, const int *__restrict__ skip = nullptr) {
+  if (skip && *skip) h2 = h1;
   const VT *vp[kFuseMax];
 #pragma unroll
   for (int q = 0; q < kFuseMax; ++q) vp[q] = V[q < nvec ? q : 0];
@@ -264,6 +276,56 @@ __global__ void __launch_bounds__(kBlock)
     acc[0] = fma(wr, wr, acc[0]);
   }
   block_partials<1>(acc, 1, partial);
+}
+
+// First Gram-Schmidt pass with the squared norm of w: partial dots of V_i
+// (i < nvec <= kFuseMax) with w in slots 0..nvec-1 and w.w in slot nvec.
+template <class VT>
+__global__ void __launch_bounds__(kBlock)
+    dot_ww_kernel(const VT *const *__restrict__ V, int nvec, const double *__restrict__ w,
+                  uint64_t n, double *__restrict__ partial) {
+  const VT *vp[kFuseMax];
+#pragma unroll
+  for (int q = 0; q < kFuseMax; ++q) vp[q] = V[q < nvec ? q : 0];
+  double acc[kFuseMax + 1];
+#pragma unroll
+  for (int q = 0; q <= kFuseMax; ++q) acc[q] = 0.0;
+  for (uint64_t r = blockIdx.x * static_cast<uint64_t>(kBlock) + threadIdx.x; r < n;
+       r += static_cast<uint64_t>(gridDim.x) * kBlock) {
+    double vals[kFuseMax];
+#pragma unroll
+    for (int q = 0; q < kFuseMax; ++q) vals[q] = to_d(vp[q][r]);
+    const double wr = w[r];
+#pragma unroll
+    for (int q = 0; q < kFuseMax; ++q) acc[q] = q < nvec ? fma(vals[q], wr, acc[q]) : acc[q];
+    acc[kFuseMax] = fma(wr, wr, acc[kFuseMax]);
+  }
+  // w.w moves to slot nvec
+  double out[kFuseMax + 1];
+#pragma unroll
+  for (int q = 0; q <= kFuseMax; ++q) out[q] = q < nvec ? acc[q] : (q == nvec ? acc[kFuseMax] : 0.0);
+  block_partials<kFuseMax + 1>(out, nvec + 1, partial);
+}
+
+// Whether the second Gram-Schmidt pass can be left out: h[0..nvec) are the
+// first-pass coefficients and h[nvec] = w.w.  After the first update
+// |w'|^2 = w.w - |h|^2; when |w'| >= eta |w| the first pass lost at most a
+// factor 1/eta of accuracy to cancellation and its result is kept (the
+// criterion of Daniel, Gragg, Kaufman & Stewart, Math. Comp. 30 (1976) 772,
+// with eta below their 1/sqrt(2)).  Sets *skip and, when set, h2 = 0.
+__global__ void second_pass_test_kernel(const double *__restrict__ h, int nvec, double eta,
+                                        double *__restrict__ h2, int *__restrict__ skip) {
+  __shared__ int flag;
+  if (threadIdx.x == 0) {
+    double s = 0.0;
+    for (int i = 0; i < nvec; ++i) s = fma(h[i], h[i], s);
+    const double ww = h[nvec];
+    flag = (ww > 0.0 && ww - s >= eta * eta * ww) ? 1 : 0;
+    *skip = flag;
+  }
+  __syncthreads();
+  if (flag)
+    for (int i = threadIdx.x; i < nvec; i += blockDim.x) h2[i] = 0.0;
 }
 
 // v[r] = w[r] / (*beta) (beta == 0 -> zero vector)
@@ -398,6 +460,7 @@ template <class VT, bool C = false> struct Phase {
   DeviceBuffer<VT *> vptr;
   DeviceBuffer<const double *> wptr;
   DeviceBuffer<double> w, partial, h, T, theta, S, beta;
+  DeviceBuffer<int> skip; // second Gram-Schmidt pass left out (FP64 phase)
   std::vector<hipEvent_t> ev;
   int64_t matvecs = 0;
   double matvec_seconds = 0.0;
@@ -428,6 +491,7 @@ template <class VT, bool C = false> struct Phase {
     theta.allocate(kMaxBasis);
     S.allocate(kMaxBasis * kMaxBasis);
     beta.allocate(kMaxBasis + 1);
+    skip.allocate(1);
     reset_projected_kernel<<<1, 256, 0, s>>>(T.ptr, kMaxBasis, theta.ptr, 0);
     SPINED_CHECK_LAUNCH();
     small = n <= (1u << 20);
@@ -521,8 +585,32 @@ template <class VT, bool C = false> struct Phase {
 
   // Twice classical Gram-Schmidt against V[0..nvec) and the norm of the
   // result, in three passes over V: dots; update + dots; update + norm.
-  // Same arithmetic as reorth() followed by norm_w().
+  // Same arithmetic as reorth() followed by norm_w().  With FP64 storage the
+  // first pass also takes w.w, and the middle pass is left out (on the
+  // device, no host round trip) when the first update loses little to
+  // cancellation; FP32-stored vectors are orthogonal only to FP32 rounding,
+  // so the FP32 phase always takes both passes.
   void reorth_norm_fused(int nvec, double *h1, double *h2, double *norm_out) {
+    if constexpr (std::is_same<VT, double>::value && !C) {
+      dot_ww_kernel<VT><<<G, kBlock, 0, s>>>(vptr.ptr, nvec, w.ptr, n, partial.ptr);
+      SPINED_CHECK_LAUNCH();
+      reduce_partials_kernel<<<nvec + 1, kBlock, 0, s>>>(partial.ptr, nvec + 1, G, h1, 0);
+      SPINED_CHECK_LAUNCH();
+      second_pass_test_kernel<<<1, 64, 0, s>>>(h1, nvec, kSecondPassEta, h2, skip.ptr);
+      SPINED_CHECK_LAUNCH();
+      sub_dot_kernel<VT><<<G, kBlock, 0, s>>>(vptr.ptr, nvec, h1, w.ptr, n, partial.ptr, skip.ptr);
+      SPINED_CHECK_LAUNCH();
+      reduce_partials_kernel<<<nvec, kBlock, 0, s>>>(partial.ptr, nvec, G, h2, 0, skip.ptr);
+      SPINED_CHECK_LAUNCH();
+      sub_norm_kernel<VT><<<G, kBlock, 0, s>>>(vptr.ptr, nvec, h2, w.ptr, n, partial.ptr, h1,
+                                               skip.ptr);
+      SPINED_CHECK_LAUNCH();
+      reduce_partials_kernel<<<1, kBlock, 0, s>>>(partial.ptr, 1, G, norm_out, 0);
+      SPINED_CHECK_LAUNCH();
+      sqrt_inplace_kernel<<<1, 1, 0, s>>>(norm_out);
+      SPINED_CHECK_LAUNCH();
+      return;
+    }
     multidot<VT>(vptr.ptr, nvec, w.ptr, h1);
     sub_dot_kernel<VT><<<G, kBlock, 0, s>>>(vptr.ptr, nvec, h1, w.ptr, n, partial.ptr);
     SPINED_CHECK_LAUNCH();

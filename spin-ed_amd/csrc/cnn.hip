@@ -46,17 +46,23 @@ constexpr int kInPlaceMinN = 8; // layer 2 in place for n >= 8 (see cnn_forward_
 constexpr float GELU_A = 0.7978845608028654f;
 constexpr float GELU_B = 0.044715f;
 
-// tanh(z) = 1 - 2 / (exp(2z) + 1): one native exponential; saturates to
-// +-1 through exp's overflow / underflow, absolute error ~1e-7 (the network
-// runs on BF16 operands).
-__device__ inline float tanh_fast(float z) { return 1.0f - 2.0f / (__expf(2.0f * z) + 1.0f); }
+// GELU in its tanh form, 0.5 x (1 + tanh(u)) with u = A (x + B x^3), is
+// x s(2u) with the logistic s(y) = 1 / (1 + exp(-y)): one native base-2
+// exponential and one native reciprocal (no division sequence).  s
+// saturates to 0 / 1 through the exponential's overflow / underflow; the
+// error is a few FP32 ulp (the network's operands are BF16).
+constexpr float LOG2E = 1.4426950408889634f;
+__device__ inline float logistic2(float u) {
+  return __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(-2.0f * LOG2E * u));
+}
 
 __device__ inline float gelu_f(float x) {
-  return 0.5f * x * (1.0f + tanh_fast(GELU_A * (x + GELU_B * x * x * x)));
+  return x * logistic2(GELU_A * (x + GELU_B * x * x * x));
 }
+// d/dx [x s(2u)] = s + 2 x s (1 - s) A (1 + 3 B x^2)
 __device__ inline float gelu_grad_f(float x) {
-  const float t = tanh_fast(GELU_A * (x + GELU_B * x * x * x));
-  return 0.5f * (1.0f + t) + 0.5f * x * (1.0f - t * t) * GELU_A * (1.0f + 3.0f * GELU_B * x * x);
+  const float sg = logistic2(GELU_A * (x + GELU_B * x * x * x));
+  return sg + 2.0f * x * sg * (1.0f - sg) * GELU_A * (1.0f + 3.0f * GELU_B * x * x);
 }
 
 __device__ inline floatx4 mfma(bf16x8 a, bf16x8 b, floatx4 c) {

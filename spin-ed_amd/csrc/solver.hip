@@ -188,8 +188,8 @@ __global__ void __launch_bounds__(kBlock)
 }
 
 constexpr int kFuseMax = 32; // vectors held in registers by the fused passes
-// FP64 phase: the second Gram-Schmidt pass runs only when the first update
-// keeps less than this fraction of |w| (second_pass_test_kernel).
+// The second Gram-Schmidt pass runs only when the first update keeps less
+// than this fraction of |w| (second_pass_test_kernel).
 constexpr double kSecondPassEta = 1e-2;
 
 // Block reduction of acc[q] (q < nq) into partial[q * gridDim.x + blockIdx.x]
@@ -583,40 +583,25 @@ template <class VT, bool C = false> struct Phase {
     SPINED_CHECK_LAUNCH();
   }
 
-  // Twice classical Gram-Schmidt against V[0..nvec) and the norm of the
-  // result, in three passes over V: dots; update + dots; update + norm.
-  // Same arithmetic as reorth() followed by norm_w().  With FP64 storage the
-  // first pass also takes w.w, and the middle pass is left out (on the
-  // device, no host round trip) when the first update loses little to
-  // cancellation; FP32-stored vectors are orthogonal only to FP32 rounding,
-  // so the FP32 phase always takes both passes.
+  // Classical Gram-Schmidt against V[0..nvec) (real vectors, nvec <=
+  // kFuseMax) and the norm of the result: dots with w.w; update + dots
+  // (the second pass); update + norm.  The second pass is left out on the
+  // device (no host round trip) when the first update loses little to
+  // cancellation (second_pass_test_kernel).  Complex vectors take reorth()
+  // and norm_w().
   void reorth_norm_fused(int nvec, double *h1, double *h2, double *norm_out) {
-    if constexpr (std::is_same<VT, double>::value && !C) {
-      dot_ww_kernel<VT><<<G, kBlock, 0, s>>>(vptr.ptr, nvec, w.ptr, n, partial.ptr);
-      SPINED_CHECK_LAUNCH();
-      reduce_partials_kernel<<<nvec + 1, kBlock, 0, s>>>(partial.ptr, nvec + 1, G, h1, 0);
-      SPINED_CHECK_LAUNCH();
-      second_pass_test_kernel<<<1, 64, 0, s>>>(h1, nvec, kSecondPassEta, h2, skip.ptr);
-      SPINED_CHECK_LAUNCH();
-      sub_dot_kernel<VT><<<G, kBlock, 0, s>>>(vptr.ptr, nvec, h1, w.ptr, n, partial.ptr, skip.ptr);
-      SPINED_CHECK_LAUNCH();
-      reduce_partials_kernel<<<nvec, kBlock, 0, s>>>(partial.ptr, nvec, G, h2, 0, skip.ptr);
-      SPINED_CHECK_LAUNCH();
-      sub_norm_kernel<VT><<<G, kBlock, 0, s>>>(vptr.ptr, nvec, h2, w.ptr, n, partial.ptr, h1,
-                                               skip.ptr);
-      SPINED_CHECK_LAUNCH();
-      reduce_partials_kernel<<<1, kBlock, 0, s>>>(partial.ptr, 1, G, norm_out, 0);
-      SPINED_CHECK_LAUNCH();
-      sqrt_inplace_kernel<<<1, 1, 0, s>>>(norm_out);
-      SPINED_CHECK_LAUNCH();
-      return;
-    }
-    multidot<VT>(vptr.ptr, nvec, w.ptr, h1);
-    sub_dot_kernel<VT><<<G, kBlock, 0, s>>>(vptr.ptr, nvec, h1, w.ptr, n, partial.ptr);
+    dot_ww_kernel<VT><<<G, kBlock, 0, s>>>(vptr.ptr, nvec, w.ptr, n, partial.ptr);
     SPINED_CHECK_LAUNCH();
-    reduce_partials_kernel<<<nvec, kBlock, 0, s>>>(partial.ptr, nvec, G, h2, 0);
+    reduce_partials_kernel<<<nvec + 1, kBlock, 0, s>>>(partial.ptr, nvec + 1, G, h1, 0);
     SPINED_CHECK_LAUNCH();
-    sub_norm_kernel<VT><<<G, kBlock, 0, s>>>(vptr.ptr, nvec, h2, w.ptr, n, partial.ptr);
+    second_pass_test_kernel<<<1, 64, 0, s>>>(h1, nvec, kSecondPassEta, h2, skip.ptr);
+    SPINED_CHECK_LAUNCH();
+    sub_dot_kernel<VT><<<G, kBlock, 0, s>>>(vptr.ptr, nvec, h1, w.ptr, n, partial.ptr, skip.ptr);
+    SPINED_CHECK_LAUNCH();
+    reduce_partials_kernel<<<nvec, kBlock, 0, s>>>(partial.ptr, nvec, G, h2, 0, skip.ptr);
+    SPINED_CHECK_LAUNCH();
+    sub_norm_kernel<VT><<<G, kBlock, 0, s>>>(vptr.ptr, nvec, h2, w.ptr, n, partial.ptr, h1,
+                                             skip.ptr);
     SPINED_CHECK_LAUNCH();
     reduce_partials_kernel<<<1, kBlock, 0, s>>>(partial.ptr, 1, G, norm_out, 0);
     SPINED_CHECK_LAUNCH();
@@ -630,7 +615,7 @@ template <class VT, bool C = false> struct Phase {
            w.ptr);
     double *h1 = h.ptr, *h2 = h.ptr + 2 * kMaxBasis + 2;
     if (!C && j + 1 <= kFuseMax) {
-      reorth_norm_fused(j + 1, h1, h2, beta.ptr + j);
+      if constexpr (!C) reorth_norm_fused(j + 1, h1, h2, beta.ptr + j);
     } else {
       reorth(j + 1, h1, h2);
       norm_w(beta.ptr + j);

@@ -54,7 +54,7 @@ def parse():
                    help="headline storage mode (mixed = FP32-storage phase first)")
     p.add_argument("--start", default="learned", choices=["random", "learned"])
     p.add_argument("--train-sizes", default="16,18,20,22,24,26,28,30,32")
-    p.add_argument("--train-steps", type=int, default=800)
+    p.add_argument("--train-steps", type=int, default=2400)
     p.add_argument("--no-compare", action="store_true",
                    help="skip the plain FP64 random-start reference solve")
     p.add_argument("--tol", type=float, default=1e-10)

@@ -193,7 +193,30 @@ __global__ void __launch_bounds__(64)
   }
   float bias[4];
   // layer 1
-  {
+  if (!z1) {
+    // inference: layer 1 at a position sees only the five spins around it,
+    // so its 32 possible outputs (x 16 channels) are evaluated once per
+    // workgroup (BF16 weights, FP32 sums, as the MFMA path) and looked up
+    __bf16 *tab = lds + (in_place ? 1 : 2) * 16 * n * CH; // [32][16]
+    for (int e = lane; e < 32 * CH; e += 64) {
+      const int pat = e / CH, ch = e % CH;
+      float v = 0.0f;
+      for (int k = 0; k < KW; ++k) {
+        const float wk = static_cast<float>(static_cast<__bf16>(P[OFF_W1 + ch * KW + k]));
+        v += ((pat >> k) & 1) ? wk : -wk;
+      }
+      tab[e] = static_cast<__bf16>(gelu_f(v + P[OFF_B1 + ch]));
+    }
+    __syncthreads();
+    for (int pos = 0; pos < n; ++pos) {
+      // bit k of the pattern: spin at pos + k - 2 (cyclic)
+      const int start = pos >= 2 ? pos - 2 : pos - 2 + n;
+      const uint64_t rot = start == 0 ? state : ((state >> start) | (state << (n - start)));
+      const int pat = static_cast<int>(rot & 31u);
+      *reinterpret_cast<bf16x4 *>(h1 + (static_cast<size_t>(s) * n + pos) * CH + 4 * g) =
+          *reinterpret_cast<const bf16x4 *>(tab + pat * CH + 4 * g);
+    }
+  } else {
     const bf16x8 a = load_first_weights(P + OFF_W1);
 #pragma unroll
     for (int r = 0; r < 4; ++r) bias[r] = P[OFF_B1 + 4 * g + r];
@@ -567,8 +590,9 @@ __global__ void mfma_probe_kernel(float *out) {
   for (int r = 0; r < 4; ++r) out[(4 * g + r) * 16 + row] = c[r]; // D[4g+r][col]
 }
 
+// activations (one or two 16 x n x 16 BF16 buffers) + the layer-1 table
 size_t lds_bytes_forward(int n) {
-  return static_cast<size_t>(n >= kInPlaceMinN ? 1 : 2) * 16 * n * CH * 2;
+  return static_cast<size_t>(n >= kInPlaceMinN ? 1 : 2) * 16 * n * CH * 2 + 32 * CH * 2;
 }
 size_t lds_bytes_backward(int n) { return static_cast<size_t>(3) * 16 * n * CH * 2; }
 

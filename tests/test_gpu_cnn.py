@@ -47,6 +47,12 @@ def test_forward_matches_oracle(n):
     assert np.max(np.abs(a - ra)) <= 2e-2 * scale
     assert np.max(np.abs(t - rt)) <= 2e-2 * (np.max(np.abs(rt)) + 1.0)
     assert np.corrcoef(a, ra)[0, 1] > 0.999
+    # the inference path (layer 1 from a table of its 32 input patterns) and
+    # the training path (layer 1 on MFMA, pre-activations stored) agree to
+    # BF16 rounding
+    a2, t2, _, _ = model.forward(n, states, store=True)
+    assert np.max(np.abs(a - a2.numpy())) <= 5e-3 * scale
+    assert np.max(np.abs(t - t2.numpy())) <= 5e-3 * (np.max(np.abs(rt)) + 1.0)
 
 
 @pytest.mark.parametrize("n", [16, 26])

@@ -71,6 +71,8 @@ def parse():
                    help="matvecs-only mode: row-interleaved block width")
     p.add_argument("--index", default="auto", choices=("auto", "bucket", "hash"),
                    help="index of representatives (spined_basis_set_index_kind)")
+    p.add_argument("--gather-cache", choices=["auto", "on", "off"], default="auto",
+                   help="gather cache of the operator (auto: built when it fits in free memory)")
     p.add_argument("--no-autotune", action="store_true",
                    help="skip the matvec autotuner (run before the timed region)")
     p.add_argument("--bucket-bits", type=int, default=-1,
@@ -111,6 +113,7 @@ def main():
     op = spined.mk_operator(basis, [spined.mk_interaction(lattice.HEISENBERG,
                                                           lattice.ring_bonds(n))])
     spined.set_index_kind(basis, {"auto": -1, "bucket": 0, "hash": 1}[a.index])
+    spined.set_gather_cache(op, {"auto": -1, "on": 1, "off": 0}[a.gather_cache])
     spined.set_bucket_bits(basis, a.bucket_bits)
     spined.build_basis(basis)
     device_synchronize()
@@ -174,6 +177,7 @@ def main():
         dt = (time.perf_counter() - t) / a.matvecs_only
         print(json.dumps({"mode": "matvecs-only", "config": desc, "dimension": D, "index": a.index,
                           "bucket_bits": spined.bucket_bits(basis),
+                          "gather_cache": spined.gather_cache(op),
                           "mv_precision": a.mv_precision, "matvecs": a.matvecs_only,
                           "ms_per_matvec": dt * 1e3, "offdiag_gathers_per_launch": e_off}))
         return 0

@@ -147,6 +147,17 @@ int spined_operator_set_orbit_method(ls_operator *op, int method);
  * 1 on; built at the next matvec.  get: 1 if a schedule is in use. */
 int spined_operator_set_schedule(ls_operator *op, int mode);
 int spined_operator_get_schedule(const ls_operator *op);
+/* Gather cache of a real operator: the target of every term of every row
+ * (4 bytes per term: 71 GB at D = 8.6e8), written once by the orbit and
+ * index kernels so that later real single-vector matvecs read it instead of
+ * recomputing orbits and searching the index; the sums are identical.
+ * mode: -1 automatic (built at a real matvec of a basis of at least 2^20
+ * rows when free device memory exceeds it by 4 GiB, and given back whenever
+ * a device allocation of the library needs the memory), 0 off, 1 on (built
+ * at the next real matvec; an error if it does not fit).  get: 1 while a
+ * cache is built.  Changing the row schedule drops the cache. */
+int spined_operator_set_gather_cache(ls_operator *op, int mode);
+int spined_operator_get_gather_cache(const ls_operator *op);
 /* Bucket bits of the bucket index (about 2^bits level-2 buckets of 4 bytes,
  * 1..36; -1 = automatic: ceil(log2 D) + 3).  The build lowers it until the
  * table takes at most half the free device memory.  Setting it on a built

@@ -84,6 +84,7 @@ void *device_alloc(size_t bytes, bool pooled) {
   if (e != hipSuccess) {
     (void)hipGetLastError();
     release_blocks(d);
+    reclaim_gather_caches(d);
     e = hipMalloc(&p, bytes);
   }
   if (e != hipSuccess) {
@@ -105,6 +106,15 @@ void device_free(void *p, size_t bytes, bool pooled, int device) {
 }
 
 void pool_release_device() { release_blocks(current_device()); }
+
+size_t pool_bytes(size_t bytes) {
+  const int d = current_device();
+  std::lock_guard<std::mutex> lk(g_pool_mu);
+  size_t s = 0;
+  for (const PoolBlock &b : g_pool)
+    if (b.device == d && b.bytes == bytes) s += b.bytes;
+  return s;
+}
 
 hipStream_t stream() {
   static std::mutex mu;
@@ -734,9 +744,26 @@ int spined_operator_autotune(ls_operator *op, int32_t repeats, spined_autotune_r
         o.schedule_request = st.sched;
         o.schedule_built = false;
         o.sched.release();
+        drop_gather_cache(o);
       }
       ensure_schedule(o);
     };
+    // the tuner times the orbit-and-index kernels: the gather cache is off
+    // while it runs, and its request is restored afterwards
+    struct CacheOff {
+      GatherCache &gc;
+      int saved;
+      explicit CacheOff(GatherCache &g) : gc(g) {
+        std::lock_guard<std::mutex> lk(gc.mu);
+        saved = gc.request;
+        gc.request = 0;
+        gc.drop();
+      }
+      ~CacheOff() {
+        std::lock_guard<std::mutex> lk(gc.mu);
+        gc.request = saved;
+      }
+    } cache_off(*o.cache);
     report->ms_per_matvec_first = time_matvec(o, 1, repeats, x.ptr, y.ptr);
     // bucket searches that need no confirming load beat the hash table
     const bool exact = b.complete && (b.w < 0 || o.tables.keeps_weight);
@@ -814,12 +841,30 @@ int spined_operator_set_schedule(ls_operator *op, int mode) {
       o.schedule_request = mode;
       o.schedule_built = false;
       o.sched.release();
+      drop_gather_cache(o);
     }
   });
 }
 
 int spined_operator_get_schedule(const ls_operator *op) {
   return op ? (op->p->schedule_built ? 1 : 0) : -1;
+}
+
+int spined_operator_set_gather_cache(ls_operator *op, int mode) {
+  return guarded([&] {
+    if (!op || mode < -1 || mode > 1) throw Error(LS_INVALID_ARGUMENT, "bad argument");
+    GatherCache &gc = *op->p->cache;
+    std::lock_guard<std::mutex> lk(gc.mu);
+    gc.request = mode;
+    if (mode == 0) gc.drop();
+  });
+}
+
+int spined_operator_get_gather_cache(const ls_operator *op) {
+  if (!op) return -1;
+  GatherCache &gc = *op->p->cache;
+  std::lock_guard<std::mutex> lk(gc.mu);
+  return gc.built ? 1 : 0;
 }
 
 int spined_basis_set_bucket_bits(ls_spin_basis *basis, int bits) {

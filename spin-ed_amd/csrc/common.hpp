@@ -52,6 +52,7 @@ constexpr size_t kPoolMinBytes = size_t(256) << 20;
 void *device_alloc(size_t bytes, bool pooled); // throws Error(LS_OUT_OF_MEMORY)
 void device_free(void *p, size_t bytes, bool pooled, int device);
 void pool_release_device();                    // hipFree the current device's pooled blocks
+size_t pool_bytes(size_t bytes);               // pooled bytes in blocks of this size (current device)
 
 // Device buffer with RAII release (any thread).  allocate_pooled(count,
 // bytes) takes a block of max(count * sizeof(T), bytes) bytes from the pool.

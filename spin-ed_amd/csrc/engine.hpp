@@ -181,6 +181,30 @@ struct OperatorTables {
 
 void ensure_device(OperatorTables &t);
 
+// Gather cache of a real operator (matvec.hip): the target of every term of
+// every row, written once by the orbit and index kernels, so that later real
+// matvecs read it instead of recomputing orbits and searching the index.
+// Rows in schedule order, one wave per 64 slots; term t of lane l of wave w
+// at entries[woff[w] + 64 t + l] (coalesced).  An entry holds the target
+// index (bits 0..29, all ones: no term), a negative coefficient (bit 30) and
+// a stabiliser ratio other than 1 (bit 31, taken from stab[target]).
+struct GatherCache {
+  std::mutex mu;
+  int request = -1;       // -1 automatic (built when it fits in free memory), 0 off, 1 on
+  bool counted = false;   // woff and total known
+  bool built = false;
+  uint64_t total = 0;     // entries
+  DeviceBuffer<uint64_t> woff;
+  DeviceBuffer<uint32_t> entries;
+  void drop() {           // caller holds mu
+    built = false;
+    counted = false;
+    total = 0;
+    entries.release();
+    woff.release();
+  }
+};
+
 struct Operator {
   std::shared_ptr<Basis> basis;
   std::vector<std::shared_ptr<const Interaction>> terms;
@@ -196,10 +220,22 @@ struct Operator {
   int schedule_request = -1; // -1 automatic, 0 off, 1 on
   bool schedule_built = false;
   DeviceBuffer<uint16_t> sched;
+  std::unique_ptr<GatherCache> cache = std::make_unique<GatherCache>();
+  Operator();
+  ~Operator();
 };
 
 constexpr int kScheduleTile = 4096;
 void ensure_schedule(Operator &op); // matvec.hip
+// Drops the operator's gather cache (the schedule or the basis changed).
+void drop_gather_cache(Operator &op);  // matvec.hip
+// Frees the gather caches on a device (a device allocation failed).
+void reclaim_gather_caches(int device); // matvec.hip
+// Builds the operator's gather cache (unless it is off) when it fits with
+// reserve bytes of device memory (free or pooled in blocks of block_bytes)
+// to spare; true when a cache is ready.
+bool prepare_gather_cache(Operator &op, uint64_t reserve, uint64_t block_bytes); // matvec.hip
+void drop_gather_cache_if_built(Operator &op);                                   // matvec.hip
 
 void compile_operator(Operator &op);
 

@@ -21,6 +21,7 @@
 
 #include "engine.hpp"
 #include "rowgather.hpp"
+#include "scan.hpp"
 
 namespace spined {
 
@@ -59,6 +60,107 @@ __global__ void __launch_bounds__(kRowBlock)
   const uint64_t i = scheduled_row(sched, slot);
   st_re(y, i,
         gather_row_real<METHOD, NW, BONDS>(o, bd, rt, ga, ra, ix, stab, i, x, BONDS ? so : nullptr));
+}
+
+// ---- gather cache -----------------------------------------------------------
+constexpr uint32_t kCacheIndex = 0x3FFFFFFFu; // index bits; all ones: no term
+constexpr uint32_t kCacheNeg = 1u << 30;      // negative coefficient
+constexpr uint32_t kCacheRatio = 1u << 31;    // sqrt(|Stab target| / |Stab row|) != 1
+
+// Terms of the busiest row of each wave (one wave = 64 slots).
+template <int NW, bool BONDS>
+__global__ void __launch_bounds__(kRowBlock)
+    cache_count_kernel(OpArgs o, BondArgs bd, const uint64_t *__restrict__ reps, uint64_t D,
+                       const uint16_t *__restrict__ sched, uint32_t *__restrict__ wave_terms) {
+  const uint64_t slot = static_cast<uint64_t>(blockIdx.x) * kRowBlock + threadIdx.x;
+  uint32_t c = 0;
+  if (slot < D) {
+    uint64_t act[NW];
+    (void)row_pass_a<NW, BONDS>(o, bd, reps[scheduled_row(sched, slot)], act);
+#pragma unroll
+    for (int w = 0; w < NW; ++w) c += static_cast<uint32_t>(__popcll(act[w]));
+  }
+  for (int off = 32; off > 0; off >>= 1) c = max(c, static_cast<uint32_t>(__shfl_xor(c, off, 64)));
+  if (threadIdx.x == 0) wave_terms[blockIdx.x] = c;
+}
+
+// Entries of every term, in the order and with the orbits and lookups of
+// gather_row_real.
+template <int METHOD, int NW, bool BONDS>
+__global__ void __launch_bounds__(kRowBlock)
+    cache_fill_kernel(OpArgs o, BondArgs bd, GenericOrbitArgs ga, RingOrbitArgs ra, IndexArgs ix,
+                      const uint32_t *__restrict__ stab, uint64_t D,
+                      const uint16_t *__restrict__ sched, const uint64_t *__restrict__ woff,
+                      uint32_t *__restrict__ entries) {
+  const uint64_t slot = static_cast<uint64_t>(blockIdx.x) * kRowBlock + threadIdx.x;
+  if (slot >= D) return;
+  const uint64_t i = scheduled_row(sched, slot);
+  const uint64_t r = ix.reps[i];
+  uint64_t act[NW];
+  (void)row_pass_a<NW, BONDS>(o, bd, r, act);
+  const RowOrbit<METHOD, BONDS> rob(ra, bd, r);
+  const int32_t si = static_cast<int32_t>(stab[i]);
+  uint32_t *out = entries + woff[blockIdx.x] * 64 + threadIdx.x;
+  auto entry = [&](int64_t k, const OrbitResult &orb) {
+    if (k < 0) return kCacheIndex;
+    return static_cast<uint32_t>(k) | (orb.phase != 0 ? kCacheNeg : 0u) |
+           (orb.count != si ? kCacheRatio : 0u);
+  };
+  int t = 0;
+  while (any_set<NW>(act)) {
+    const int ja = pop_first<NW>(act);
+    const int jb = pop_first<NW>(act);
+    const OrbitResult oa = rob.orbit(o, bd, ga, ra, r, ja);
+    OrbitResult ob{0, 0, 0, false};
+    if (jb >= 0) ob = rob.orbit(o, bd, ga, ra, r, jb);
+    int64_t ka, kb;
+    find_index2(ix, oa.rep, oa.valid, ob.rep, ob.valid, ka, kb);
+    out[64 * static_cast<uint64_t>(t++)] = entry(ka, oa);
+    if (jb >= 0) out[64 * static_cast<uint64_t>(t++)] = entry(kb, ob);
+  }
+}
+
+// Real matvec from the gather cache: the sums of matvec_real_kernel (same
+// terms, coefficients and order) without orbits or index searches.
+template <int NW, bool BONDS, class XT, class YT>
+__global__ void __launch_bounds__(kRowBlock)
+    matvec_cached_kernel(OpArgs o, BondArgs bd, RatioArgs rt, const uint64_t *__restrict__ reps,
+                         const uint32_t *__restrict__ stab, uint64_t D,
+                         const XT *__restrict__ x, YT *__restrict__ y,
+                         const uint16_t *__restrict__ sched, const uint64_t *__restrict__ woff,
+                         const uint32_t *__restrict__ entries) {
+  __shared__ double so[BONDS ? 64 * NW : 1];
+  if constexpr (BONDS) {
+    for (int k = threadIdx.x; k < bd.nbonds; k += kRowBlock) so[k] = bd.o[k];
+    __syncthreads();
+  }
+  const uint64_t lb = xcd_block(blockIdx.x, gridDim.x);
+  const uint64_t slot = lb * kRowBlock + threadIdx.x;
+  if (slot >= D) return;
+  const uint64_t i = scheduled_row(sched, slot);
+  uint64_t act[NW];
+  const double diag = row_pass_a<NW, BONDS>(o, bd, reps[i], act);
+  double acc = diag * ld_re(x, i);
+  const uint32_t si = stab[i];
+  const double sr = static_cast<double>(si);
+  const uint32_t *in = entries + woff[lb] * 64 + threadIdx.x;
+  int t = 0;
+  while (any_set<NW>(act)) {
+    const int j = pop_first<NW>(act);
+    const uint32_t e = in[64 * static_cast<uint64_t>(t++)];
+    const uint32_t k = e & kCacheIndex;
+    if (k == kCacheIndex) continue;
+    double f = 1.0;
+    if (e & kCacheRatio) {
+      const uint32_t ck = stab[k];
+      f = rt.nd ? rt.ratio[rt.div_idx[ck] * rt.nd + rt.div_idx[si]]
+                : sqrt(static_cast<double>(ck) / sr);
+    }
+    double c = (BONDS ? so[j] : o.coef_re[j]) * f;
+    c = (e & kCacheNeg) ? -c : c;
+    acc = fma(c, ld_re(x, k), acc);
+  }
+  st_re(y, i, acc);
 }
 
 // Row-interleaved block of K real vectors (x[row * K + c]); per column the
@@ -349,6 +451,109 @@ template <int METHOD> void dispatch(const MatvecCall &m, const Ctx &c) {
     throw Error(LS_INVALID_DATATYPE, "unsupported matvec type combination");
 }
 
+// Calls f(integral_constant<NW>, bool_constant<BONDS>) for the row-gather
+// kernels' word count; false when the operator has too many templates.
+template <class F> bool with_nw(const Ctx &c, F &&f) {
+  const int units = c.bonds ? c.bd.nbonds : c.o.ntemplates;
+  if (c.bonds && units <= 64)
+    f(std::integral_constant<int, 1>{}, std::true_type{});
+  else if (c.bonds && units <= 128)
+    f(std::integral_constant<int, 2>{}, std::true_type{});
+  else if (!c.bonds && units <= 64)
+    f(std::integral_constant<int, 1>{}, std::false_type{});
+  else if (!c.bonds && units <= 128)
+    f(std::integral_constant<int, 2>{}, std::false_type{});
+  else if (!c.bonds && units <= 256)
+    f(std::integral_constant<int, 4>{}, std::false_type{});
+  else
+    return false;
+  return true;
+}
+
+// Free memory that an automatically built gather cache leaves untouched,
+// and the smallest basis worth one (the build costs about one matvec).
+constexpr uint64_t kCacheReserve = uint64_t(4) << 30;
+constexpr uint64_t kCacheMinRows = uint64_t(1) << 20;
+
+// Builds the gather cache when it is requested (1), or when it is automatic
+// (-1) and fits in free memory with kCacheReserve to spare.  Caller holds
+// gc.mu.  True when the cache is ready.
+template <int METHOD>
+bool ensure_cache(const Operator &op, const Ctx &c, GatherCache &gc, hipStream_t s,
+                  uint64_t reserve = kCacheReserve, uint64_t reusable = 0) {
+  if (gc.built) return true;
+  const Basis &b = *op.basis;
+  if (gc.request == 0 || !op.tables.is_real || b.D >= kCacheIndex) return false;
+  if (gc.request < 0 && b.D < kCacheMinRows) return false;
+  const uint64_t waves = (b.D + kRowBlock - 1) / kRowBlock;
+  const unsigned g = static_cast<unsigned>(waves);
+  if (!gc.counted) {
+    DeviceBuffer<uint32_t> wt(waves);
+    DeviceBuffer<uint64_t> tot(1);
+    if (!with_nw(c, [&](auto nw, auto bonds) {
+          cache_count_kernel<decltype(nw)::value, decltype(bonds)::value>
+              <<<g, kRowBlock, 0, s>>>(c.o, c.bd, b.reps.ptr, b.D, c.sched, wt.ptr);
+        }))
+      return false;
+    SPINED_CHECK_LAUNCH();
+    gc.woff.allocate(waves);
+    scan_counts_kernel<<<1, 1024, 0, s>>>(wt.ptr, waves, gc.woff.ptr, tot.ptr);
+    SPINED_CHECK_LAUNCH();
+    uint64_t t = 0;
+    SPINED_HIP(hipMemcpyAsync(&t, tot.ptr, 8, hipMemcpyDeviceToHost, s));
+    SPINED_HIP(hipStreamSynchronize(s));
+    gc.total = t * kRowBlock;
+    gc.counted = true;
+  }
+  if (gc.request < 0) {
+    size_t free_b = 0, total_b = 0;
+    SPINED_HIP(hipMemGetInfo(&free_b, &total_b));
+    if (free_b < gc.total * 4 + kCacheReserve || free_b + reusable < gc.total * 4 + reserve)
+      return false;
+  }
+  try {
+    gc.entries.allocate(std::max<uint64_t>(gc.total, 1));
+  } catch (const Error &) {
+    if (gc.request > 0) throw;
+    return false;
+  }
+  with_nw(c, [&](auto nw, auto bonds) {
+    cache_fill_kernel<METHOD, decltype(nw)::value, decltype(bonds)::value><<<g, kRowBlock, 0, s>>>(
+        c.o, c.bd, c.ga, c.ra, c.ix, c.stab, c.D, c.sched, gc.woff.ptr, gc.entries.ptr);
+  });
+  SPINED_CHECK_LAUNCH();
+  gc.built = true;
+  return true;
+}
+
+template <class XT, class YT>
+void launch_cached(const Ctx &c, const GatherCache &gc, const XT *x, YT *y, hipStream_t s) {
+  const unsigned g = static_cast<unsigned>((c.D + kRowBlock - 1) / kRowBlock);
+  with_nw(c, [&](auto nw, auto bonds) {
+    matvec_cached_kernel<decltype(nw)::value, decltype(bonds)::value, XT, YT>
+        <<<g, kRowBlock, 0, s>>>(c.o, c.bd, c.rt, c.ix.reps, c.stab, c.D, x, y, c.sched,
+                                 gc.woff.ptr, gc.entries.ptr);
+  });
+  SPINED_CHECK_LAUNCH();
+}
+
+// Real matvec from the gather cache when there is (or can be) one.
+template <int METHOD> bool try_cached(const MatvecCall &m, const Ctx &c) {
+  GatherCache &gc = *m.op->cache;
+  std::lock_guard<std::mutex> lk(gc.mu);
+  if (!ensure_cache<METHOD>(*m.op, c, gc, m.stream)) return false;
+  using VT = VecType;
+  if (m.xt == VT::F64 && m.yt == VT::F64)
+    launch_cached(c, gc, static_cast<const double *>(m.x), static_cast<double *>(m.y), m.stream);
+  else if (m.xt == VT::F32 && m.yt == VT::F32)
+    launch_cached(c, gc, static_cast<const float *>(m.x), static_cast<float *>(m.y), m.stream);
+  else if (m.xt == VT::F32 && m.yt == VT::F64)
+    launch_cached(c, gc, static_cast<const float *>(m.x), static_cast<double *>(m.y), m.stream);
+  else
+    return false;
+  return true;
+}
+
 template <int METHOD, int NW, bool BONDS, int K, class XT, class YT>
 void launch_block_nw(const Ctx &c, const XT *x, YT *y, hipStream_t s) {
   matvec_block_kernel<METHOD, NW, BONDS, K, XT, YT><<<grid_for(c.D), kBlock, 0, s>>>(
@@ -457,10 +662,67 @@ void launch_matvec(const MatvecCall &m) {
     ps = pt.sin_t.ptr;
   }
   const Ctx c = make_ctx(op, pc, ps);
+  if (!cplx) {
+    const bool done = op.orbit_method() == 1 ? try_cached<1>(m, c) : try_cached<0>(m, c);
+    if (done) return;
+  }
   if (op.orbit_method() == 1)
     dispatch<1>(m, c);
   else
     dispatch<0>(m, c);
+}
+
+namespace {
+std::mutex g_ops_mu;
+std::vector<Operator *> g_ops; // live operators (their gather caches can be reclaimed)
+} // namespace
+
+bool prepare_gather_cache(Operator &op, uint64_t reserve, uint64_t block_bytes) {
+  if (!op.tables.is_real || op.basis->D == 0) return false;
+  const Ctx c = make_ctx(op, nullptr, nullptr);
+  GatherCache &gc = *op.cache;
+  std::lock_guard<std::mutex> lk(gc.mu);
+  const uint64_t reusable = pool_bytes(block_bytes);
+  return op.orbit_method() == 1 ? ensure_cache<1>(op, c, gc, stream(), reserve, reusable)
+                                : ensure_cache<0>(op, c, gc, stream(), reserve, reusable);
+}
+
+void drop_gather_cache_if_built(Operator &op) {
+  std::lock_guard<std::mutex> lk(op.cache->mu);
+  if (op.cache->built) {
+    op.cache->built = false;
+    op.cache->entries.release();
+  }
+}
+
+Operator::Operator() {
+  std::lock_guard<std::mutex> lk(g_ops_mu);
+  g_ops.push_back(this);
+}
+
+Operator::~Operator() {
+  std::lock_guard<std::mutex> lk(g_ops_mu);
+  g_ops.erase(std::remove(g_ops.begin(), g_ops.end(), this), g_ops.end());
+}
+
+void drop_gather_cache(Operator &op) {
+  std::lock_guard<std::mutex> lk(op.cache->mu);
+  op.cache->drop();
+}
+
+// Caches in use (their lock held) are left alone.
+void reclaim_gather_caches(int device) {
+  std::lock_guard<std::mutex> lk(g_ops_mu);
+  for (Operator *op : g_ops) {
+    if (!op->basis || op->basis->device != device) continue;
+    GatherCache &gc = *op->cache;
+    if (!gc.mu.try_lock()) continue;
+    if (gc.built) {
+      gc.built = false;
+      gc.entries.release();
+    }
+    gc.mu.unlock();
+  }
 }
 
 bool matvec_block_supported(const Operator &op) {
@@ -523,11 +785,13 @@ void ensure_schedule(Operator &op) {
   const bool want = op.schedule_request == 1 ||
                     (op.schedule_request < 0 && b.D >= static_cast<uint64_t>(kScheduleTile));
   if (!want || !t.is_real || units > (t.bonds ? 128 : 256) || b.D == 0) {
+    if (op.schedule_built || op.sched.ptr) drop_gather_cache(op); // cache entries follow the slots
     op.schedule_built = false;
     op.sched.release();
     return;
   }
   if (op.schedule_built) return;
+  drop_gather_cache(op);
   hipStream_t s = stream();
   op.sched.allocate(b.D);
   const Ctx c = make_ctx(op, nullptr, nullptr);

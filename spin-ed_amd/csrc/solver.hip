@@ -33,6 +33,7 @@ namespace {
 
 constexpr int kBlock = 256;
 constexpr int kMaxBasis = 64;
+constexpr int kCacheMinBasis = 16; // smallest automatic basis that keeps a gather cache
 constexpr int kDotChunk = 8;
 
 unsigned grid_for(uint64_t n, uint64_t cap = 8192) {
@@ -471,8 +472,8 @@ template <class VT, bool C = false> struct Phase {
   Phase(Operator &o, uint64_t n_, int m_, int k_, int keep_, hipStream_t s_)
       : op(o), n(n_), ne(C ? 2 * n_ : n_), m(m_), k(k_), keep(keep_), s(s_) {
     static_assert(!C || std::is_same<VT, double>::value, "complex vectors are stored in FP64");
-    // FP64-sized blocks for either storage type: the FP32 phase of a mixed
-    // solve hands them on to the FP64 phase through the pool
+    // FP64-sized pooled blocks for either storage type: the FP32 phase of a
+    // mixed solve hands them on to the FP64 phase without a free/allocate
     vecs.resize(m + 1);
     for (auto &v : vecs) v.allocate_pooled(ne, ne * sizeof(double));
     std::vector<VT *> hp(m + 1);
@@ -1023,6 +1024,28 @@ int eigsh(Operator &op, int dtype, uint64_t k, double tol, const spined_eigsh_op
     std::lock_guard<std::mutex> lk(op.mu);
     op.last_stats = st;
     return LS_SUCCESS;
+  }
+
+  // Gather cache (real one-vector solves): a cached matvec costs about a
+  // third of the orbit-and-index one, so with an automatic basis size the
+  // cache is kept at the price of a smaller Krylov basis, down to
+  // kCacheMinBasis vectors; otherwise it gives its memory to the vectors.
+  if (!cplx && block_width(opts.max_block_size) == 1) {
+    const uint64_t blk = n * 8;
+    const int extra = 5; // w, start, y, hy and one block of slack
+    const int mmin = opts.max_basis_size > 0 ? m : std::min(m, kCacheMinBasis);
+    if (prepare_gather_cache(op, static_cast<uint64_t>(mmin + 1 + extra) * blk, blk)) {
+      size_t free_b = 0, total_b = 0;
+      SPINED_HIP(hipMemGetInfo(&free_b, &total_b));
+      const int fit = static_cast<int>((free_b + pool_bytes(blk)) / blk) - 1 - extra;
+      if (fit < mmin) {
+        drop_gather_cache_if_built(op);
+      } else if (fit < m) {
+        spined_eigsh_options o2 = opts;
+        o2.max_basis_size = fit;
+        pick_sizes(n, k, o2, m, keep);
+      }
+    }
   }
 
   // ---- start vector (FP64, in a scratch buffer) --------------------------

@@ -35,7 +35,8 @@ SPINED_SYMBOLS = (
     "spined_operator_set_orbit_method", "spined_operator_set_schedule",
     "spined_operator_get_schedule", "spined_basis_set_bucket_bits",
     "spined_basis_get_bucket_bits", "spined_basis_set_index_kind", "spined_basis_get_index_kind",
-    "spined_basis_is_complete",
+    "spined_basis_is_complete", "spined_operator_set_gather_cache",
+    "spined_operator_get_gather_cache",
     "spined_basis_sector_dimension", "spined_debug_orbit",
     "spined_debug_ring_available", "spined_debug_orbit_hop", "spined_device_synchronize",
     "spined_device_malloc", "spined_device_release_pool",
@@ -163,6 +164,8 @@ def load():
         "spined_operator_set_schedule": (I, [P, I]),
         "spined_operator_autotune": (I, [P, C.c_int32, C.POINTER(SpinedAutotuneReport)]),
         "spined_operator_get_schedule": (I, [P]),
+        "spined_operator_set_gather_cache": (I, [P, I]),
+        "spined_operator_get_gather_cache": (I, [P]),
         "spined_basis_set_bucket_bits": (I, [P, I]),
         "spined_basis_get_bucket_bits": (I, [P]),
         "spined_basis_set_index_kind": (I, [P, I]),

@@ -383,6 +383,17 @@ def schedule(op: Operator_) -> int:
     return int(_lib.load().spined_operator_get_schedule(op.ptr))
 
 
+def set_gather_cache(op: Operator_, mode: int) -> None:
+    """Gather cache of a real operator: -1 automatic (built when it fits in
+    free device memory), 0 off, 1 on.  Matvec sums are identical either way."""
+    check_status(_lib.load().spined_operator_set_gather_cache(op.ptr, mode))
+
+
+def gather_cache(op: Operator_) -> int:
+    """1 while the operator's gather cache is built."""
+    return int(_lib.load().spined_operator_get_gather_cache(op.ptr))
+
+
 def set_index_kind(basis: SpinBasis, kind: int) -> None:
     """0 = bucket table + binary search, 1 = hash table, -1 = automatic."""
     check_status(_lib.load().spined_basis_set_index_kind(basis.ptr, kind))

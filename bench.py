@@ -199,7 +199,12 @@ def main():
                       "train_s": round(t_train, 2), "overlap_on_training_sizes": ov}
         log(f"learned start: {train_info}")
 
+    cache_mode = {"auto": -1, "on": 1, "off": 0}[a.gather_cache]
+
     def solve(precision=a.precision, use_model=True):
+        # every solve builds its own gather cache inside its own time
+        spined.set_gather_cache(op, 0)
+        spined.set_gather_cache(op, cache_mode)
         return spined.eigsh(op, k=1, tol=a.tol, precision=precision, seed=0x5EED,
                             want_vectors=False, max_basis_size=a.basis_size, verbose=True,
                             max_block_size=a.solve_block if use_model else 0,
@@ -222,6 +227,7 @@ def main():
     elapsed = time.perf_counter() - t_start
     per_step = elapsed / steps
     last = results[-1]
+    cached = spined.gather_cache(op) == 1
     matvecs = sum(r.stats["matvecs"] for r in results)
     mv_seconds = sum(r.stats["seconds_matvec"] for r in results)
     mv_avg = mv_seconds / max(matvecs, 1)
@@ -230,8 +236,11 @@ def main():
     # the FP64 phase dominates the count in fp64 mode; in mixed mode weigh phases
     mixed_mv = sum(r.stats["matvecs_mixed"] for r in results)
     fp64_mv = matvecs - mixed_mv
-    bytes_fp64 = D * (8 + 4 + 8 + 8) + e_off * 8
-    bytes_mixed = D * (8 + 4 + 4 + 8) + e_off * 4
+    # per row: representative, stabiliser, own x, y; per off-diagonal term:
+    # the x gather, plus the 4-byte cache entry when the gather cache is used
+    ent = 4 if cached else 0
+    bytes_fp64 = D * (8 + 4 + 8 + 8) + e_off * (8 + ent)
+    bytes_mixed = D * (8 + 4 + 4 + 8) + e_off * (4 + ent)
     alg_bytes = (bytes_fp64 * fp64_mv + bytes_mixed * mixed_mv) / max(matvecs, 1)
     achieved = alg_bytes / mv_avg if mv_avg > 0 else 0.0
     # HBM bytes per matvec launch from the committed rocprofv3 FETCH_SIZE /
@@ -242,8 +251,9 @@ def main():
     if tj.exists():
         tdat = json.loads(tj.read_text())
         per = {k: v["fetch_bytes"] + v["write_bytes"] for k, v in tdat.get("per_launch", {}).items()}
-        if "fp64" in per and "mixed" in per:
-            traffic = (per["fp64"] * fp64_mv + per["mixed"] * mixed_mv) / max(matvecs, 1)
+        kf, km = ("cached_fp64", "cached_mixed") if cached else ("fp64", "mixed")
+        if kf in per and km in per:
+            traffic = (per[kf] * fp64_mv + per[km] * mixed_mv) / max(matvecs, 1)
 
     # comparison solves (outside the timed region): plain FP64 from a random
     # start, and the other storage mode from the same learned start
@@ -268,6 +278,31 @@ def main():
             compare[f"{tag}_residual"] = float(rc.rnorms[0])
             compare[f"speedup_vs_{tag}"] = t_ref / per_step
         compare["E0_difference_vs_fp64_random"] = float(last.evals[0] - compare["fp64_random_E0"])
+
+    # gather cache: built inside each solve; its build time apart (outside
+    # the timed region): a matvec that builds it minus one that reads it
+    cache_info = {"used": cached}
+    if cached:
+        spined.set_gather_cache(op, 0)
+        spined.release_device_pool()
+        xb = DeviceArray.from_numpy(np.ones(D))
+        yb = DeviceArray(D, np.float64)
+        spined.set_gather_cache(op, cache_mode)
+        device_synchronize()
+        t = time.perf_counter()
+        spined.matvec_device(op, xb, yb, 1)
+        device_synchronize()
+        t_first = time.perf_counter() - t
+        t = time.perf_counter()
+        spined.matvec_device(op, xb, yb, 1)
+        device_synchronize()
+        t_cached = time.perf_counter() - t
+        cache_info.update({"build_s": round(t_first - t_cached, 3),
+                           "cached_fp64_matvec_ms": round(t_cached * 1e3, 2),
+                           "entries_bytes_approx": e_off * 4})
+        xb.free()
+        yb.free()
+        log(f"gather cache: {cache_info}")
 
     cpu = None
     if not a.no_cpu:
@@ -298,12 +333,14 @@ def main():
                    "matvecs_per_solve": matvecs / len(results),
                    "matvecs_fp32_phase": mixed_mv / len(results),
                    "start_vector_s": last.stats["seconds_start"],
+                   "gather_cache": cache_info,
                    "matvec_ms": mv_avg * 1e3,
                    "matvec_rows_per_s": D / mv_avg if mv_avg > 0 else None,
                    "matvec_fraction_of_solve": mv_seconds / elapsed},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": PEAK_HBM / 1e9,
                      "unit": "GB/s", "frac": achieved / PEAK_HBM, "traffic": traffic,
-                     "kernel": "matvec_real_kernel", "algorithmic_bytes_per_launch": alg_bytes,
+                     "kernel": "matvec_cached_kernel" if cached else "matvec_real_kernel",
+                     "algorithmic_bytes_per_launch": alg_bytes,
                      "offdiag_gathers_per_launch": e_off},
         "cpu_baseline": cpu,
     }

@@ -144,21 +144,39 @@ __global__ void __launch_bounds__(kRowBlock)
   const uint32_t si = stab[i];
   const double sr = static_cast<double>(si);
   const uint32_t *in = entries + woff[lb] * 64 + threadIdx.x;
+  // four terms per step: four entry loads, then four x gathers in flight,
+  // then the four terms added in template order
+  constexpr int U = 4;
   int t = 0;
   while (any_set<NW>(act)) {
-    const int j = pop_first<NW>(act);
-    const uint32_t e = in[64 * static_cast<uint64_t>(t++)];
-    const uint32_t k = e & kCacheIndex;
-    if (k == kCacheIndex) continue;
-    double f = 1.0;
-    if (e & kCacheRatio) {
-      const uint32_t ck = stab[k];
-      f = rt.nd ? rt.ratio[rt.div_idx[ck] * rt.nd + rt.div_idx[si]]
-                : sqrt(static_cast<double>(ck) / sr);
+    int j[U];
+    uint32_t e[U];
+#pragma unroll
+    for (int q = 0; q < U; ++q) {
+      j[q] = pop_first<NW>(act);
+      e[q] = j[q] >= 0 ? in[64 * static_cast<uint64_t>(t + q)] : kCacheIndex;
     }
-    double c = (BONDS ? so[j] : o.coef_re[j]) * f;
-    c = (e & kCacheNeg) ? -c : c;
-    acc = fma(c, ld_re(x, k), acc);
+    t += U;
+    double xv[U];
+#pragma unroll
+    for (int q = 0; q < U; ++q) {
+      const uint32_t k = e[q] & kCacheIndex;
+      xv[q] = k != kCacheIndex ? ld_re(x, k) : 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < U; ++q) {
+      const uint32_t k = e[q] & kCacheIndex;
+      if (k == kCacheIndex) continue;
+      double f = 1.0;
+      if (e[q] & kCacheRatio) {
+        const uint32_t ck = stab[k];
+        f = rt.nd ? rt.ratio[rt.div_idx[ck] * rt.nd + rt.div_idx[si]]
+                  : sqrt(static_cast<double>(ck) / sr);
+      }
+      double c = (BONDS ? so[j[q]] : o.coef_re[j[q]]) * f;
+      c = (e[q] & kCacheNeg) ? -c : c;
+      acc = fma(c, xv[q], acc);
+    }
   }
   st_re(y, i, acc);
 }

@@ -220,7 +220,7 @@ __device__ inline void block_partials(const double (&acc)[NQ], int nq, double *_
 //   w[r] -= sum_i h1[i] V_i[r];  partial dots of V_i with the new w.
 // Per row the arithmetic equals gemv_sub_kernel followed by multidot_kernel.
 // Does nothing when skip is given and *skip != 0.
-template <class VT>
+template <class VT, int NQ>
 __global__ void __launch_bounds__(kBlock)
     sub_dot_kernel(const VT *const *__restrict__ V, int nvec, const double *__restrict__ h1,
                    double *__restrict__ w, uint64_t n, double *__restrict__ partial,
@@ -229,49 +229,49 @@ __global__ void __launch_bounds__(kBlock)
   // pointers hoisted out of the row loop; slots past nvec read V[0] (a valid
   // address) so every load is unconditional and the loads of a row are all
   // in flight together; their values are masked out of the arithmetic
-  const VT *vp[kFuseMax];
+  const VT *vp[NQ];
 #pragma unroll
-  for (int q = 0; q < kFuseMax; ++q) vp[q] = V[q < nvec ? q : 0];
-  double acc[kFuseMax];
+  for (int q = 0; q < NQ; ++q) vp[q] = V[q < nvec ? q : 0];
+  double acc[NQ];
 #pragma unroll
-  for (int q = 0; q < kFuseMax; ++q) acc[q] = 0.0;
+  for (int q = 0; q < NQ; ++q) acc[q] = 0.0;
   for (uint64_t r = blockIdx.x * static_cast<uint64_t>(kBlock) + threadIdx.x; r < n;
        r += static_cast<uint64_t>(gridDim.x) * kBlock) {
-    double vals[kFuseMax];
+    double vals[NQ];
 #pragma unroll
-    for (int q = 0; q < kFuseMax; ++q) vals[q] = to_d(vp[q][r]);
+    for (int q = 0; q < NQ; ++q) vals[q] = to_d(vp[q][r]);
     double sub = 0.0;
 #pragma unroll
-    for (int q = 0; q < kFuseMax; ++q) sub = q < nvec ? fma(h1[q], vals[q], sub) : sub;
+    for (int q = 0; q < NQ; ++q) sub = q < nvec ? fma(h1[q], vals[q], sub) : sub;
     const double wr = w[r] - sub;
     w[r] = wr;
 #pragma unroll
-    for (int q = 0; q < kFuseMax; ++q) acc[q] = q < nvec ? fma(vals[q], wr, acc[q]) : acc[q];
+    for (int q = 0; q < NQ; ++q) acc[q] = q < nvec ? fma(vals[q], wr, acc[q]) : acc[q];
   }
-  block_partials<kFuseMax>(acc, nvec, partial);
+  block_partials<NQ>(acc, nvec, partial);
 }
 
 // Last update fused with the norm: w[r] -= sum_i c[i] V_i[r]; partial
 // sums of w[r]^2 (the norm_w multidot's arithmetic).  c = h2, or h1 when
 // skip is given and *skip != 0 (the first pass was the only one).
-template <class VT>
+template <class VT, int NQ>
 __global__ void __launch_bounds__(kBlock)
     sub_norm_kernel(const VT *const *__restrict__ V, int nvec, const double *__restrict__ h2,
                     double *__restrict__ w, uint64_t n, double *__restrict__ partial,
                     const double *__restrict__ h1 = nullptr, const int *__restrict__ skip = nullptr) {
   if (skip && *skip) h2 = h1;
-  const VT *vp[kFuseMax];
+  const VT *vp[NQ];
 #pragma unroll
-  for (int q = 0; q < kFuseMax; ++q) vp[q] = V[q < nvec ? q : 0];
+  for (int q = 0; q < NQ; ++q) vp[q] = V[q < nvec ? q : 0];
   double acc[1] = {0.0};
   for (uint64_t r = blockIdx.x * static_cast<uint64_t>(kBlock) + threadIdx.x; r < n;
        r += static_cast<uint64_t>(gridDim.x) * kBlock) {
-    double vals[kFuseMax];
+    double vals[NQ];
 #pragma unroll
-    for (int q = 0; q < kFuseMax; ++q) vals[q] = to_d(vp[q][r]);
+    for (int q = 0; q < NQ; ++q) vals[q] = to_d(vp[q][r]);
     double sub = 0.0;
 #pragma unroll
-    for (int q = 0; q < kFuseMax; ++q) sub = q < nvec ? fma(h2[q], vals[q], sub) : sub;
+    for (int q = 0; q < NQ; ++q) sub = q < nvec ? fma(h2[q], vals[q], sub) : sub;
     const double wr = w[r] - sub;
     w[r] = wr;
     acc[0] = fma(wr, wr, acc[0]);
@@ -281,31 +281,31 @@ __global__ void __launch_bounds__(kBlock)
 
 // First Gram-Schmidt pass with the squared norm of w: partial dots of V_i
 // (i < nvec <= kFuseMax) with w in slots 0..nvec-1 and w.w in slot nvec.
-template <class VT>
+template <class VT, int NQ>
 __global__ void __launch_bounds__(kBlock)
     dot_ww_kernel(const VT *const *__restrict__ V, int nvec, const double *__restrict__ w,
                   uint64_t n, double *__restrict__ partial) {
-  const VT *vp[kFuseMax];
+  const VT *vp[NQ];
 #pragma unroll
-  for (int q = 0; q < kFuseMax; ++q) vp[q] = V[q < nvec ? q : 0];
-  double acc[kFuseMax + 1];
+  for (int q = 0; q < NQ; ++q) vp[q] = V[q < nvec ? q : 0];
+  double acc[NQ + 1];
 #pragma unroll
-  for (int q = 0; q <= kFuseMax; ++q) acc[q] = 0.0;
+  for (int q = 0; q <= NQ; ++q) acc[q] = 0.0;
   for (uint64_t r = blockIdx.x * static_cast<uint64_t>(kBlock) + threadIdx.x; r < n;
        r += static_cast<uint64_t>(gridDim.x) * kBlock) {
-    double vals[kFuseMax];
+    double vals[NQ];
 #pragma unroll
-    for (int q = 0; q < kFuseMax; ++q) vals[q] = to_d(vp[q][r]);
+    for (int q = 0; q < NQ; ++q) vals[q] = to_d(vp[q][r]);
     const double wr = w[r];
 #pragma unroll
-    for (int q = 0; q < kFuseMax; ++q) acc[q] = q < nvec ? fma(vals[q], wr, acc[q]) : acc[q];
-    acc[kFuseMax] = fma(wr, wr, acc[kFuseMax]);
+    for (int q = 0; q < NQ; ++q) acc[q] = q < nvec ? fma(vals[q], wr, acc[q]) : acc[q];
+    acc[NQ] = fma(wr, wr, acc[NQ]);
   }
   // w.w moves to slot nvec
-  double out[kFuseMax + 1];
+  double out[NQ + 1];
 #pragma unroll
-  for (int q = 0; q <= kFuseMax; ++q) out[q] = q < nvec ? acc[q] : (q == nvec ? acc[kFuseMax] : 0.0);
-  block_partials<kFuseMax + 1>(out, nvec + 1, partial);
+  for (int q = 0; q <= NQ; ++q) out[q] = q < nvec ? acc[q] : (q == nvec ? acc[NQ] : 0.0);
+  block_partials<NQ + 1>(out, nvec + 1, partial);
 }
 
 // Whether the second Gram-Schmidt pass can be left out: h[0..nvec) are the
@@ -591,19 +591,34 @@ template <class VT, bool C = false> struct Phase {
   // cancellation (second_pass_test_kernel).  Complex vectors take reorth()
   // and norm_w().
   void reorth_norm_fused(int nvec, double *h1, double *h2, double *norm_out) {
-    dot_ww_kernel<VT><<<G, kBlock, 0, s>>>(vptr.ptr, nvec, w.ptr, n, partial.ptr);
-    SPINED_CHECK_LAUNCH();
-    reduce_partials_kernel<<<nvec + 1, kBlock, 0, s>>>(partial.ptr, nvec + 1, G, h1, 0);
-    SPINED_CHECK_LAUNCH();
-    second_pass_test_kernel<<<1, 64, 0, s>>>(h1, nvec, kSecondPassEta, h2, skip.ptr);
-    SPINED_CHECK_LAUNCH();
-    sub_dot_kernel<VT><<<G, kBlock, 0, s>>>(vptr.ptr, nvec, h1, w.ptr, n, partial.ptr, skip.ptr);
-    SPINED_CHECK_LAUNCH();
-    reduce_partials_kernel<<<nvec, kBlock, 0, s>>>(partial.ptr, nvec, G, h2, 0, skip.ptr);
-    SPINED_CHECK_LAUNCH();
-    sub_norm_kernel<VT><<<G, kBlock, 0, s>>>(vptr.ptr, nvec, h2, w.ptr, n, partial.ptr, h1,
-                                             skip.ptr);
-    SPINED_CHECK_LAUNCH();
+    // register width: the smallest multiple of 8 holding nvec vectors (the
+    // slots past nvec still issue loads, of V[0])
+    const int nq = (nvec + 7) / 8 * 8;
+    auto passes = [&](auto width) {
+      constexpr int NQ = decltype(width)::value;
+      dot_ww_kernel<VT, NQ><<<G, kBlock, 0, s>>>(vptr.ptr, nvec, w.ptr, n, partial.ptr);
+      SPINED_CHECK_LAUNCH();
+      reduce_partials_kernel<<<nvec + 1, kBlock, 0, s>>>(partial.ptr, nvec + 1, G, h1, 0);
+      SPINED_CHECK_LAUNCH();
+      second_pass_test_kernel<<<1, 64, 0, s>>>(h1, nvec, kSecondPassEta, h2, skip.ptr);
+      SPINED_CHECK_LAUNCH();
+      sub_dot_kernel<VT, NQ>
+          <<<G, kBlock, 0, s>>>(vptr.ptr, nvec, h1, w.ptr, n, partial.ptr, skip.ptr);
+      SPINED_CHECK_LAUNCH();
+      reduce_partials_kernel<<<nvec, kBlock, 0, s>>>(partial.ptr, nvec, G, h2, 0, skip.ptr);
+      SPINED_CHECK_LAUNCH();
+      sub_norm_kernel<VT, NQ><<<G, kBlock, 0, s>>>(vptr.ptr, nvec, h2, w.ptr, n, partial.ptr,
+                                                   h1, skip.ptr);
+      SPINED_CHECK_LAUNCH();
+    };
+    if (nq <= 8)
+      passes(std::integral_constant<int, 8>{});
+    else if (nq <= 16)
+      passes(std::integral_constant<int, 16>{});
+    else if (nq <= 24)
+      passes(std::integral_constant<int, 24>{});
+    else
+      passes(std::integral_constant<int, kFuseMax>{});
     reduce_partials_kernel<<<1, kBlock, 0, s>>>(partial.ptr, 1, G, norm_out, 0);
     SPINED_CHECK_LAUNCH();
     sqrt_inplace_kernel<<<1, 1, 0, s>>>(norm_out);

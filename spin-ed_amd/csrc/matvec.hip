@@ -181,6 +181,61 @@ __global__ void __launch_bounds__(kRowBlock)
   st_re(y, i, acc);
 }
 
+// Row-interleaved block of K vectors from the gather cache: per column the
+// sums of matvec_block_kernel (and so of matvec_real_kernel).
+template <int NW, bool BONDS, int K, class XT, class YT>
+__global__ void __launch_bounds__(kRowBlock)
+    matvec_cached_block_kernel(OpArgs o, BondArgs bd, RatioArgs rt,
+                               const uint64_t *__restrict__ reps, const uint32_t *__restrict__ stab,
+                               uint64_t D, const XT *__restrict__ x, YT *__restrict__ y,
+                               const uint16_t *__restrict__ sched, const uint64_t *__restrict__ woff,
+                               const uint32_t *__restrict__ entries) {
+  __shared__ double so[BONDS ? 64 * NW : 1];
+  if constexpr (BONDS) {
+    for (int k = threadIdx.x; k < bd.nbonds; k += kRowBlock) so[k] = bd.o[k];
+    __syncthreads();
+  }
+  using Row = RowK<XT, K>;
+  const Row *__restrict__ xr = reinterpret_cast<const Row *>(x);
+  const uint64_t lb = xcd_block(blockIdx.x, gridDim.x);
+  const uint64_t slot = lb * kRowBlock + threadIdx.x;
+  if (slot >= D) return;
+  const uint64_t i = scheduled_row(sched, slot);
+  uint64_t act[NW];
+  const double diag = row_pass_a<NW, BONDS>(o, bd, reps[i], act);
+  double acc[K];
+  {
+    const Row xi = xr[i];
+#pragma unroll
+    for (int c = 0; c < K; ++c) acc[c] = diag * static_cast<double>(xi.v[c]);
+  }
+  const uint32_t si = stab[i];
+  const double sr = static_cast<double>(si);
+  const uint32_t *in = entries + woff[lb] * 64 + threadIdx.x;
+  int t = 0;
+  while (any_set<NW>(act)) {
+    const int j = pop_first<NW>(act);
+    const uint32_t e = in[64 * static_cast<uint64_t>(t++)];
+    const uint32_t k = e & kCacheIndex;
+    if (k == kCacheIndex) continue;
+    const Row xk = xr[k];
+    double f = 1.0;
+    if (e & kCacheRatio) {
+      const uint32_t ck = stab[k];
+      f = rt.nd ? rt.ratio[rt.div_idx[ck] * rt.nd + rt.div_idx[si]]
+                : sqrt(static_cast<double>(ck) / sr);
+    }
+    double cf = (BONDS ? so[j] : o.coef_re[j]) * f;
+    cf = (e & kCacheNeg) ? -cf : cf;
+#pragma unroll
+    for (int c = 0; c < K; ++c) acc[c] = fma(cf, static_cast<double>(xk.v[c]), acc[c]);
+  }
+  RowK<YT, K> out;
+#pragma unroll
+  for (int c = 0; c < K; ++c) out.v[c] = static_cast<YT>(acc[c]);
+  reinterpret_cast<RowK<YT, K> *>(y)[i] = out;
+}
+
 // Row-interleaved block of K real vectors (x[row * K + c]); per column the
 // same sums as matvec_real_kernel.
 template <int METHOD, int NW, bool BONDS, int K, class XT, class YT>
@@ -607,6 +662,50 @@ void launch_block(const Ctx &c, int K, const XT *x, YT *y, hipStream_t s) {
   }
 }
 
+template <int K, class XT, class YT>
+void launch_cached_block(const Ctx &c, const GatherCache &gc, const XT *x, YT *y, hipStream_t s) {
+  const unsigned g = static_cast<unsigned>((c.D + kRowBlock - 1) / kRowBlock);
+  with_nw(c, [&](auto nw, auto bonds) {
+    matvec_cached_block_kernel<decltype(nw)::value, decltype(bonds)::value, K, XT, YT>
+        <<<g, kRowBlock, 0, s>>>(c.o, c.bd, c.rt, c.ix.reps, c.stab, c.D, x, y, c.sched,
+                                 gc.woff.ptr, gc.entries.ptr);
+  });
+  SPINED_CHECK_LAUNCH();
+}
+
+template <class XT, class YT>
+void launch_cached_block_k(const Ctx &c, const GatherCache &gc, int K, const XT *x, YT *y,
+                           hipStream_t s) {
+  if (K == 8)
+    launch_cached_block<8>(c, gc, x, y, s);
+  else if (K == 4)
+    launch_cached_block<4>(c, gc, x, y, s);
+  else if (K == 2)
+    launch_cached_block<2>(c, gc, x, y, s);
+  else
+    launch_cached_block<1>(c, gc, x, y, s);
+}
+
+// Block matvec from the gather cache when there is (or can be) one.
+template <int METHOD> bool try_cached_block(const MatvecBlockCall &m, const Ctx &c) {
+  GatherCache &gc = *m.op->cache;
+  std::lock_guard<std::mutex> lk(gc.mu);
+  if (!ensure_cache<METHOD>(*m.op, c, gc, m.stream)) return false;
+  using VT = VecType;
+  if (m.xt == VT::F64 && m.yt == VT::F64)
+    launch_cached_block_k(c, gc, m.K, static_cast<const double *>(m.x), static_cast<double *>(m.y),
+                          m.stream);
+  else if (m.xt == VT::F32 && m.yt == VT::F32)
+    launch_cached_block_k(c, gc, m.K, static_cast<const float *>(m.x), static_cast<float *>(m.y),
+                          m.stream);
+  else if (m.xt == VT::F32 && m.yt == VT::F64)
+    launch_cached_block_k(c, gc, m.K, static_cast<const float *>(m.x), static_cast<double *>(m.y),
+                          m.stream);
+  else
+    return false;
+  return true;
+}
+
 template <int METHOD> void dispatch_block(const MatvecBlockCall &m, const Ctx &c) {
   hipStream_t s = m.stream;
   using VT = VecType;
@@ -758,6 +857,8 @@ void launch_matvec_block(const MatvecBlockCall &m) {
   if (!matvec_block_supported(op))
     throw Error(LS_NOT_IMPLEMENTED, "block matvec needs a real operator with <= 256 templates");
   const Ctx c = make_ctx(op, nullptr, nullptr);
+  const bool done = op.orbit_method() == 1 ? try_cached_block<1>(m, c) : try_cached_block<0>(m, c);
+  if (done) return;
   if (op.orbit_method() == 1)
     dispatch_block<1>(m, c);
   else

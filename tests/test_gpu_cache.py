@@ -104,3 +104,29 @@ def test_cache_eigsh_identical_and_auto():
     assert spined.gather_cache(op3) == 1
     spined.set_gather_cache(op3, 0)
     np.testing.assert_array_equal(spined.apply(op3, x), y)
+
+
+@pytest.mark.parametrize("K", [2, 4, 8])
+def test_cache_block_matvec_bit_identical(K):
+    """Row-interleaved blocks (spined_operator_matmat_rows_device) from the
+    gather cache: per column the sums of the orbit-and-index block kernel."""
+    from spined.device import DeviceArray
+    for spec in (ring(20, k=0, p=0, z=1),
+                 Spec(16, 8, 1, lattice.square_symmetries(4),
+                      [(lattice.HEISENBERG, lattice.square_bonds(4))], "square4x4")):
+        print("case", spec.name, K, flush=True)
+        basis, op = engine(spec)
+        D = spined.get_number_states(basis)
+        X = np.random.default_rng(K).uniform(-1, 1, (D, K))
+        out = {}
+        for mode in (0, 1):
+            spined.set_gather_cache(op, mode)
+            for code, dt in ((1, np.float64), (0, np.float32)):
+                xd = DeviceArray.from_numpy(np.ascontiguousarray(X.astype(dt)))
+                yd = DeviceArray.zeros(D * K, np.float64)
+                spined.matvec_rows_device(op, xd, yd, code, K, y_is_fp64=code == 0)
+                spined.synchronize()
+                out[(mode, code)] = yd.numpy()
+            assert spined.gather_cache(op) == mode
+        np.testing.assert_array_equal(out[(1, 1)], out[(0, 1)])
+        np.testing.assert_array_equal(out[(1, 0)], out[(0, 0)])

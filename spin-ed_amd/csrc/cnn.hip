@@ -308,6 +308,113 @@ __global__ void __launch_bounds__(64)
   }
 }
 
+// Streaming inference (no stored pre-activations, n >= 8): the forward
+// kernel's outputs (same operands and MFMA sums) from 7 KB of LDS per wave
+// instead of 21 KB, so about three times the waves stay resident.
+//  * layer 1 is the table of its 32 input patterns; layer 2 reads its B
+//    fragments from the table directly, so layer 1 is never stored;
+//  * layer 2's outputs go to a ring of the last 8 positions (plus a copy of
+//    positions 0..3 for the wrap-around), and layer 3 at position p - 2
+//    runs as soon as layer 2 at p is done.
+constexpr int kRing = 8;
+
+__device__ inline int pattern_at(uint64_t state, int n, int pos) {
+  // bit k: spin at pos + k - 2 (cyclic), for a window inside [0, n)
+  int start = pos - 2;
+  start = start < 0 ? start + n : start;
+  start = start >= n ? start - n : start;
+  const uint64_t rot = start == 0 ? state : ((state >> start) | (state << (n - start)));
+  return static_cast<int>(rot & 31u);
+}
+
+__global__ void __launch_bounds__(64)
+    cnn_infer_kernel(CnnArgs c, const uint64_t *__restrict__ states, float *__restrict__ out_a,
+                     float *__restrict__ out_t) {
+  __shared__ __attribute__((aligned(16))) __bf16 tab[32 * CH];
+  __shared__ __attribute__((aligned(16))) __bf16 ring[kRing * 16 * CH];
+  __shared__ __attribute__((aligned(16))) __bf16 head[4 * 16 * CH];
+  const int n = c.n;
+  const int lane = threadIdx.x, s = lane & 15, g = lane >> 4;
+  const uint64_t base = static_cast<uint64_t>(blockIdx.x) * 16;
+  const uint64_t sample = base + s;
+  const uint64_t state = sample < c.count ? states[sample] : 0;
+  const float *P = c.params;
+  for (int e = lane; e < 32 * CH; e += 64) {
+    const int pat = e / CH, ch = e % CH;
+    float v = 0.0f;
+    for (int k = 0; k < KW; ++k) {
+      const float wk = static_cast<float>(static_cast<__bf16>(P[OFF_W1 + ch * KW + k]));
+      v += ((pat >> k) & 1) ? wk : -wk;
+    }
+    tab[e] = static_cast<__bf16>(gelu_f(v + P[OFF_B1 + ch]));
+  }
+  __syncthreads();
+  bf16x8 a2[3], a3[3];
+  load_conv_weights(P + OFF_W2, false, a2);
+  load_conv_weights(P + OFF_W3, false, a3);
+  float b2[4], b3[4];
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    b2[r] = P[OFF_B2 + 4 * g + r];
+    b3[r] = P[OFF_B3 + 4 * g + r];
+  }
+  // B fragment q (rows k = 32 q + 8 g .. + 7: offset d = k >> 4, channels
+  // c0 = k & 15) of a conv at position pos
+  auto frag2 = [&](int pos, int q) {
+    const int k0 = 32 * q + 8 * g, d = k0 >> 4, c0 = k0 & 15;
+    if (d >= KW) return zero8();
+    int p = pos + d - 2;
+    p = p < 0 ? p + n : (p >= n ? p - n : p);
+    return *reinterpret_cast<const bf16x8 *>(tab + pattern_at(state, n, p) * CH + c0);
+  };
+  auto frag3 = [&](int pos, int q, bool tail) {
+    const int k0 = 32 * q + 8 * g, d = k0 >> 4, c0 = k0 & 15;
+    if (d >= KW) return zero8();
+    int p = pos + d - 2;
+    p = p < 0 ? p + n : (p >= n ? p - n : p);
+    const __bf16 *src = (tail && p < 4) ? head + (p * 16 + s) * CH : ring + ((p & (kRing - 1)) * 16 + s) * CH;
+    return *reinterpret_cast<const bf16x8 *>(src + c0);
+  };
+  float f[4] = {0.f, 0.f, 0.f, 0.f};
+  auto layer3 = [&](int q, bool tail) {
+    floatx4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) acc = mfma(a3[k], frag3(q, k, tail), acc);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) f[r] += gelu_f(acc[r] + b3[r]);
+  };
+  // (one wave per workgroup: its LDS accesses stay in program order)
+  for (int pos = 0; pos < n; ++pos) {
+    floatx4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) acc = mfma(a2[k], frag2(pos, k), acc);
+    bf16x4 o;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) o[r] = static_cast<__bf16>(gelu_f(acc[r] + b2[r]));
+    *reinterpret_cast<bf16x4 *>(ring + ((pos & (kRing - 1)) * 16 + s) * CH + 4 * g) = o;
+    if (pos < 4) *reinterpret_cast<bf16x4 *>(head + (pos * 16 + s) * CH + 4 * g) = o;
+    if (pos >= 4) layer3(pos - 2, false);
+  }
+  layer3(n - 2, true);
+  layer3(n - 1, true);
+  layer3(0, true);
+  layer3(1, true);
+  float pa = 0.f, pt = 0.f;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    pa += P[OFF_U + 4 * g + r] * f[r];
+    pt += P[OFF_V + 4 * g + r] * f[r];
+  }
+  pa += __shfl_xor(pa, 16, 64);
+  pa += __shfl_xor(pa, 32, 64);
+  pt += __shfl_xor(pt, 16, 64);
+  pt += __shfl_xor(pt, 32, 64);
+  if (g == 0 && sample < c.count) {
+    out_a[sample] = pa + P[OFF_U0];
+    out_t[sample] = pt + P[OFF_V0];
+  }
+}
+
 // Weight gradient of one conv layer: C_d[co][ci] += sum_slots dz[slot][co] *
 // h[slot shifted by d][ci]; 32 (sample, position) slots per MFMA.
 // Layer 1 (first = true): C[co][d] with h = spins.
@@ -657,9 +764,12 @@ int spined_cnn_forward_device(const spined_cnn *c, int n, uint64_t count, const 
     if (count == 0) return;
     spined::CnnArgs a{c->m.params.ptr, n, count};
     const unsigned blocks = static_cast<unsigned>((count + 15) / 16);
-    hipLaunchKernelGGL(spined::cnn_forward_kernel, dim3(blocks), dim3(64),
-                       spined::lds_bytes_forward(n), spined::stream(), a, states, out_a, out_t,
-                       z_store);
+    if (!z_store && n >= spined::kRing)
+      spined::cnn_infer_kernel<<<blocks, 64, 0, spined::stream()>>>(a, states, out_a, out_t);
+    else
+      hipLaunchKernelGGL(spined::cnn_forward_kernel, dim3(blocks), dim3(64),
+                         spined::lds_bytes_forward(n), spined::stream(), a, states, out_a, out_t,
+                         z_store);
     SPINED_CHECK_LAUNCH();
   });
 }
@@ -739,8 +849,11 @@ void cnn_start_vector(const spined_cnn *c, const Basis &b, uint64_t marshall_mas
   DeviceBuffer<float> a(D), t(D);
   CnnArgs args{c->m.params.ptr, b.n, D};
   const unsigned blocks = static_cast<unsigned>((D + 15) / 16);
-  hipLaunchKernelGGL(cnn_forward_kernel, dim3(blocks), dim3(64), lds_bytes_forward(b.n), s, args,
-                     b.reps.ptr, a.ptr, t.ptr, static_cast<float *>(nullptr));
+  if (b.n >= kRing)
+    cnn_infer_kernel<<<blocks, 64, 0, s>>>(args, b.reps.ptr, a.ptr, t.ptr);
+  else
+    hipLaunchKernelGGL(cnn_forward_kernel, dim3(blocks), dim3(64), lds_bytes_forward(b.n), s,
+                       args, b.reps.ptr, a.ptr, t.ptr, static_cast<float *>(nullptr));
   SPINED_CHECK_LAUNCH();
   const unsigned mb = 1024;
   DeviceBuffer<float> part(mb);

@@ -40,7 +40,9 @@ typedef struct spined_eigsh_options {
   int32_t precision;        /* spined_precision */
   int32_t start;            /* spined_start */
   uint64_t seed;
-  int32_t max_basis_size;   /* Lanczos basis m (max_primme_basis_size); 0 = auto */
+  int32_t max_basis_size;   /* Lanczos basis m (max_primme_basis_size); 0 = auto:
+                               max(2k + 10, 24), lowered (down to 16) when that
+                               keeps the operator's gather cache in memory */
   int32_t min_restart_size; /* vectors kept on restart (min_primme_restart_size); 0 = auto */
   int32_t max_block_size;   /* max_primme_block_size: 0/1 = one-vector recurrence;
                                2, 4, 8 = block Lanczos phase on row-interleaved

@@ -201,10 +201,14 @@ def main():
 
     cache_mode = {"auto": -1, "on": 1, "off": 0}[a.gather_cache]
 
-    def solve(precision=a.precision, use_model=True):
-        # every solve builds its own gather cache inside its own time
+    def fresh_cache():
+        # every timed solve builds its own gather cache inside its own time;
+        # the previous one is freed before the clock starts
         spined.set_gather_cache(op, 0)
         spined.set_gather_cache(op, cache_mode)
+        device_synchronize()
+
+    def solve(precision=a.precision, use_model=True):
         return spined.eigsh(op, k=1, tol=a.tol, precision=precision, seed=0x5EED,
                             want_vectors=False, max_basis_size=a.basis_size, verbose=True,
                             max_block_size=a.solve_block if use_model else 0,
@@ -215,16 +219,20 @@ def main():
         r = solve()
         log(f"warmup {i}: E0={r.evals[0]:.15f} res={r.rnorms[0]:.2e} {r.stats}")
 
+    # each step is bracketed by a device synchronisation on both sides; the
+    # previous step's gather cache is freed between steps, outside the clock,
+    # and every step builds its own inside it
     results = []
-    device_synchronize()
-    t_start = time.perf_counter()
+    elapsed = 0.0
     for i in range(steps):
+        fresh_cache()
+        t_step = time.perf_counter()
         r = solve()
+        device_synchronize()
+        elapsed += time.perf_counter() - t_step
         results.append(r)
         log(f"step {i}: E0={r.evals[0]:.15f} res={r.rnorms[0]:.2e} matvecs={r.stats['matvecs']} "
             f"t={r.stats['seconds_total']:.3f}s matvec_t={r.stats['seconds_matvec']:.3f}s")
-    device_synchronize()
-    elapsed = time.perf_counter() - t_start
     per_step = elapsed / steps
     last = results[-1]
     cached = spined.gather_cache(op) == 1
@@ -265,7 +273,7 @@ def main():
             other = "mixed" if a.precision == "fp64" else "fp64"
             runs.append((f"{other}_learned", other, True))
         for tag, prec, use_model in runs:
-            device_synchronize()
+            fresh_cache()
             tc = time.perf_counter()
             rc = solve(prec, use_model=use_model)
             device_synchronize()

@@ -104,7 +104,8 @@ int spined_device_malloc(void **out, uint64_t bytes);
 int spined_device_free(void *ptr);
 /* The eigensolver keeps its vector blocks (those of 256 MiB or more) after a
  * solve and reuses them for the next solve or phase of the same size on the
- * device; this returns them to the driver (hipFree).  The library also does
+ * device, and so does a dropped gather cache for its next build; this returns
+ * them to the driver (hipFree).  The library also does
  * so by itself before any allocation the driver would refuse.  Call it
  * before allocating large device memory outside this library. */
 int spined_device_release_pool(void);

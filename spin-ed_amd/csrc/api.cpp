@@ -83,8 +83,8 @@ void *device_alloc(size_t bytes, bool pooled) {
   hipError_t e = hipMalloc(&p, bytes);
   if (e != hipSuccess) {
     (void)hipGetLastError();
+    reclaim_gather_caches(d); // their entries go to the pool, freed next
     release_blocks(d);
-    reclaim_gather_caches(d);
     e = hipMalloc(&p, bytes);
   }
   if (e != hipSuccess) {

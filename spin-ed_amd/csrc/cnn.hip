@@ -846,7 +846,11 @@ void cnn_start_vector(const spined_cnn *c, const Basis &b, uint64_t marshall_mas
   hipStream_t s = stream();
   const uint64_t D = b.D;
   if (D == 0) return;
-  DeviceBuffer<float> a(D), t(D);
+  // network outputs in FP64-vector-sized pooled blocks: a solve that follows
+  // takes them over as Krylov vectors without a free and a fresh allocation
+  DeviceBuffer<float> a, t;
+  a.allocate_pooled(D, D * sizeof(double));
+  t.allocate_pooled(D, D * sizeof(double));
   CnnArgs args{c->m.params.ptr, b.n, D};
   const unsigned blocks = static_cast<unsigned>((D + 15) / 16);
   if (b.n >= kRing)

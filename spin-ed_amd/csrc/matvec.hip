@@ -581,11 +581,14 @@ bool ensure_cache(const Operator &op, const Ctx &c, GatherCache &gc, hipStream_t
   if (gc.request < 0) {
     size_t free_b = 0, total_b = 0;
     SPINED_HIP(hipMemGetInfo(&free_b, &total_b));
+    free_b += pool_bytes(gc.total * 4) > 0 ? gc.total * 4 : 0; // a pooled block of this size
     if (free_b < gc.total * 4 + kCacheReserve || free_b + reusable < gc.total * 4 + reserve)
       return false;
   }
   try {
-    gc.entries.allocate(std::max<uint64_t>(gc.total, 1));
+    // pooled: a rebuild of the same operator's cache takes the block back
+    // without a free and a fresh multi-GB mapping
+    gc.entries.allocate_pooled(std::max<uint64_t>(gc.total, 1));
   } catch (const Error &) {
     if (gc.request > 0) throw;
     return false;

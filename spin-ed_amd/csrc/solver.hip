@@ -1066,6 +1066,7 @@ int eigsh(Operator &op, int dtype, uint64_t k, double tol, const spined_eigsh_op
   // ---- start vector (FP64, in a scratch buffer) --------------------------
   DeviceBuffer<double> start;
   start.allocate_pooled(ne);
+
   const auto ts = std::chrono::steady_clock::now();
   std::vector<double> host_start;
   if (opts.start == SPINED_START_VECTOR) {
@@ -1088,6 +1089,8 @@ int eigsh(Operator &op, int dtype, uint64_t k, double tol, const spined_eigsh_op
   }
   SPINED_HIP(hipStreamSynchronize(s));
   st.seconds_start = seconds_since(ts);
+  if (opts.verbose) log_line("start vector %.3f s (set-up before it %.3f s)", st.seconds_start,
+                             std::chrono::duration<double>(ts - t0).count());
 
   std::vector<double> theta(k), est(k);
   DeviceBuffer<double> y, hy;
@@ -1118,7 +1121,11 @@ int eigsh(Operator &op, int dtype, uint64_t k, double tol, const spined_eigsh_op
       n > static_cast<uint64_t>(m)) {
     // FP32-storage phase down to its floor, then hand the FP64 Ritz vector
     // (sum of the k lowest) to the FP64 phase.
+    const auto ta = std::chrono::steady_clock::now();
     Phase<float> p(op, n, m, static_cast<int>(k), keep, s);
+    if (opts.verbose)
+      log_line("FP32 phase: %d vectors allocated in %.3f s (since the start vector %.3f s)", m + 1,
+               seconds_since(ta), seconds_since(ts) - st.seconds_start);
     SPINED_HIP(hipMemcpyAsync(p.w.ptr, start.ptr, n * 8, hipMemcpyDeviceToDevice, s));
     p.set_start_from_w();
     const double tol32 = std::max(tol, 1e-5);

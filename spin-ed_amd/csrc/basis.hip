@@ -413,8 +413,7 @@ void build_basis(Basis &b) {
     else
       enumerate_kernel<0><<<nb, kBlock, 0, s>>>(e, ga, ra, w0, nw, masks.ptr, counts.ptr);
     SPINED_CHECK_LAUNCH();
-    scan_counts_kernel<<<1, 1024, 0, s>>>(counts.ptr, nb, offsets.ptr, seg_total.ptr);
-    SPINED_CHECK_LAUNCH();
+    scan_counts(counts.ptr, nb, offsets.ptr, seg_total.ptr, s);
     uint64_t segc = 0;
     SPINED_HIP(hipMemcpyAsync(&segc, seg_total.ptr, 8, hipMemcpyDeviceToHost, s));
     SPINED_HIP(hipStreamSynchronize(s));

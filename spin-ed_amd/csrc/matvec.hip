@@ -570,8 +570,7 @@ bool ensure_cache(const Operator &op, const Ctx &c, GatherCache &gc, hipStream_t
       return false;
     SPINED_CHECK_LAUNCH();
     gc.woff.allocate(waves);
-    scan_counts_kernel<<<1, 1024, 0, s>>>(wt.ptr, waves, gc.woff.ptr, tot.ptr);
-    SPINED_CHECK_LAUNCH();
+    scan_counts(wt.ptr, waves, gc.woff.ptr, tot.ptr, s);
     uint64_t t = 0;
     SPINED_HIP(hipMemcpyAsync(&t, tot.ptr, 8, hipMemcpyDeviceToHost, s));
     SPINED_HIP(hipStreamSynchronize(s));

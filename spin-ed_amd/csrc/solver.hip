@@ -361,27 +361,48 @@ __global__ void __launch_bounds__(256)
 
 constexpr int kMaxKeepOne = 32; // kept Ritz vectors of the one-vector restart
 
-// In place, row-local: V_i[r] <- sum_q S[i*m + q] V_q[r] for i < keep <= 32.
-// The outputs live in registers (unrolled, guarded), the inputs stream once.
-template <class VT>
+// In place, row-local: V_i[r] <- sum_q S[i*m + q] V_q[r] for i < keep <= KB
+// (KB = 8, 16 or 32).  The coefficients are staged once per workgroup in LDS
+// as [q][i] (zero for i >= keep), so each input value meets its KB
+// coefficients in broadcast LDS reads; the outputs live in registers and the
+// inputs stream once.  Each output sums in q order, as a plain loop would.
+template <class VT, int KB>
 __global__ void __launch_bounds__(kBlock)
     restart_kernel(VT *const *__restrict__ V, int m, int keep, const double *__restrict__ S,
                    uint64_t n) {
+  __shared__ double sS[KB * kMaxBasis];
+  for (int t = threadIdx.x; t < KB * m; t += kBlock) {
+    const int q = t / KB, i = t % KB;
+    sS[t] = i < keep ? S[i * m + q] : 0.0;
+  }
+  __syncthreads();
   for (uint64_t r = blockIdx.x * static_cast<uint64_t>(kBlock) + threadIdx.x; r < n;
        r += static_cast<uint64_t>(gridDim.x) * kBlock) {
-    double out[kMaxKeepOne];
+    double out[KB];
 #pragma unroll
-    for (int i = 0; i < kMaxKeepOne; ++i) out[i] = 0.0;
+    for (int i = 0; i < KB; ++i) out[i] = 0.0;
     for (int q = 0; q < m; ++q) {
       const double x = to_d(V[q][r]);
+      const double *c = sS + q * KB;
 #pragma unroll
-      for (int i = 0; i < kMaxKeepOne; ++i)
-        if (i < keep) out[i] = fma(S[i * m + q], x, out[i]);
+      for (int i = 0; i < KB; ++i) out[i] = fma(c[i], x, out[i]);
     }
 #pragma unroll
-    for (int i = 0; i < kMaxKeepOne; ++i)
+    for (int i = 0; i < KB; ++i)
       if (i < keep) V[i][r] = static_cast<VT>(out[i]);
   }
+}
+
+template <class VT>
+void launch_restart(VT *const *V, int m, int keep, const double *S, uint64_t n, unsigned G,
+                    hipStream_t s) {
+  if (keep <= 8)
+    restart_kernel<VT, 8><<<G, kBlock, 0, s>>>(V, m, keep, S, n);
+  else if (keep <= 16)
+    restart_kernel<VT, 16><<<G, kBlock, 0, s>>>(V, m, keep, S, n);
+  else
+    restart_kernel<VT, 32><<<G, kBlock, 0, s>>>(V, m, keep, S, n);
+  SPINED_CHECK_LAUNCH();
 }
 
 // out[r] = sum_q S[col*m + q] V_q[r] (FP64 Ritz vector)
@@ -739,8 +760,7 @@ template <class VT, bool C = false> struct Phase {
         }
       }
       // thick restart: keep the lowest `keep` Ritz vectors and v_m
-      restart_kernel<VT><<<G, kBlock, 0, s>>>(vptr.ptr, m, keep, S.ptr, ne);
-      SPINED_CHECK_LAUNCH();
+      launch_restart<VT>(vptr.ptr, m, keep, S.ptr, ne, G, s);
       SPINED_HIP(hipMemcpyAsync(vecs[keep].ptr, vecs[m].ptr, ne * sizeof(VT),
                                 hipMemcpyDeviceToDevice, s));
       reset_projected_kernel<<<1, 256, 0, s>>>(T.ptr, kMaxBasis, theta.ptr, keep);

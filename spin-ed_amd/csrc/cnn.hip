@@ -318,15 +318,6 @@ __global__ void __launch_bounds__(64)
 //    runs as soon as layer 2 at p is done.
 constexpr int kRing = 8;
 
-__device__ inline int pattern_at(uint64_t state, int n, int pos) {
-  // bit k: spin at pos + k - 2 (cyclic), for a window inside [0, n)
-  int start = pos - 2;
-  start = start < 0 ? start + n : start;
-  start = start >= n ? start - n : start;
-  const uint64_t rot = start == 0 ? state : ((state >> start) | (state << (n - start)));
-  return static_cast<int>(rot & 31u);
-}
-
 __global__ void __launch_bounds__(64)
     cnn_infer_kernel(CnnArgs c, const uint64_t *__restrict__ states, float *__restrict__ out_a,
                      float *__restrict__ out_t) {
@@ -359,13 +350,23 @@ __global__ void __launch_bounds__(64)
     b3[r] = P[OFF_B3 + 4 * g + r];
   }
   // B fragment q (rows k = 32 q + 8 g .. + 7: offset d = k >> 4, channels
-  // c0 = k & 15) of a conv at position pos
-  auto frag2 = [&](int pos, int q) {
+  // c0 = k & 15) of a conv at position pos.  Layer 2's fragment reads the
+  // layer-1 table at the 5-spin pattern around p = pos + d - 2, i.e. bits
+  // p - 2 .. p + 2 (cyclic) of the state: win[q] holds the state rotated
+  // right by pos + d - 4 (mod n, exactly n bits), advanced by one bit per
+  // position, so the pattern is its low five bits.
+  const uint64_t nmask = n == 64 ? ~0ull : ((1ull << n) - 1);
+  uint64_t win[3];
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    const int d = (32 * q + 8 * g) >> 4;
+    const int st = d - 4 < 0 ? d - 4 + n : d - 4;
+    win[q] = st == 0 ? state : (((state >> st) | (state << (n - st))) & nmask);
+  }
+  auto frag2 = [&](int q) {
     const int k0 = 32 * q + 8 * g, d = k0 >> 4, c0 = k0 & 15;
     if (d >= KW) return zero8();
-    int p = pos + d - 2;
-    p = p < 0 ? p + n : (p >= n ? p - n : p);
-    return *reinterpret_cast<const bf16x8 *>(tab + pattern_at(state, n, p) * CH + c0);
+    return *reinterpret_cast<const bf16x8 *>(tab + static_cast<int>(win[q] & 31u) * CH + c0);
   };
   auto frag3 = [&](int pos, int q, bool tail) {
     const int k0 = 32 * q + 8 * g, d = k0 >> 4, c0 = k0 & 15;
@@ -387,7 +388,9 @@ __global__ void __launch_bounds__(64)
   for (int pos = 0; pos < n; ++pos) {
     floatx4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-    for (int k = 0; k < 3; ++k) acc = mfma(a2[k], frag2(pos, k), acc);
+    for (int k = 0; k < 3; ++k) acc = mfma(a2[k], frag2(k), acc);
+#pragma unroll
+    for (int q = 0; q < 3; ++q) win[q] = (win[q] >> 1) | ((win[q] & 1u) << (n - 1));
     bf16x4 o;
 #pragma unroll
     for (int r = 0; r < 4; ++r) o[r] = static_cast<__bf16>(gelu_f(acc[r] + b2[r]));

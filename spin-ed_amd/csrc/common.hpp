@@ -42,12 +42,14 @@ hipStream_t stream();
 int current_device();
 
 // Pool of large device blocks (at least kPoolMinBytes).  The eigensolver's
-// vectors (6.9 GB each at D = 8.6e8, dozens per solve) go back to the pool
-// instead of to hipFree, which unmaps them page by page, and the next
-// allocation of the same size on the device takes them again.  Every library
-// kernel runs on the device's library stream, so a reused block is only
-// touched after the work that used it before.  An allocation that the
-// driver refuses returns the pooled blocks of the device and retries once.
+// vectors (6.9 GB each at D = 8.6e8, dozens per solve) and the entry blocks
+// of dropped gather caches go back to the pool instead of to hipFree, which
+// unmaps them page by page, and the next allocation of the same size on the
+// device takes them again.  Every library kernel runs on the device's
+// library stream, so a reused block is only touched after the work that
+// used it before.  An allocation that the driver refuses drops the gather
+// caches not in use (into the pool), returns the pooled blocks of the
+// device and retries once.
 constexpr size_t kPoolMinBytes = size_t(256) << 20;
 void *device_alloc(size_t bytes, bool pooled); // throws Error(LS_OUT_OF_MEMORY)
 void device_free(void *p, size_t bytes, bool pooled, int device);

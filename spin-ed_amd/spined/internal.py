@@ -341,8 +341,9 @@ def synchronize() -> None:
 
 
 def release_device_pool() -> None:
-    """Return the eigensolver's pooled vector blocks to the driver (they are
-    otherwise kept for the next solve of the same size)."""
+    """Return the pooled device blocks to the driver: the eigensolver's vector
+    blocks (otherwise kept for the next solve of the same size) and the entry
+    blocks of dropped gather caches (kept for the operator's next build)."""
     check_status(_lib.load().spined_device_release_pool())
 
 

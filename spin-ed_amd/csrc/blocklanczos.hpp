@@ -213,30 +213,37 @@ __global__ void store_block_column_kernel(const double *__restrict__ C1,
 }
 
 // In place, row-local restart: the first p = keepb*K vectors become
-// Y_i = sum_{q < M} S[i*M + q] V_q (vector q = row element q%K of block q/K).
-template <class VT, int K>
+// Y_i = sum_{q < M} S[i*M + q] V_q (vector q = row element q%K of block q/K),
+// p <= KP (8, 16 or 32).  The coefficients are staged once per workgroup in
+// LDS as [q][i] (zero for i >= p); each output sums in q order.
+template <class VT, int K, int KP>
 __global__ void __launch_bounds__(kBlock)
     block_restart_kernel(VT *const *__restrict__ V, int nblk, int keepb,
                          const double *__restrict__ S, uint64_t n) {
   const int M = nblk * K, p = keepb * K;
+  __shared__ double sS[KP * kMaxBasis];
+  for (int t = threadIdx.x; t < KP * M; t += kBlock) {
+    const int q = t / KP, i = t % KP;
+    sS[t] = i < p ? S[i * M + q] : 0.0;
+  }
+  __syncthreads();
   for (uint64_t r = blockIdx.x * static_cast<uint64_t>(kBlock) + threadIdx.x; r < n;
        r += static_cast<uint64_t>(gridDim.x) * kBlock) {
-    double out[kMaxKeep];
+    double out[KP];
 #pragma unroll
-    for (int i = 0; i < kMaxKeep; ++i) out[i] = 0.0;
+    for (int i = 0; i < KP; ++i) out[i] = 0.0;
     for (int b = 0; b < nblk; ++b) {
       const RowK<VT, K> v = reinterpret_cast<const RowK<VT, K> *>(V[b])[r];
 #pragma unroll
       for (int a = 0; a < K; ++a) {
-        const int q = b * K + a;
         const double x = to_d(v.v[a]);
+        const double *c = sS + (b * K + a) * KP;
 #pragma unroll
-        for (int i = 0; i < kMaxKeep; ++i)
-          if (i < p) out[i] = fma(S[i * M + q], x, out[i]);
+        for (int i = 0; i < KP; ++i) out[i] = fma(c[i], x, out[i]);
       }
     }
 #pragma unroll
-    for (int ob = 0; ob < kMaxKeep / K; ++ob) {
+    for (int ob = 0; ob < KP / K; ++ob) {
       if (ob < keepb) {
         RowK<VT, K> o;
 #pragma unroll
@@ -245,6 +252,19 @@ __global__ void __launch_bounds__(kBlock)
       }
     }
   }
+}
+
+template <class VT, int K>
+void launch_block_restart(VT *const *V, int nblk, int keepb, const double *S, uint64_t n,
+                          unsigned G, hipStream_t s) {
+  const int p = keepb * K;
+  if (p <= 8)
+    block_restart_kernel<VT, K, (K > 8 ? K : 8)><<<G, kBlock, 0, s>>>(V, nblk, keepb, S, n);
+  else if (p <= 16)
+    block_restart_kernel<VT, K, (K > 16 ? K : 16)><<<G, kBlock, 0, s>>>(V, nblk, keepb, S, n);
+  else
+    block_restart_kernel<VT, K, kMaxKeep><<<G, kBlock, 0, s>>>(V, nblk, keepb, S, n);
+  SPINED_CHECK_LAUNCH();
 }
 
 // out[r] = sum_q s[q] V_q[r] over M = nblk*K block vectors (FP64)
@@ -494,8 +514,7 @@ template <class VT, int K> struct BlockPhase {
         }
       }
       // thick restart: keep the lowest keepb*K Ritz vectors, then the last Q
-      block_restart_kernel<VT, K><<<Gd, kBlock, 0, s>>>(vptr.ptr, mb, keepb, S.ptr, n);
-      SPINED_CHECK_LAUNCH();
+      launch_block_restart<VT, K>(vptr.ptr, mb, keepb, S.ptr, n, Gd, s);
       std::swap(hptr[keepb], hptr[mb]);
       upload_pointers();
       reset_projected_kernel<<<1, 256, 0, s>>>(T.ptr, kMaxBasis, theta.ptr, keepb * K);

@@ -151,7 +151,7 @@ int spined_operator_set_orbit_method(ls_operator *op, int method);
 int spined_operator_set_schedule(ls_operator *op, int mode);
 int spined_operator_get_schedule(const ls_operator *op);
 /* Gather cache of a real operator: the target of every term of every row
- * (4 bytes per term: 71 GB at D = 8.6e8), written once by the orbit and
+ * (4 bytes per term, padded per wave: ~74 GB at D = 8.6e8), written once by the orbit and
  * index kernels so that later real single-vector matvecs read it instead of
  * recomputing orbits and searching the index; the sums are identical.
  * mode: -1 automatic (built at a real matvec of a basis of at least 2^20

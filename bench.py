@@ -65,6 +65,9 @@ def parse():
                    help="JSON with measured FETCH/WRITE bytes per matvec launch")
     p.add_argument("--copy-calibration-bytes", type=int, default=0,
                    help="run the plain copy kernel once on this many bytes (counter calibration)")
+    p.add_argument("--gather-calibration-lines", type=int, default=0,
+                   help="run the random-gather calibration kernel over this many 128-byte lines "
+                        "(a power of two), once reading 4 bytes and once the whole line of each")
     p.add_argument("--matvecs-only", type=int, default=0,
                    help="profiling mode: run this many FP64 matvecs instead of solves")
     p.add_argument("--block", type=int, default=1, choices=(1, 2, 4, 8),
@@ -138,6 +141,17 @@ def main():
         dst_b = DeviceArray(a.copy_calibration_bytes // 8, np.float64)
         spined.check_status(spined._lib.load().spined_debug_copy_device(
             dst_b.ptr, src_b.ptr, a.copy_calibration_bytes))
+        src_b.free()
+        dst_b.free()
+
+    if a.gather_calibration_lines:
+        L = spined._lib.load()
+        nl = a.gather_calibration_lines
+        src_b = DeviceArray.zeros(nl * 32, np.uint32)
+        nw = L.spined_debug_gather_out_words()
+        dst_b = DeviceArray(nw, np.uint32)
+        for words in (1, 32):
+            spined.check_status(L.spined_debug_gather_device(dst_b.ptr, nw, src_b.ptr, nl, words))
         src_b.free()
         dst_b.free()
 
@@ -244,28 +258,26 @@ def main():
     mv_seconds = sum(r.stats["seconds_matvec"] for r in results)
     mv_avg = mv_seconds / max(matvecs, 1)
 
-    sx = 4 if a.precision == "mixed" else 8
-    # the FP64 phase dominates the count in fp64 mode; in mixed mode weigh phases
-    mixed_mv = sum(r.stats["matvecs_mixed"] for r in results)
-    fp64_mv = matvecs - mixed_mv
-    # per row: representative, stabiliser, own x, y; per off-diagonal term:
-    # the x gather, plus the 4-byte cache entry when the gather cache is used
-    ent = 4 if cached else 0
-    bytes_fp64 = D * (8 + 4 + 8 + 8) + e_off * (8 + ent)
-    bytes_mixed = D * (8 + 4 + 4 + 8) + e_off * (4 + ent)
-    alg_bytes = (bytes_fp64 * fp64_mv + bytes_mixed * mixed_mv) / max(matvecs, 1)
-    achieved = alg_bytes / mv_avg if mv_avg > 0 else 0.0
+    # launches by x type: FP32 x (the FP32-storage phase) and FP64 x (the FP64
+    # phase and the FP64 anchor vector of the FP32-storage phase)
+    anchor_mv = sum(r.stats["matvecs_anchor"] for r in results)
+    anchor_s = sum(r.stats["seconds_matvec_anchor"] for r in results)
+    n32 = sum(r.stats["matvecs_mixed"] for r in results) - anchor_mv
+    t32 = sum(r.stats["seconds_matvec_mixed"] for r in results) - anchor_s
+    n64 = matvecs - n32
+    t64 = mv_seconds - t32
+    roof = roofline(D, e_off, n64, t64, n32, t32, cached)
     # HBM bytes per matvec launch from the committed rocprofv3 FETCH_SIZE /
-    # WRITE_SIZE passes (calibrated against the plain copy kernel), weighted
-    # by this run's FP64 / FP32-storage matvec counts
-    traffic = None
+    # WRITE_SIZE passes over the same kernels (scaled by the copy-kernel
+    # calibration), weighted by this run's FP64-x / FP32-x launch counts
     tj = Path(a.traffic_json) if a.traffic_json else ROOT / "profiles" / f"matvec_traffic_{a.config}.json"
     if tj.exists():
         tdat = json.loads(tj.read_text())
         per = {k: v["fetch_bytes"] + v["write_bytes"] for k, v in tdat.get("per_launch", {}).items()}
         kf, km = ("cached_fp64", "cached_mixed") if cached else ("fp64", "mixed")
-        if kf in per and km in per:
-            traffic = (per[kf] * fp64_mv + per[km] * mixed_mv) / max(matvecs, 1)
+        if kf in per and (km in per or n32 == 0):
+            roof["traffic"] = (per[kf] * n64 + per.get(km, 0.0) * n32) / max(n64 + n32, 1)
+            roof["traffic_source"] = str(tj.relative_to(ROOT))
 
     # comparison solves (outside the timed region): plain FP64 from a random
     # start, and the other storage mode from the same learned start
@@ -343,21 +355,50 @@ def main():
         "result": {"E0": float(last.evals[0]), "residual": float(last.rnorms[0]),
                    "E0_error_vs_bethe": (float(last.evals[0]) - e_exact) if e_exact else None,
                    "matvecs_per_solve": matvecs / len(results),
-                   "matvecs_fp32_phase": mixed_mv / len(results),
+                   "matvecs_fp32_x": n32 / len(results),
+                   "matvecs_fp64_x": n64 / len(results),
+                   "matvecs_fp64_anchor": anchor_mv / len(results),
                    "start_vector_s": last.stats["seconds_start"],
                    "gather_cache": cache_info,
                    "matvec_ms": mv_avg * 1e3,
                    "matvec_rows_per_s": D / mv_avg if mv_avg > 0 else None,
                    "matvec_fraction_of_solve": mv_seconds / elapsed},
-        "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": PEAK_HBM / 1e9,
-                     "unit": "GB/s", "frac": achieved / PEAK_HBM, "traffic": traffic,
-                     "kernel": "matvec_cached_kernel" if cached else "matvec_real_kernel",
-                     "algorithmic_bytes_per_launch": alg_bytes,
-                     "offdiag_gathers_per_launch": e_off},
+        "roofline": roof,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
     return 0
+
+
+def roofline(D, e_off, n64, t64, n32, t32, cached):
+    """Roofline of the matvec by its algorithmic bytes per launch,
+    B = D (8 + 4 + s_x + s_y) + E_off s_x: representative (8), stored
+    stabiliser size (4), own x and y element, one x element per
+    off-diagonal gather; s_y = 8 (y is FP64 in both modes), s_x = 8 or 4.
+    The gather cache's 4-byte entries are index traffic, not algorithmic
+    bytes: they are reported apart (index_bytes_per_launch), with the
+    fraction they would add."""
+    b64 = D * (8 + 4 + 8 + 8) + e_off * 8
+    b32 = D * (8 + 4 + 4 + 8) + e_off * 4
+    idx = e_off * 4 if cached else 0
+    tot_b = b64 * n64 + b32 * n32
+    tot_t = t64 + t32
+    achieved = tot_b / tot_t if tot_t > 0 else 0.0
+    per = {}
+    for tag, b, nl, t in (("fp64_x", b64, n64, t64), ("fp32_x", b32, n32, t32)):
+        if nl and t > 0:
+            ms = t / nl * 1e3
+            per[tag] = {"launches": nl, "ms_per_launch": ms, "algorithmic_bytes": b,
+                        "achieved_GBps": b / (ms * 1e-3) / 1e9,
+                        "frac": b / (ms * 1e-3) / PEAK_HBM,
+                        "frac_with_index_bytes": (b + idx) / (ms * 1e-3) / PEAK_HBM}
+    return {"bound": "hbm", "achieved": achieved / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
+            "frac": achieved / PEAK_HBM, "traffic": None,
+            "kernel": "matvec_cached_kernel" if cached else "matvec_real_kernel",
+            "algorithmic_bytes_per_launch": tot_b / max(n64 + n32, 1),
+            "index_bytes_per_launch": idx,
+            "offdiag_gathers_per_launch": e_off,
+            "per_kernel": per}
 
 
 def bench_batch(a, desc):

@@ -142,7 +142,8 @@ void ls_destroy_operator(ls_operator *op);
 int ls_operator_matmat(const ls_operator *op, int dtype, uint64_t size, uint64_t block_size,
                        const void *x, uint64_t x_stride, void *y, uint64_t y_stride);
 /* out[j] = <x_j|H|x_j> (complex double per column; not divided by <x_j|x_j>)
- * (expectation, Internal.hs:437-452). */
+ * (expectation, Internal.hs:437-452).  Each column is copied to the device
+ * once; H x_j and the FP64 dot are computed there and only out returns. */
 int ls_operator_expectation(const ls_operator *op, int dtype, uint64_t size, uint64_t block_size,
                             const void *x, uint64_t x_stride, void *out);
 bool ls_operator_is_real(const ls_operator *op);

@@ -76,6 +76,9 @@ typedef struct spined_stats {
   double seconds_total;
   double seconds_matvec;    /* wall time inside matvec kernels (events) */
   double seconds_start;     /* start-vector construction */
+  double seconds_matvec_mixed; /* of seconds_matvec: the FP32-x matvecs (FP32-storage phase) */
+  int64_t matvecs_anchor;   /* of matvecs_mixed: FP64-x matvecs of the FP64 anchor vector */
+  double seconds_matvec_anchor; /* of seconds_matvec_mixed: those anchor matvecs */
 } spined_stats;
 
 void spined_eigsh_options_init(spined_eigsh_options *opts);
@@ -124,6 +127,14 @@ int spined_basis_get_stabilizers(const ls_spin_basis *basis, uint32_t *out);
 int spined_operator_matmat_device(const ls_operator *op, int dtype, uint64_t size,
                                   uint64_t block_size, const void *x, uint64_t x_stride,
                                   void *y, uint64_t y_stride, int y_is_fp64);
+/* <x_c|H|x_c> for the block_size columns of x (DEVICE, column-major with
+ * leading dimension x_stride), not divided by <x_c|x_c>: the device form of
+ * ls_operator_expectation (src/SpinED/Internal.hs:437-452, used by
+ * src/SpinED.hs:296-313).  y_scratch: one DEVICE vector of `size` elements
+ * of dtype.  out: HOST, 2 * block_size doubles (re, im per column). */
+int spined_operator_expectation_device(const ls_operator *op, int dtype, uint64_t size,
+                                       uint64_t block_size, const void *x, uint64_t x_stride,
+                                       void *y_scratch, double *out);
 /* Y = H X for a row-interleaved block of block_size in {1, 2, 4, 8} real
  * vectors (element (row, c) at row * block_size + c; DEVICE pointers): one
  * orbit scan and index search per term serve the whole block, and every
@@ -272,6 +283,13 @@ int spined_debug_ring_available(const ls_spin_basis *basis);
 /* Plain device copy of `bytes` (multiple of 16): the calibration kernel for
  * HBM byte counters in profiles. */
 int spined_debug_copy_device(void *dst, const void *src, uint64_t bytes);
+/* Random-gather calibration for the same counters: reads `words` (1..32)
+ * 4-byte words of each of the n_lines (a power of two) 128-byte lines of
+ * src, every line once, in a scattered order; dst holds dst_words >=
+ * spined_debug_gather_out_words() words of per-wave sums. */
+int spined_debug_gather_device(void *dst, uint64_t dst_words, const void *src, uint64_t n_lines,
+                               int32_t words);
+uint64_t spined_debug_gather_out_words(void);
 
 #ifdef __cplusplus
 }

@@ -320,6 +320,10 @@ int ls_build_unsafe(ls_spin_basis *basis, uint64_t size, const uint64_t represen
     if (!basis || (size && !representatives)) throw Error(LS_INVALID_ARGUMENT, "NULL argument");
     Basis &b = *basis->p;
     std::lock_guard<std::mutex> lk(b.mu);
+    // operators bound to this basis keep row schedules, gather caches and
+    // index arguments sized for its current list: it cannot be replaced
+    if (b.built)
+      throw Error(LS_INVALID_ARGUMENT, "basis is already built; its representatives cannot be replaced");
     validate_representatives(b, size, representatives);
     ensure_basis_device(b);
     adopt_basis(b, size, representatives);
@@ -447,46 +451,53 @@ int ls_operator_matmat(const ls_operator *op, int dtype, uint64_t size, uint64_t
 
 int ls_operator_expectation(const ls_operator *op, int dtype, uint64_t size, uint64_t block_size,
                             const void *x, uint64_t x_stride, void *out) {
+  // <x_c|H|x_c> per column c (not divided by <x_c|x_c>).  Each column goes to
+  // the device once; H x_c and the FP64 dot stay there, and only the two
+  // doubles of the result come back.
   return guarded([&] {
     Operator &o = prepare_operator(op);
-    (void)vec_type(dtype);
+    const VecType vt = vec_type(dtype);
     if (size != o.basis->D) throw Error(LS_DIMENSION_MISMATCH, "size differs from dimension");
-    if (block_size && (!x || !out)) throw Error(LS_INVALID_ARGUMENT, "NULL argument");
+    if (block_size == 0) return;
+    if (!x || !out) throw Error(LS_INVALID_ARGUMENT, "NULL argument");
+    if (x_stride < size) throw Error(LS_INVALID_ARGUMENT, "stride smaller than size");
     const size_t es = elem_size(dtype);
-    std::vector<char> y(size * es);
-    double *res = static_cast<double *>(out);
+    hipStream_t s = stream();
+    DeviceBuffer<char> dx(std::max<uint64_t>(size, 1) * es), dy(std::max<uint64_t>(size, 1) * es);
+    const unsigned nb = vdot_blocks(size);
+    DeviceBuffer<double> partial(2 * static_cast<uint64_t>(nb)), res(2 * block_size);
     for (uint64_t c = 0; c < block_size; ++c) {
       const char *xc = static_cast<const char *>(x) + c * x_stride * es;
-      const int st = ls_operator_matmat(op, dtype, size, 1, xc, size, y.data(), size);
-      if (st) throw Error(st, "matvec failed");
-      double re = 0.0, im = 0.0;
-      for (uint64_t i = 0; i < size; ++i) {
-        switch (dtype) {
-        case LS_FLOAT32:
-          re += static_cast<double>(reinterpret_cast<const float *>(xc)[i]) *
-                reinterpret_cast<const float *>(y.data())[i];
-          break;
-        case LS_FLOAT64:
-          re += reinterpret_cast<const double *>(xc)[i] * reinterpret_cast<const double *>(y.data())[i];
-          break;
-        case LS_COMPLEX64: {
-          const float *a = reinterpret_cast<const float *>(xc) + 2 * i;
-          const float *b = reinterpret_cast<const float *>(y.data()) + 2 * i;
-          re += static_cast<double>(a[0]) * b[0] + static_cast<double>(a[1]) * b[1];
-          im += static_cast<double>(a[0]) * b[1] - static_cast<double>(a[1]) * b[0];
-          break;
-        }
-        default: {
-          const double *a = reinterpret_cast<const double *>(xc) + 2 * i;
-          const double *b = reinterpret_cast<const double *>(y.data()) + 2 * i;
-          re += a[0] * b[0] + a[1] * b[1];
-          im += a[0] * b[1] - a[1] * b[0];
-        }
-        }
-      }
-      res[2 * c] = re;
-      res[2 * c + 1] = im;
+      SPINED_HIP(hipMemcpyAsync(dx.ptr, xc, size * es, hipMemcpyHostToDevice, s));
+      launch_matvec(MatvecCall{&o, vt, vt, dx.ptr, dy.ptr, size, s});
+      launch_vdot(vt, dx.ptr, dy.ptr, size, partial.ptr, nb, res.ptr + 2 * c, s);
     }
+    SPINED_HIP(hipMemcpyAsync(out, res.ptr, 16 * block_size, hipMemcpyDeviceToHost, s));
+    SPINED_HIP(hipStreamSynchronize(s));
+  });
+}
+
+int spined_operator_expectation_device(const ls_operator *op, int dtype, uint64_t size,
+                                       uint64_t block_size, const void *x, uint64_t x_stride,
+                                       void *y_scratch, double *out) {
+  return guarded([&] {
+    Operator &o = prepare_operator(op);
+    const VecType vt = vec_type(dtype);
+    if (size != o.basis->D) throw Error(LS_DIMENSION_MISMATCH, "size differs from dimension");
+    if (block_size == 0) return;
+    if (!x || !out || !y_scratch) throw Error(LS_INVALID_ARGUMENT, "NULL argument");
+    if (x_stride < size) throw Error(LS_INVALID_ARGUMENT, "stride smaller than size");
+    const size_t es = elem_size(dtype);
+    hipStream_t s = stream();
+    const unsigned nb = vdot_blocks(size);
+    DeviceBuffer<double> partial(2 * static_cast<uint64_t>(nb)), res(2 * block_size);
+    for (uint64_t c = 0; c < block_size; ++c) {
+      const char *xc = static_cast<const char *>(x) + c * x_stride * es;
+      launch_matvec(MatvecCall{&o, vt, vt, xc, y_scratch, size, s});
+      launch_vdot(vt, xc, y_scratch, size, partial.ptr, nb, res.ptr + 2 * c, s);
+    }
+    SPINED_HIP(hipMemcpyAsync(out, res.ptr, 16 * block_size, hipMemcpyDeviceToHost, s));
+    SPINED_HIP(hipStreamSynchronize(s));
   });
 }
 
@@ -968,6 +979,21 @@ int spined_debug_copy_device(void *dst, const void *src, uint64_t bytes) {
     SPINED_HIP(hipStreamSynchronize(stream()));
   });
 }
+
+int spined_debug_gather_device(void *dst, uint64_t dst_words, const void *src, uint64_t n_lines,
+                               int32_t words) {
+  return guarded([&] {
+    if (!dst || !src || n_lines == 0 || (n_lines & (n_lines - 1)) || words < 1 || words > 32)
+      throw Error(LS_INVALID_ARGUMENT,
+                  "n_lines must be a power of two and words per line in [1, 32]");
+    if (dst_words < gather_calibration_out_words())
+      throw Error(LS_INVALID_ARGUMENT, "dst must hold spined_debug_gather_out_words() words");
+    launch_gather_calibration(src, dst, n_lines, words, stream());
+    SPINED_HIP(hipStreamSynchronize(stream()));
+  });
+}
+
+uint64_t spined_debug_gather_out_words(void) { return gather_calibration_out_words(); }
 
 int spined_debug_ring_available(const ls_spin_basis *basis) {
   return basis && basis->p->tables.ring_available ? 1 : 0;

@@ -9,6 +9,7 @@
 #include <complex>
 #include <cstdint>
 #include <memory>
+#include <atomic>
 #include <mutex>
 #include <vector>
 
@@ -190,6 +191,10 @@ void ensure_device(OperatorTables &t);
 // a stabiliser ratio other than 1 (bit 31, taken from stab[target]).
 struct GatherCache {
   std::mutex mu;
+  // set while the holder of mu allocates: a refused allocation reclaims
+  // caches from the same thread, which must pass this one by instead of
+  // locking mu a second time
+  std::atomic<bool> building{false};
   int request = -1;       // -1 automatic (built when it fits in free memory), 0 off, 1 on
   bool counted = false;   // woff and total known
   bool built = false;
@@ -251,6 +256,11 @@ struct MatvecCall {
   hipStream_t stream;
 };
 void launch_matvec(const MatvecCall &c);
+// out[0], out[1] (device) = re, im of sum_r conj(x_r) y_r in FP64, in a fixed
+// order; partial holds 2 * nb doubles, nb = vdot_blocks(n).
+unsigned vdot_blocks(uint64_t n);
+void launch_vdot(VecType t, const void *x, const void *y, uint64_t n, double *partial,
+                 unsigned nb, double *out, hipStream_t s);
 // Row-interleaved block of K in {1, 2, 4, 8} real vectors: x[row * K + c].
 struct MatvecBlockCall {
   const Operator *op;
@@ -286,6 +296,9 @@ int batch_eigsh(const std::vector<Operator *> &ops, double tol, int m_req, uint6
 // ---- misc kernels (solver.hip) ----------------------------------------------
 void launch_random_fill(double *v, uint64_t n, uint64_t seed, hipStream_t s);
 void launch_copy_calibration(const void *src, void *dst, uint64_t bytes, hipStream_t s);
+uint64_t gather_calibration_out_words();
+void launch_gather_calibration(const void *src, void *dst, uint64_t n_lines, int words,
+                               hipStream_t s);
 
 // ---- learned start (cnn.hip) ------------------------------------------------
 void cnn_start_vector(const spined_cnn *c, const Basis &b, uint64_t marshall_mask, double *v);

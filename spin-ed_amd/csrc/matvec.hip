@@ -577,6 +577,11 @@ bool ensure_cache(const Operator &op, const Ctx &c, GatherCache &gc, hipStream_t
   if (gc.built) return true;
   const Basis &b = *op.basis;
   if (gc.request == 0 || !op.tables.is_real || b.D >= kCacheIndex) return false;
+  struct Building {
+    std::atomic<bool> &f;
+    explicit Building(std::atomic<bool> &x) : f(x) { f.store(true); }
+    ~Building() { f.store(false); }
+  } building(gc.building);
   if (gc.request < 0 && b.D < kCacheMinRows) return false;
   const uint64_t waves = (b.D + kRowBlock - 1) / kRowBlock;
   const unsigned g = static_cast<unsigned>(waves);
@@ -855,7 +860,7 @@ void reclaim_gather_caches(int device) {
   for (Operator *op : g_ops) {
     if (!op->basis || op->basis->device != device) continue;
     GatherCache &gc = *op->cache;
-    if (!gc.mu.try_lock()) continue;
+    if (gc.building.load() || !gc.mu.try_lock()) continue;
     if (gc.built) {
       gc.built = false;
       gc.entries.release();
@@ -951,6 +956,110 @@ void ensure_schedule(Operator &op) {
   SPINED_HIP(hipStreamSynchronize(s));
   op.schedule_built = true;
 }
+
+namespace {
+// <x|y> = sum_r conj(x_r) y_r in FP64 (real types: im = 0).  One partial
+// (re, im) per workgroup, then one workgroup sums them in a fixed order.
+__device__ inline void load_pair(const float *v, uint64_t r, double &re, double &im) {
+  re = v[r];
+  im = 0.0;
+}
+__device__ inline void load_pair(const double *v, uint64_t r, double &re, double &im) {
+  re = v[r];
+  im = 0.0;
+}
+__device__ inline void load_pair(const float2 *v, uint64_t r, double &re, double &im) {
+  const float2 t = v[r];
+  re = t.x;
+  im = t.y;
+}
+__device__ inline void load_pair(const double2 *v, uint64_t r, double &re, double &im) {
+  const double2 t = v[r];
+  re = t.x;
+  im = t.y;
+}
+
+template <class T>
+__global__ void __launch_bounds__(kBlock)
+    vdot_partial_kernel(const T *__restrict__ x, const T *__restrict__ y, uint64_t n,
+                        double *__restrict__ partial) {
+  __shared__ double lds[2][kBlock / 64];
+  double re = 0.0, im = 0.0;
+  for (uint64_t r = blockIdx.x * static_cast<uint64_t>(kBlock) + threadIdx.x; r < n;
+       r += static_cast<uint64_t>(gridDim.x) * kBlock) {
+    double xr, xi, yr, yi;
+    load_pair(x, r, xr, xi);
+    load_pair(y, r, yr, yi);
+    re = fma(xr, yr, fma(xi, yi, re));
+    im = fma(xr, yi, fma(-xi, yr, im));
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    re += __shfl_down(re, off, 64);
+    im += __shfl_down(im, off, 64);
+  }
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) {
+    lds[0][wid] = re;
+    lds[1][wid] = im;
+  }
+  __syncthreads();
+  if (threadIdx.x < 2) {
+    double t = 0.0;
+    for (int k = 0; k < kBlock / 64; ++k) t += lds[threadIdx.x][k];
+    partial[2 * static_cast<uint64_t>(blockIdx.x) + threadIdx.x] = t;
+  }
+}
+
+__global__ void __launch_bounds__(kBlock)
+    vdot_reduce_kernel(const double *__restrict__ partial, unsigned nb, double *__restrict__ out) {
+  __shared__ double lds[2][kBlock];
+  double re = 0.0, im = 0.0;
+  for (unsigned b = threadIdx.x; b < nb; b += kBlock) {
+    re += partial[2 * b];
+    im += partial[2 * b + 1];
+  }
+  lds[0][threadIdx.x] = re;
+  lds[1][threadIdx.x] = im;
+  __syncthreads();
+  for (int off = kBlock / 2; off > 0; off >>= 1) {
+    if (threadIdx.x < static_cast<unsigned>(off)) {
+      lds[0][threadIdx.x] += lds[0][threadIdx.x + off];
+      lds[1][threadIdx.x] += lds[1][threadIdx.x + off];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    out[0] = lds[0][0];
+    out[1] = lds[1][0];
+  }
+}
+} // namespace
+
+void launch_vdot(VecType t, const void *x, const void *y, uint64_t n, double *partial,
+                 unsigned nb, double *out, hipStream_t s) {
+  switch (t) {
+  case VecType::F32:
+    vdot_partial_kernel<<<nb, kBlock, 0, s>>>(static_cast<const float *>(x),
+                                              static_cast<const float *>(y), n, partial);
+    break;
+  case VecType::F64:
+    vdot_partial_kernel<<<nb, kBlock, 0, s>>>(static_cast<const double *>(x),
+                                              static_cast<const double *>(y), n, partial);
+    break;
+  case VecType::C64:
+    vdot_partial_kernel<<<nb, kBlock, 0, s>>>(static_cast<const float2 *>(x),
+                                              static_cast<const float2 *>(y), n, partial);
+    break;
+  default:
+    vdot_partial_kernel<<<nb, kBlock, 0, s>>>(static_cast<const double2 *>(x),
+                                              static_cast<const double2 *>(y), n, partial);
+  }
+  SPINED_CHECK_LAUNCH();
+  vdot_reduce_kernel<<<1, kBlock, 0, s>>>(partial, nb, out);
+  SPINED_CHECK_LAUNCH();
+}
+
+unsigned vdot_blocks(uint64_t n) { return grid_for(n); }
 
 uint64_t count_offdiag_device(const Operator &op) {
   const Basis &b = *op.basis;

@@ -31,6 +31,7 @@ SPINED_SYMBOLS = (
     "spined_eigsh_options_init", "spined_eigsh", "spined_get_last_stats", "spined_stream",
     "spined_synchronize", "spined_basis_device_states", "spined_basis_get_stabilizers",
     "spined_operator_matmat_device", "spined_operator_matmat_rows_device",
+    "spined_operator_expectation_device",
     "spined_operator_count_offdiag", "spined_operator_orbit_method",
     "spined_operator_set_orbit_method", "spined_operator_set_schedule",
     "spined_operator_get_schedule", "spined_basis_set_bucket_bits",
@@ -44,6 +45,7 @@ SPINED_SYMBOLS = (
     "spined_cnn_create", "spined_cnn_destroy", "spined_cnn_get_params", "spined_cnn_set_params",
     "spined_cnn_forward_device", "spined_cnn_backward_device", "spined_cnn_adam_step",
     "spined_cnn_start_vector", "spined_debug_mfma_probe", "spined_debug_copy_device",
+    "spined_debug_gather_device", "spined_debug_gather_out_words",
     "spined_batch_eigsh", "spined_operator_autotune",
 )
 
@@ -97,6 +99,9 @@ class SpinedStats(C.Structure):
         ("seconds_total", C.c_double),
         ("seconds_matvec", C.c_double),
         ("seconds_start", C.c_double),
+        ("seconds_matvec_mixed", C.c_double),
+        ("matvecs_anchor", C.c_int64),
+        ("seconds_matvec_anchor", C.c_double),
     ]
 
 
@@ -158,6 +163,7 @@ def load():
         "spined_basis_get_stabilizers": (I, [P, P]),
         "spined_operator_matmat_device": (I, [P, I, U64, U64, P, U64, P, U64, I]),
         "spined_operator_matmat_rows_device": (I, [P, I, U64, U64, P, P, I]),
+        "spined_operator_expectation_device": (I, [P, I, U64, U64, P, U64, P, P]),
         "spined_operator_count_offdiag": (I, [P, C.POINTER(U64)]),
         "spined_operator_orbit_method": (I, [P]),
         "spined_operator_set_orbit_method": (I, [P, I]),
@@ -192,6 +198,8 @@ def load():
         "spined_cnn_start_vector": (I, [P, P, U64, P]),
         "spined_debug_mfma_probe": (I, [P]),
         "spined_debug_copy_device": (I, [P, P, U64]),
+        "spined_debug_gather_device": (I, [P, U64, P, U64, C.c_int32]),
+        "spined_debug_gather_out_words": (U64, []),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

@@ -32,7 +32,7 @@ from spined import config  # noqa: E402
 EXAMPLES = Path("/root/reference/example")
 DECKS = ["heisenberg_chain_4", "heisenberg_chain_10", "heisenberg_kagome_12",
          "heisenberg_square_4x4", "heisenberg_triangular_19", "xxz_triangular_19",
-         "heisenberg_square_5x5"]
+         "heisenberg_square_5x5", "heisenberg_pyrochlore_32"]
 OUT = Path(__file__).resolve().parent
 SAMPLE = 4096
 SEEDS = (20261202, 777)

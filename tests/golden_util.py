@@ -5,7 +5,9 @@ from pathlib import Path
 import numpy as np
 
 GOLDEN = Path(__file__).resolve().parent / "golden"
-NAMES = sorted(p.stem for p in GOLDEN.glob("*.npz"))
+# deck fixtures (make_golden.py); other fixtures (make_xxz512_golden.py)
+# hold no deck
+NAMES = sorted(p.stem for p in GOLDEN.glob("*.npz") if not p.stem.endswith("_e0"))
 
 
 def load(name):

@@ -12,6 +12,7 @@ import pytest
 import pyoracle
 import spined
 from spined import lattice
+from spined.internal import LatticeSymmetriesException
 from helpers import Spec, engine, oracle, ring
 
 pytestmark = pytest.mark.gpu
@@ -159,3 +160,61 @@ def test_build_unsafe_roundtrip():
     o = oracle(spec)
     x = np.random.default_rng(2).uniform(-1, 1, o.dimension)
     np.testing.assert_allclose(spined.apply(op2, x), o.matvec(x), rtol=0, atol=1e-12)
+
+
+def test_build_unsafe_refuses_a_built_basis():
+    """A built basis keeps its list: operators bound to it hold row schedules
+    and gather caches sized for it, so ls_build_unsafe on it is refused (and
+    ls_build stays a no-op), and the operator still matches the oracle."""
+    spec = ring(16)
+    basis, op = engine(spec)
+    o = oracle(spec)
+    x = np.random.default_rng(3).uniform(-1, 1, o.dimension)
+    y0 = spined.apply(op, x)
+    reps = spined.basis_get_states(basis)
+    with pytest.raises(LatticeSymmetriesException) as e:
+        spined.build_basis(basis, reps[: len(reps) // 2])
+    assert e.value.code == 9
+    spined.build_basis(basis)
+    np.testing.assert_array_equal(spined.basis_get_states(basis), reps)
+    np.testing.assert_array_equal(spined.apply(op, x), y0)
+    np.testing.assert_allclose(y0, o.matvec(x), rtol=0, atol=1e-12)
+
+
+@pytest.mark.parametrize("spec,dtype", [
+    (ring(16), np.float64), (ring(16), np.float32),
+    (ring(14, k=3, p=None, z=None, name="ring14_k3"), np.complex128),
+    (ring(14, k=3, p=None, z=None, name="ring14_k3"), np.complex64),
+], ids=["f64", "f32", "c128", "c64"])
+def test_expectation_on_device_matches_oracle(spec, dtype):
+    """ls_operator_expectation: <x_c|H|x_c> per column, H x_c and the FP64 dot
+    on the device (Internal.hs:437-452), against numpy's dot of the oracle's
+    H x_c; also the device-pointer entry point on the same columns."""
+    from spined import _lib
+    from spined.device import DeviceArray
+    _, op = engine(spec)
+    o = oracle(spec)
+    D = o.dimension
+    rng = np.random.default_rng(11)
+    cplx = np.dtype(dtype).kind == "c"
+    X = rng.uniform(-1, 1, (D, 3))
+    if cplx:
+        X = X + 1j * rng.uniform(-1, 1, (D, 3))
+    X = np.asfortranarray(X.astype(dtype))
+    got = spined.expectation(op, X)
+    tol = 1e-12 if np.dtype(dtype) in (np.float64, np.complex128) else 1e-5
+    ref = []
+    for c in range(3):
+        xc = X[:, c].astype(np.complex128 if cplx else np.float64)
+        ref.append(np.vdot(xc, o.matvec(xc)))
+    ref = np.array(ref)
+    scale = np.max(np.abs(ref))
+    assert np.max(np.abs(got - ref)) <= tol * scale, (got, ref)
+    code = {np.dtype(np.float32): 0, np.dtype(np.float64): 1,
+            np.dtype(np.complex64): 2, np.dtype(np.complex128): 3}[np.dtype(dtype)]
+    dx = DeviceArray.from_numpy(np.ascontiguousarray(X.T))
+    dy = DeviceArray((D,), dtype)
+    out = np.zeros(6)
+    spined.check_status(_lib.load().spined_operator_expectation_device(
+        op.ptr, code, D, 3, dx.ptr, D, dy.ptr, out.ctypes.data))
+    np.testing.assert_array_equal(out[0::2] + 1j * out[1::2], np.asarray(got, dtype=np.complex128))

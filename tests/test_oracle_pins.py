@@ -174,6 +174,38 @@ def projected_basis_vectors(n, w, z, gens, reps):
     return B
 
 
+def random_hermitian(k, seed, real=False, conserve_weight=False):
+    """Random Hermitian 2^k x 2^k matrix with no site-exchange symmetry;
+    with conserve_weight, zero between local states of different popcount."""
+    rng = np.random.default_rng(seed)
+    d = 1 << k
+    a = rng.normal(size=(d, d)) + (0 if real else 1j * rng.normal(size=(d, d)))
+    h = (a + a.conj().T) / 2
+    if conserve_weight:
+        pc = np.array([bin(i).count("1") for i in range(d)])
+        h = np.where(pc[:, None] == pc[None, :], h, 0)
+    return h
+
+
+def ring_tuples(n, k):
+    return [[(i + j) % n for j in range(k)] for i in range(n)]
+
+
+# 3- and 4-site terms (ls_create_interaction3/4): random Hermitian matrices
+# that no site permutation leaves alone, so the local-index bit order (the
+# first listed site is the most significant bit) is exercised; translation
+# invariant sets of tuples, so they commute with the ring's translations
+MULTI = [
+    (8, None, None, lattice.ring_symmetries(8, 0, None),
+     [(random_hermitian(3, 31, real=True), ring_tuples(8, 3))]),
+    (8, None, None, lattice.ring_symmetries(8, 3, None),                            # complex
+     [(random_hermitian(3, 32), ring_tuples(8, 3)), (random_hermitian(4, 41), ring_tuples(8, 4))]),
+    (9, 4, None, lattice.ring_symmetries(9, 0, None),
+     [(random_hermitian(4, 42, real=True, conserve_weight=True), ring_tuples(9, 4)),
+      (H2, lattice.ring_bonds(9))]),
+    (8, 4, None, [], [(random_hermitian(3, 33, conserve_weight=True), ring_tuples(8, 3))]),
+]
+
 DENSE = [
     (8, 4, 1, lattice.ring_symmetries(8, 0, 0), [(H2, lattice.ring_bonds(8))]),
     (8, 4, -1, lattice.ring_symmetries(8, 4, 1), [(H2, lattice.ring_bonds(8))]),
@@ -183,7 +215,7 @@ DENSE = [
     (9, None, None, [([1, 2, 0, 4, 5, 3, 7, 8, 6], 1)],     # H must commute with the group
      [(H2, [[0, 3], [1, 4], [2, 5], [3, 6], [4, 7], [5, 8], [0, 1], [1, 2], [2, 0]]),
       (np.array([[0.3, 1 - 0.5j], [1 + 0.5j, -0.2]]), [[0], [1], [2], [3], [4], [5], [6], [7], [8]])]),
-]
+] + MULTI
 
 
 @pytest.mark.parametrize("n,w,z,gens,terms", DENSE)

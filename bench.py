@@ -61,6 +61,8 @@ def parse():
     p.add_argument("--basis-size", type=int, default=0)
     p.add_argument("--cpu-sample-seconds", type=float, default=15.0)
     p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--no-batch", action="store_true",
+                   help="skip the 512-instance batched solve reported beside the headline")
     p.add_argument("--traffic-json", default=None,
                    help="JSON with measured FETCH/WRITE bytes per matvec launch")
     p.add_argument("--copy-calibration-bytes", type=int, default=0,
@@ -68,6 +70,9 @@ def parse():
     p.add_argument("--gather-calibration-lines", type=int, default=0,
                    help="run the random-gather calibration kernel over this many 128-byte lines "
                         "(a power of two), once reading 4 bytes and once the whole line of each")
+    p.add_argument("--start-vector-only", type=int, default=0,
+                   help="profiling mode: evaluate the CNN start vector over every representative "
+                        "this many times (untrained parameters) instead of solves")
     p.add_argument("--matvecs-only", type=int, default=0,
                    help="profiling mode: run this many FP64 matvecs instead of solves")
     p.add_argument("--block", type=int, default=1, choices=(1, 2, 4, 8),
@@ -130,7 +135,8 @@ def main():
     e_off = spined.count_offdiag(op)
     log(f"off-diagonal gathers per matvec: {e_off}")
     tune = None
-    if not a.no_autotune and not a.matvecs_only and a.bucket_bits < 0 and a.index == "auto":
+    if (not a.no_autotune and not a.matvecs_only and not a.start_vector_only
+            and a.bucket_bits < 0 and a.index == "auto"):
         t = time.perf_counter()
         tune = spined.autotune(op)
         tune["seconds"] = round(time.perf_counter() - t, 2)
@@ -154,6 +160,21 @@ def main():
             spined.check_status(L.spined_debug_gather_device(dst_b.ptr, nw, src_b.ptr, nl, words))
         src_b.free()
         dst_b.free()
+
+    if a.start_vector_only:
+        from spined import learned
+        model = learned.CNN(seed=1)
+        v = model.start_vector_device(basis, n)
+        device_synchronize()
+        t = time.perf_counter()
+        for _ in range(a.start_vector_only):
+            v.free()
+            v = model.start_vector_device(basis, n)
+        device_synchronize()
+        dt = (time.perf_counter() - t) / a.start_vector_only
+        print(json.dumps({"mode": "start-vector-only", "config": desc, "dimension": D,
+                          "evaluations": a.start_vector_only, "s_per_evaluation": dt}))
+        return 0
 
     # one untimed matvec warms the kernels even when warmup = 0
     x = DeviceArray.from_numpy(np.ones(D))
@@ -258,12 +279,9 @@ def main():
     mv_seconds = sum(r.stats["seconds_matvec"] for r in results)
     mv_avg = mv_seconds / max(matvecs, 1)
 
-    # launches by x type: FP32 x (the FP32-storage phase) and FP64 x (the FP64
-    # phase and the FP64 anchor vector of the FP32-storage phase)
-    anchor_mv = sum(r.stats["matvecs_anchor"] for r in results)
-    anchor_s = sum(r.stats["seconds_matvec_anchor"] for r in results)
-    n32 = sum(r.stats["matvecs_mixed"] for r in results) - anchor_mv
-    t32 = sum(r.stats["seconds_matvec_mixed"] for r in results) - anchor_s
+    # launches by x type: FP32 x (the FP32-storage phase) and FP64 x
+    n32 = sum(r.stats["matvecs_mixed"] for r in results)
+    t32 = sum(r.stats["seconds_matvec_mixed"] for r in results)
     n64 = matvecs - n32
     t64 = mv_seconds - t32
     roof = roofline(D, e_off, n64, t64, n32, t32, cached)
@@ -333,6 +351,18 @@ def main():
         cpu = cpu_baseline(n, basis, gens, lattice, a.cpu_sample_seconds, D,
                            matvecs / len(results))
 
+    # the batched-solver workload beside the headline (its own JSON object):
+    # the ring's device memory goes back first
+    batch_res = None
+    if not a.no_batch:
+        spined.set_gather_cache(op, 0)
+        del op, basis
+        import gc
+        gc.collect()
+        spined.release_device_pool()
+        batch_res = run_batch(a, CONFIGS["xxz512"][1], steps=1, warmup=0, with_cpu=not a.no_cpu)
+        log(f"xxz512: {batch_res['value']:.1f} instances/s, launch {batch_res['result']['launch_s']:.2f} s")
+
     e_exact = {40: -70.986091153332861, 24: -42.680058066148944}.get(n)
     out = {
         "metric": "s to E0 (residual<=1e-10), matvec HBM GB/s; mixed+learned vs plain FP64, 1 GPU",
@@ -357,7 +387,6 @@ def main():
                    "matvecs_per_solve": matvecs / len(results),
                    "matvecs_fp32_x": n32 / len(results),
                    "matvecs_fp64_x": n64 / len(results),
-                   "matvecs_fp64_anchor": anchor_mv / len(results),
                    "start_vector_s": last.stats["seconds_start"],
                    "gather_cache": cache_info,
                    "matvec_ms": mv_avg * 1e3,
@@ -365,6 +394,7 @@ def main():
                    "matvec_fraction_of_solve": mv_seconds / elapsed},
         "roofline": roof,
         "cpu_baseline": cpu,
+        "batch_xxz512": batch_res,
     }
     print(json.dumps(out))
     return 0
@@ -401,8 +431,37 @@ def roofline(D, e_off, n64, t64, n32, t32, cached):
             "per_kernel": per}
 
 
-def bench_batch(a, desc):
-    """Config 4: all 512 instances per step, timed as one batched solve."""
+def batch_roofline(sysb, res, m=20):
+    """Roofline of the persistent batched launch by a model of its
+    algorithmic bytes: per matvec of instance i, D_i (8 + 4 + 8 + 8) +
+    E_i 8 (representative, stabiliser, own x, y, one x per off-diagonal term;
+    E_i counted on the GPU per ring size), plus per Lanczos step the
+    Gram-Schmidt and store passes, (2 j + 7) 8 D_i at the mean basis size
+    j = (keep + m - 1) / 2, and per restart cycle (m + keep + 2) 8 D_i;
+    divided by the launch time (HIP events around the one launch)."""
+    from spined import internal as I
+    keep = m // 2
+    e_off = {}
+    for x, op in zip(sysb.instances, sysb.ops):
+        if x["n"] not in e_off:
+            e_off[x["n"]] = I.count_offdiag(op)
+    dims = sysb.dimensions
+    tot = 0.0
+    for x, D, mv, cy in zip(sysb.instances, dims, res["matvecs"], res["cycles"]):
+        D = float(D)
+        jbar = (keep + m - 1) / 2.0
+        per_mv = D * 28 + e_off[x["n"]] * 8 + (2 * jbar + 7) * 8 * D
+        tot += mv * per_mv + max(int(cy), 1) * (m + keep + 2) * 8 * D
+    t = res["stats"]["seconds_cycles"]
+    ach = tot / t if t > 0 else 0.0
+    return {"bound": "hbm", "achieved": ach / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
+            "frac": ach / PEAK_HBM, "traffic": None, "kernel": "batch_persistent_kernel",
+            "algorithmic_bytes_per_launch": tot, "launch_s": t,
+            "offdiag_terms_per_matvec_by_size": {str(k): v for k, v in sorted(e_off.items())}}
+
+
+def run_batch(a, desc, steps=1, warmup=0, with_cpu=True):
+    """Config 4: all 512 instances per step, one persistent launch per step."""
     from spined import batch
     from spined.device import device_synchronize
     t = time.perf_counter()
@@ -412,8 +471,6 @@ def bench_batch(a, desc):
     t_setup = time.perf_counter() - t
     dims = sysb.dimensions
     log(f"xxz512: {len(inst)} instances, sum of dimensions {int(dims.sum())}, setup {t_setup:.2f} s")
-    steps = a.steps if a.steps is not None else 1
-    warmup = a.warmup if a.warmup is not None else 0
     for _ in range(warmup):
         batch.batch_eigsh(sysb.ops, tol=1e-9)
     device_synchronize()
@@ -426,9 +483,9 @@ def bench_batch(a, desc):
     per = (time.perf_counter() - t0) / steps
     sec = res["seconds"]
     cpu = None
-    if not a.no_cpu:
+    if with_cpu:
         cpu = batch_cpu_baseline(inst, dims, a.cpu_sample_seconds, per, len(inst))
-    out = {
+    return {
         "metric": "instances/s to residual<=1e-9 (batched uneven XXZ solves), 1 GPU",
         "value": len(inst) / per, "unit": "instances/s", "n_gpus": 1, "steps": steps,
         "warmup": warmup, "ms_per_step": per * 1e3, "higher_is_better": True, "scaling": "weak",
@@ -437,14 +494,20 @@ def bench_batch(a, desc):
                    "setup_s": round(t_setup, 2)},
         "result": {"converged": int(res["stats"]["converged"]),
                    "max_residual": float(np.max(res["rnorms"])),
-                   "cycles": int(res["stats"]["cycles"]),
+                   "cycles_max": int(res["stats"]["cycles"]),
+                   "launch_s": float(res["stats"]["seconds_cycles"]),
                    "p50_instance_s": float(np.percentile(sec, 50)),
                    "p99_instance_s": float(np.percentile(sec, 99)),
                    "max_matvecs_per_instance": int(res["stats"]["matvecs_per_instance_max"])},
-        "roofline": None,
+        "roofline": batch_roofline(sysb, res),
         "cpu_baseline": cpu,
     }
-    print(json.dumps(out))
+
+
+def bench_batch(a, desc):
+    steps = a.steps if a.steps is not None else 1
+    warmup = a.warmup if a.warmup is not None else 0
+    print(json.dumps(run_batch(a, desc, steps, warmup, not a.no_cpu)))
     return 0
 
 

@@ -77,8 +77,6 @@ typedef struct spined_stats {
   double seconds_matvec;    /* wall time inside matvec kernels (events) */
   double seconds_start;     /* start-vector construction */
   double seconds_matvec_mixed; /* of seconds_matvec: the FP32-x matvecs (FP32-storage phase) */
-  int64_t matvecs_anchor;   /* of matvecs_mixed: FP64-x matvecs of the FP64 anchor vector */
-  double seconds_matvec_anchor; /* of seconds_matvec_mixed: those anchor matvecs */
 } spined_stats;
 
 void spined_eigsh_options_init(spined_eigsh_options *opts);
@@ -216,24 +214,29 @@ int spined_basis_sector_dimension(const ls_spin_basis *basis, double *out);
 
 /* ---- batched solver (no counterpart in the reference) ------------------ */
 /* Lowest eigenpair of `count` independent real exchange-bond operators
- * (instances may have different sizes and may share a basis), solved
- * together: every Lanczos step is one launch per kernel over all instances'
- * row tiles, one restart cycle is a replayed hipGraph, converged instances
- * drop out.  Replaces one SpinED run per Hamiltonian. */
+ * (instances may have different sizes and may share a basis), solved in ONE
+ * persistent launch: the instances, largest first, are split into team
+ * slots (one workgroup each, team size growing with the dimension) that the
+ * launch's workgroups take from a device counter; each team runs its
+ * instance's thick-restart Lanczos (FP64) with per-instance barriers, and no
+ * host round trip happens until every instance is done.  Replaces one SpinED
+ * run per Hamiltonian. */
 typedef struct spined_batch_stats {
-  int32_t cycles;                   /* restart cycles run */
+  int32_t cycles;                   /* restart cycles of the slowest instance */
   int32_t converged;                /* instances with residual <= tol */
-  double seconds_total;             /* setup + solve + finalisation */
-  double seconds_cycles;            /* graph replays only */
+  double seconds_total;             /* setup + solve + read-back */
+  double seconds_cycles;            /* the persistent launch (HIP events) */
   uint64_t total_rows;              /* sum of the instances' dimensions */
   int64_t matvecs_per_instance_max; /* matvecs of the slowest instance */
 } spined_batch_stats;
-/* evals, rnorms: [count]; cycles[i]: restart cycle at which instance i
- * converged (-1: not converged); seconds[i]: time from the first replay to
- * the end of that cycle.  max_basis_size <= 32 (0: 20). */
+/* evals, rnorms: [count]; cycles[i]: restart cycles instance i ran (-1: not
+ * converged); seconds[i]: time from the launch to the end of instance i;
+ * matvecs[i]: its matvecs (cycles, seconds, matvecs may be NULL).
+ * max_basis_size <= 24 (0: 20). */
 int spined_batch_eigsh(const ls_operator *const ops[], uint64_t count, double tol,
                        int32_t max_basis_size, uint64_t seed, int32_t max_cycles, double *evals,
-                       double *rnorms, int32_t *cycles, double *seconds, spined_batch_stats *stats);
+                       double *rnorms, int32_t *cycles, double *seconds, int64_t *matvecs,
+                       spined_batch_stats *stats);
 
 /* ---- learned start vector (no counterpart in the reference) ------------ */
 /* Periodic 1D CNN over +-1 spins: 3 circular conv layers (16 channels,

@@ -527,18 +527,19 @@ int spined_eigsh(const ls_operator *op, int dtype, uint64_t k, double tol,
 
 int spined_batch_eigsh(const ls_operator *const ops[], uint64_t count, double tol,
                        int32_t max_basis_size, uint64_t seed, int32_t max_cycles, double *evals,
-                       double *rnorms, int32_t *cycles, double *seconds,
+                       double *rnorms, int32_t *cycles, double *seconds, int64_t *matvecs,
                        spined_batch_stats *stats) {
   int status = LS_SUCCESS;
   const int g = guarded([&] {
     if (count && (!ops || !evals || !rnorms)) throw Error(LS_INVALID_ARGUMENT, "NULL argument");
-    if (max_basis_size < 0 || max_basis_size > 32)
-      throw Error(LS_INVALID_ARGUMENT, "max_basis_size must be in [0, 32]");
+    if (max_basis_size < 0 || max_basis_size > 24)
+      throw Error(LS_INVALID_ARGUMENT, "max_basis_size must be in [0, 24]");
     std::vector<Operator *> v;
     v.reserve(count);
     for (uint64_t i = 0; i < count; ++i) v.push_back(&prepare_operator(ops[i]));
     status = batch_eigsh(v, tol > 0.0 ? tol : 1e-9, max_basis_size, seed,
-                         max_cycles > 0 ? max_cycles : 200, evals, rnorms, cycles, seconds, stats);
+                         max_cycles > 0 ? max_cycles : 200, evals, rnorms, cycles, seconds,
+                         matvecs, stats);
   });
   return g ? g : status;
 }

@@ -1,17 +1,42 @@
-// batch.hip -- many independent, uneven ground-state solves at once.
+// batch.hip -- many independent, uneven ground-state solves in ONE persistent
+// launch (spined_batch_eigsh).
 //
-// Every instance (an ls_operator on a built basis; instances may share a
-// basis) gets its own thick-restart Lanczos state on the device.  All
-// instances advance in lockstep through Lanczos steps; each step is a fixed
-// sequence of batched kernels whose grid covers (instance, 256-row tile)
-// pairs of ALL instances, so one launch serves every Hamiltonian regardless
-// of its dimension.  One restart cycle (steps, projected Jacobi solves,
-// convergence test, restart) is captured once as a hipGraph and replayed
-// until every instance has converged; converged instances drop out (their
-// tiles return immediately).  The host reads one counter per cycle.
+// Work queue.  The instances, largest dimension first, are expanded into
+// slots (instance, member): one slot per workgroup of the instance's team,
+// whose size grows with the dimension (one workgroup for the small rings, up
+// to kMaxTeam for the largest).  Every workgroup of the launch takes the next
+// slot from a device counter, solves its share of that instance together
+// with its teammates, and takes the next slot when the instance is done; the
+// launch ends when the slots run out.  Nothing returns to the host between
+// the launch and the end of the last instance.
+//
+// A team runs the one-vector thick-restart Lanczos of its instance in FP64
+// (the recurrence of the single solver: classical Gram-Schmidt against every
+// basis vector, a second pass only when the first loses more than two digits
+// (Daniel, Gragg, Kaufman & Stewart), Jacobi on the projected matrix, restart
+// on the lowest Ritz vectors).  Member k owns row tiles k, k + t, k + 2t, ...
+// of every vector: matvec rows, dot partials, updates and restarts touch only
+// its own rows (the matvec gathers every row of its input).  Each member
+// writes one partial per dot; after a team barrier every member sums the t
+// partials in member order and runs the same Jacobi sweep on the same
+// projected matrix in its own LDS, so all members take the same decisions
+// without further messages, and the sums do not depend on timing.  The
+// explicit residual |H y - lambda y| / |y| ends the solve; when it is above
+// tol the iteration continues with a tighter estimate target.
+//
+// Barriers are per instance (arrival counter and generation) with release /
+// acquire at agent scope, so rows written on one XCD are read correctly on
+// another.  Deadlock freedom: a member waits only for teammates whose slots
+// come later in the queue; they are taken as soon as any earlier instance
+// finishes, and the first team is complete because the largest team fits in
+// the resident workgroups (checked on the host).  Every wait is bounded: a
+// member that waits longer than kWaitLimitTicks sets the abort flag, every
+// waiting member leaves on it, and the call reports a device error.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
+#include <numeric>
 #include <vector>
 
 #include "engine.hpp"
@@ -23,13 +48,18 @@ namespace spined {
 
 namespace {
 
-constexpr int kTile = 256;
-constexpr int kMaxM = 32;       // basis size limit of the batched solver
-constexpr int kPS = 40;         // partial-sum slots per tile (>= kMaxM + 1)
-constexpr int kChunk = 8;
+constexpr int kTile = 256;      // rows per tile = threads per workgroup
+constexpr int kMaxM = 24;       // basis size limit of the batched solver
+constexpr int kPS = kMaxM + 4;  // partial slots per member and reduction
+constexpr int kChunk = 8;       // dots per pass
+constexpr int kMaxKeep = kMaxM / 2; // kept Ritz vectors (keep = m / 2)
+constexpr int kMaxTeam = 32;
+constexpr uint64_t kRowsPerMember = 131072;
+// s_memrealtime runs at 100 MHz: 60 s
+constexpr uint64_t kWaitLimitTicks = 6000000000ull;
+constexpr double kEta = 1e-2; // second Gram-Schmidt pass below this |w'| / |w|
 
-struct BatchInst {
-  // operator / basis (read uniformly by a workgroup)
+struct PInst {
   OpArgs o;
   BondArgs bd;
   RatioArgs rt;
@@ -38,327 +68,423 @@ struct BatchInst {
   IndexArgs ix;
   const uint32_t *stab;
   uint64_t D;
-  // solver state
-  double *V;     // (m + 1) vectors of D doubles, contiguous
-  double *w;     // D
-  double *T;     // kMaxM x kMaxM
-  double *theta; // kMaxM
-  double *S;     // kMaxM x kMaxM
-  double *beta;  // kMaxM + 1 (the last slot normalises the start)
-  double *h1;    // kMaxM + 1
-  double *h2;    // kMaxM + 1
-  double *scal;  // 4 scalars: lambda, |y|^2, residual^2, spare
-  int *flag;     // 0 active, > 0: cycle at which it converged
-  uint32_t tile_begin, tile_end;
+  double *V;      // (m + 2) vectors of D doubles: basis 0..m, slot m + 1 = H y
+  double *w;      // D
+  double *pm;     // 2 x kMaxTeam x kPS member partials (double buffered)
+  unsigned *bar;  // [0] arrivals, [1] generation
+  double *out;    // [0] lambda [1] residual [2] cycles [3] matvecs [4] finish ticks [5] status
+  uint32_t team;
+  uint32_t ntiles;
 };
 
-struct Tile {
+struct Slot {
   uint32_t inst;
-  uint32_t pad;
-  uint64_t row0;
+  uint32_t member;
 };
 
-__device__ inline bool tile_row(const BatchInst *insts, const Tile *tiles, const BatchInst *&I,
-                                uint64_t &r) {
-  const Tile t = tiles[blockIdx.x];
-  I = insts + t.inst;
-  r = t.row0 + threadIdx.x;
-  return *I->flag == 0 && r < I->D;
+__device__ inline uint64_t now_ticks() { return wall_clock64(); }
+
+// Team barrier: everything this workgroup wrote before it is visible to the
+// teammates after it, on any XCD.  False when the wait was abandoned (abort).
+__device__ bool team_barrier(unsigned *bar, uint32_t t, int *abort) {
+  __shared__ int s_ok;
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int ok = 1;
+    if (t > 1) {
+      const unsigned g = __hip_atomic_load(bar + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      const unsigned a = __hip_atomic_fetch_add(bar, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+      if (a == t - 1) {
+        __hip_atomic_store(bar, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(bar + 1, g + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      } else {
+        const uint64_t t0 = now_ticks();
+        while (__hip_atomic_load(bar + 1, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == g) {
+          if (__hip_atomic_load(abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+            ok = 0;
+            break;
+          }
+          if (now_ticks() - t0 > kWaitLimitTicks) {
+            __hip_atomic_store(abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            ok = 0;
+            break;
+          }
+          __builtin_amdgcn_s_sleep(8);
+        }
+      }
+    }
+    s_ok = ok;
+  }
+  __syncthreads();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  return s_ok != 0;
+}
+
+// Workgroup sums of acc[0..nq) (wave shuffles, then the 4 waves in order)
+// into dst[q] (thread q writes).
+template <int NQ>
+__device__ inline void block_sums(const double (&acc)[NQ], int nq, double *__restrict__ dst) {
+  __shared__ double lds[NQ][kTile / 64];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) {
+    if (q < nq) {
+      double v = acc[q];
+      for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+      if (lane == 0) lds[q][wid] = v;
+    }
+  }
+  __syncthreads();
+  if (static_cast<int>(threadIdx.x) < nq) {
+    double s = 0.0;
+    for (int k = 0; k < kTile / 64; ++k) s += lds[threadIdx.x][k];
+    dst[threadIdx.x] = s;
+  }
+  __syncthreads();
+}
+
+struct Team {
+  const PInst &I;
+  uint32_t k, t;
+  int *abort;
+  int red = 0; // which half of pm the next reduction writes
+  __device__ Team(const PInst &inst, uint32_t member, int *ab)
+      : I(inst), k(member), t(inst.team), abort(ab) {}
+  __device__ double *pm_mine() { return I.pm + (static_cast<uint64_t>(red) * kMaxTeam + k) * kPS; }
+  // after the member partials are written: barrier, then out[q] = sum over
+  // members in order (every member the same); false on abort
+  __device__ bool reduce(int nq, double *out) {
+    if (!team_barrier(I.bar, t, abort)) return false;
+    const double *base = I.pm + static_cast<uint64_t>(red) * kMaxTeam * kPS;
+    if (static_cast<int>(threadIdx.x) < nq) {
+      double s = 0.0;
+      for (uint32_t mm = 0; mm < t; ++mm) s += base[mm * kPS + threadIdx.x];
+      out[threadIdx.x] = s;
+    }
+    red ^= 1;
+    __syncthreads();
+    return true;
+  }
+  __device__ bool sync() { return team_barrier(I.bar, t, abort); }
+};
+
+template <int METHOD, int NW>
+__device__ inline double matvec_row(const PInst &I, uint64_t r, const double *x) {
+  return gather_row_real<METHOD, NW, true>(I.o, I.bd, I.rt, I.ga, I.ra, I.ix, I.stab, r, x);
+}
+
+// partials of V_q . w for q in [0, nv) and (with ww) w . w in slot nv, over
+// the member's rows, into the member's partial slots
+__device__ void dots_pass(Team &tm, int nv, bool ww) {
+  const PInst &I = tm.I;
+  double *dst = tm.pm_mine();
+  const int total = nv + (ww ? 1 : 0);
+  for (int q0 = 0; q0 < total; q0 += kChunk) {
+    double acc[kChunk];
+#pragma unroll
+    for (int c = 0; c < kChunk; ++c) acc[c] = 0.0;
+    for (uint32_t tile = tm.k; tile < I.ntiles; tile += tm.t) {
+      const uint64_t r = static_cast<uint64_t>(tile) * kTile + threadIdx.x;
+      if (r >= I.D) continue;
+      const double wr = I.w[r];
+#pragma unroll
+      for (int c = 0; c < kChunk; ++c) {
+        const int q = q0 + c;
+        if (q < nv)
+          acc[c] = fma(I.V[static_cast<uint64_t>(q) * I.D + r], wr, acc[c]);
+        else if (q == nv && ww)
+          acc[c] = fma(wr, wr, acc[c]);
+      }
+    }
+    block_sums<kChunk>(acc, std::min(kChunk, total - q0), dst + q0);
+  }
+}
+
+// w -= sum_{q < nv} h[q] V_q over the member's rows; then the partials of
+// V_q . w (q < nv) when dots, else of w . w in slot 0
+__device__ void update_pass(Team &tm, int nv, const double *h, bool dots) {
+  const PInst &I = tm.I;
+  double acc[1] = {0.0};
+  for (uint32_t tile = tm.k; tile < I.ntiles; tile += tm.t) {
+    const uint64_t r = static_cast<uint64_t>(tile) * kTile + threadIdx.x;
+    if (r >= I.D) continue;
+    double s = 0.0;
+    for (int q = 0; q < nv; ++q) s = fma(h[q], I.V[static_cast<uint64_t>(q) * I.D + r], s);
+    const double wr = I.w[r] - s;
+    I.w[r] = wr;
+    acc[0] = fma(wr, wr, acc[0]);
+  }
+  if (dots)
+    dots_pass(tm, nv, false);
+  else
+    block_sums<1>(acc, 1, tm.pm_mine());
+}
+
+// v[r] = (w[r] - sum_{q < nv} h[q] V_q[r]) * inv over the member's rows
+// (the update and the store of the next basis vector in one pass)
+__device__ void update_store_pass(Team &tm, int nv, const double *h, double inv, double *v) {
+  const PInst &I = tm.I;
+  for (uint32_t tile = tm.k; tile < I.ntiles; tile += tm.t) {
+    const uint64_t r = static_cast<uint64_t>(tile) * kTile + threadIdx.x;
+    if (r >= I.D) continue;
+    double s = 0.0;
+    for (int q = 0; q < nv; ++q) s = fma(h[q], I.V[static_cast<uint64_t>(q) * I.D + r], s);
+    v[r] = (I.w[r] - s) * inv;
+  }
+}
+
+template <int METHOD, int NW>
+__device__ void solve_instance(const PInst &I, uint32_t inst_id, uint32_t member, int m, int keep,
+                               double tol, int max_cycles, uint64_t seed, int *abort,
+                               const uint64_t *t_start) {
+  __shared__ double sT[kMaxM * kMaxM];
+  __shared__ double sS[kMaxM * kMaxM];
+  __shared__ double stheta[kMaxM];
+  __shared__ double sbeta[kMaxM + 1];
+  __shared__ double sh1[kPS], sh2[kPS], sred[kPS];
+  __shared__ double sC[kMaxM * kMaxKeep]; // restart coefficients [q][i]
+  Team tm(I, member, abort);
+  const uint64_t D = I.D;
+  const int mm = static_cast<int>(std::min<uint64_t>(static_cast<uint64_t>(m), D));
+  const int kp = std::max(1, std::min({keep, mm - 1, kMaxKeep}));
+  double *V = I.V;
+  auto vec = [&](int q) { return V + static_cast<uint64_t>(q) * D; };
+  int64_t matvecs = 0;
+
+  // start vector: w = hash(seed + instance, r), v_0 = w / |w|
+  {
+    double acc[1] = {0.0};
+    for (uint32_t tile = member; tile < I.ntiles; tile += tm.t) {
+      const uint64_t r = static_cast<uint64_t>(tile) * kTile + threadIdx.x;
+      if (r >= D) continue;
+      const double v = hash_uniform(seed + inst_id, r);
+      I.w[r] = v;
+      acc[0] = fma(v, v, acc[0]);
+    }
+    block_sums<1>(acc, 1, tm.pm_mine());
+    if (!tm.reduce(1, sred)) return;
+    const double inv = 1.0 / sqrt(sred[0]);
+    for (uint32_t tile = member; tile < I.ntiles; tile += tm.t) {
+      const uint64_t r = static_cast<uint64_t>(tile) * kTile + threadIdx.x;
+      if (r < D) vec(0)[r] = I.w[r] * inv;
+    }
+    if (!tm.sync()) return;
+  }
+  for (int e = threadIdx.x; e < kMaxM * kMaxM; e += kTile) sT[e] = 0.0;
+  __syncthreads();
+
+  int j0 = 0, cycle = 0, msz = mm;
+  double est_target = tol;
+  double lambda = 0.0, resid = 0.0;
+  bool done = false;
+  while (!done) {
+    ++cycle;
+    msz = mm;
+    for (int j = j0; j < mm; ++j) {
+      // w = H v_j (own rows); dots with V_0..V_j and w.w
+      const double *x = vec(j);
+      for (uint32_t tile = member; tile < I.ntiles; tile += tm.t) {
+        const uint64_t r = static_cast<uint64_t>(tile) * kTile + threadIdx.x;
+        if (r < D) I.w[r] = matvec_row<METHOD, NW>(I, r, x);
+      }
+      ++matvecs;
+      dots_pass(tm, j + 1, true);
+      if (!tm.reduce(j + 2, sh1)) return;
+      double ss = 0.0;
+      for (int q = 0; q <= j; ++q) ss = fma(sh1[q], sh1[q], ss);
+      const double ww = sh1[j + 1];
+      // one Gram-Schmidt pass suffices when it keeps at least kEta of |w|;
+      // then |w'|^2 = w.w - |h|^2 (Pythagoras, relative error below
+      // eps / kEta^2) and the update stores v_{j+1} directly
+      const bool skip = ww > 0.0 && ww - ss >= kEta * kEta * ww;
+      double bj = 0.0;
+      if (skip) {
+        bj = sqrt(ww - ss);
+        if (threadIdx.x <= static_cast<unsigned>(j)) sh2[threadIdx.x] = 0.0;
+      } else {
+        update_pass(tm, j + 1, sh1, true);
+        if (!tm.reduce(j + 1, sh2)) return;
+        update_pass(tm, j + 1, sh2, false);
+        if (!tm.reduce(1, sred)) return;
+        bj = sqrt(std::max(sred[0], 0.0));
+      }
+      if (threadIdx.x <= static_cast<unsigned>(j)) {
+        const double v = sh1[threadIdx.x] + sh2[threadIdx.x];
+        sT[threadIdx.x * kMaxM + j] = v;
+        sT[j * kMaxM + threadIdx.x] = v;
+      }
+      if (threadIdx.x == 0) sbeta[j] = bj;
+      __syncthreads();
+      // invariant subspace: the cycle ends here (exact Ritz values)
+      double scale = 0.0;
+      for (int q = 0; q <= j; ++q) scale = std::max(scale, fabs(sT[q * kMaxM + q]));
+      if (bj <= 1e-14 * std::max(scale, 1.0)) {
+        msz = j + 1;
+        if (threadIdx.x == 0) sbeta[j] = 0.0;
+        __syncthreads();
+        break;
+      }
+      const double inv = 1.0 / bj;
+      double *vn = vec(j + 1);
+      if (skip) {
+        update_store_pass(tm, j + 1, sh1, inv, vn);
+      } else {
+        for (uint32_t tile = member; tile < I.ntiles; tile += tm.t) {
+          const uint64_t r = static_cast<uint64_t>(tile) * kTile + threadIdx.x;
+          if (r < D) vn[r] = I.w[r] * inv;
+        }
+      }
+      if (!tm.sync()) return;
+    }
+    // projected problem (every member the same)
+    jacobi_eigh_device<kMaxM>(sT, kMaxM, msz, stheta, sS, 40);
+    __syncthreads();
+    const double est = fabs(sbeta[msz - 1] * sS[msz - 1]);
+    const bool exhausted = msz < mm || static_cast<uint64_t>(mm) >= D;
+    if (est <= est_target || exhausted || cycle >= max_cycles) {
+      // Ritz vector y = V S[:, 0] into w, H y into V_{m+1}
+      for (uint32_t tile = member; tile < I.ntiles; tile += tm.t) {
+        const uint64_t r = static_cast<uint64_t>(tile) * kTile + threadIdx.x;
+        if (r >= D) continue;
+        double acc = 0.0;
+        for (int q = 0; q < msz; ++q) acc = fma(sS[q], vec(q)[r], acc);
+        I.w[r] = acc;
+      }
+      if (!tm.sync()) return;
+      double *hy = vec(m + 1);
+      double acc2[2] = {0.0, 0.0};
+      for (uint32_t tile = member; tile < I.ntiles; tile += tm.t) {
+        const uint64_t r = static_cast<uint64_t>(tile) * kTile + threadIdx.x;
+        if (r >= D) continue;
+        const double h = matvec_row<METHOD, NW>(I, r, I.w);
+        hy[r] = h;
+        const double y = I.w[r];
+        acc2[0] = fma(y, h, acc2[0]);
+        acc2[1] = fma(y, y, acc2[1]);
+      }
+      ++matvecs;
+      block_sums<2>(acc2, 2, tm.pm_mine());
+      if (!tm.reduce(2, sred)) return;
+      const double yy = sred[1];
+      lambda = sred[0] / yy;
+      double acc[1] = {0.0};
+      for (uint32_t tile = member; tile < I.ntiles; tile += tm.t) {
+        const uint64_t r = static_cast<uint64_t>(tile) * kTile + threadIdx.x;
+        if (r >= D) continue;
+        const double d = fma(-lambda, I.w[r], hy[r]);
+        acc[0] = fma(d, d, acc[0]);
+      }
+      block_sums<1>(acc, 1, tm.pm_mine());
+      if (!tm.reduce(1, sred)) return;
+      resid = sqrt(std::max(sred[0], 0.0) / yy);
+      if (resid <= tol || exhausted || cycle >= max_cycles) {
+        done = true;
+        break;
+      }
+      // the estimate was optimistic: aim lower and go on
+      est_target = std::max(0.25 * est_target, 1e-3 * tol);
+    }
+    // thick restart: V_i <- sum_q S[i][q] V_q (i < kp), V_kp <- V_msz
+    for (int e = threadIdx.x; e < msz * kMaxKeep; e += kTile) {
+      const int q = e / kMaxKeep, i = e % kMaxKeep;
+      sC[e] = i < kp ? sS[i * msz + q] : 0.0;
+    }
+    __syncthreads();
+    for (uint32_t tile = member; tile < I.ntiles; tile += tm.t) {
+      const uint64_t r = static_cast<uint64_t>(tile) * kTile + threadIdx.x;
+      if (r >= D) continue;
+      // each input read once; the kept outputs in registers, written after
+      // the whole row is read (in place)
+      double out[kMaxKeep];
+#pragma unroll
+      for (int i = 0; i < kMaxKeep; ++i) out[i] = 0.0;
+      for (int q = 0; q < msz; ++q) {
+        const double x = vec(q)[r];
+#pragma unroll
+        for (int i = 0; i < kMaxKeep; ++i) out[i] = fma(sC[q * kMaxKeep + i], x, out[i]);
+      }
+      const double last = vec(msz)[r];
+#pragma unroll
+      for (int i = 0; i < kMaxKeep; ++i)
+        if (i < kp) vec(i)[r] = out[i];
+      vec(kp)[r] = last;
+    }
+    for (int e = threadIdx.x; e < kMaxM * kMaxM; e += kTile) {
+      const int i = e / kMaxM, jj = e % kMaxM;
+      sT[e] = (i == jj && i < kp) ? stheta[i] : 0.0;
+    }
+    if (!tm.sync()) return;
+    j0 = kp;
+  }
+  if (member == 0 && threadIdx.x == 0) {
+    I.out[0] = lambda;
+    I.out[1] = resid;
+    I.out[2] = cycle;
+    I.out[3] = static_cast<double>(matvecs);
+    I.out[4] = static_cast<double>(now_ticks() - *t_start);
+    I.out[5] = resid <= tol ? 1.0 : 2.0;
+  }
 }
 
 template <int METHOD, int NW>
 __global__ void __launch_bounds__(kTile)
-    batch_matvec_kernel(const BatchInst *insts, const Tile *tiles, int src, int dst, int final_pass) {
-  const BatchInst *I;
-  uint64_t r;
-  const Tile t = tiles[blockIdx.x];
-  I = insts + t.inst;
-  r = t.row0 + threadIdx.x;
-  if (r >= I->D) return;
-  const int f = *I->flag;
-  if (final_pass ? f == 0 : f != 0) return;
-  // src / dst: vector slot (0..m) or -1 for w
-  const double *x = src < 0 ? I->w : I->V + static_cast<uint64_t>(src) * I->D;
-  double *y = dst < 0 ? I->w : I->V + static_cast<uint64_t>(dst) * I->D;
-  y[r] = gather_row_real<METHOD, NW, true>(I->o, I->bd, I->rt, I->ga, I->ra, I->ix, I->stab, r, x);
-}
-
-// partial[tile * kPS + q] = sum over the tile's rows of V_q . w (q < nvec),
-// or, with mode 1, w . w in slot 0.
-__global__ void __launch_bounds__(kTile)
-    batch_multidot_kernel(const BatchInst *insts, const Tile *tiles, int nvec, int mode,
-                          double *partial) {
-  __shared__ double lds[kChunk][kTile / 64];
-  const Tile t = tiles[blockIdx.x];
-  const BatchInst *I = insts + t.inst;
-  const int c = blockIdx.y;
-  const int q0 = c * kChunk;
-  const uint64_t r = t.row0 + threadIdx.x;
-  const bool active = *I->flag == 0;
-  double acc[kChunk];
-#pragma unroll
-  for (int q = 0; q < kChunk; ++q) acc[q] = 0.0;
-  if (active && r < I->D) {
-    const double wr = I->w[r];
-    if (mode == 1) {
-      acc[0] = wr * wr;
-    } else {
-#pragma unroll
-      for (int q = 0; q < kChunk; ++q)
-        if (q0 + q < nvec) acc[q] = I->V[static_cast<uint64_t>(q0 + q) * I->D + r] * wr;
-    }
-  }
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-#pragma unroll
-  for (int q = 0; q < kChunk; ++q) {
-    double v = acc[q];
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-    if (lane == 0) lds[q][wid] = v;
-  }
-  __syncthreads();
-  if (threadIdx.x < kChunk && q0 + static_cast<int>(threadIdx.x) < (mode == 1 ? 1 : nvec)) {
-    double s = 0.0;
-    for (int k = 0; k < kTile / 64; ++k) s += lds[threadIdx.x][k];
-    partial[static_cast<uint64_t>(blockIdx.x) * kPS + q0 + threadIdx.x] = s;
-  }
-}
-
-// Per instance (one block each): out[q] = sum of its tiles' partials
-// (fixed order).  target 0: h1, 1: h2, 2: beta[j] = sqrt(sum).
-__global__ void __launch_bounds__(256)
-    batch_reduce_kernel(const BatchInst *insts, const double *partial, int nvec, int target, int j) {
-  const BatchInst *I = insts + blockIdx.x;
-  if (*I->flag != 0) return;
-  __shared__ double lds[256];
-  for (int q = 0; q < nvec; ++q) {
-    double s = 0.0;
-    for (uint32_t t = I->tile_begin + threadIdx.x; t < I->tile_end; t += 256)
-      s += partial[static_cast<uint64_t>(t) * kPS + q];
-    lds[threadIdx.x] = s;
+    batch_persistent_kernel(const PInst *__restrict__ insts, const Slot *__restrict__ slots,
+                            uint32_t nslots, unsigned *ticket, int m, int keep, double tol,
+                            int max_cycles, uint64_t seed, int *abort, uint64_t *t_start) {
+  __shared__ uint32_t s_slot;
+  if (threadIdx.x == 0) atomicMin(reinterpret_cast<unsigned long long *>(t_start),
+                                  static_cast<unsigned long long>(now_ticks()));
+  for (;;) {
+    if (threadIdx.x == 0)
+      s_slot = __hip_atomic_load(abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                   ? nslots
+                   : atomicAdd(ticket, 1u);
     __syncthreads();
-    for (int off = 128; off > 0; off >>= 1) {
-      if (threadIdx.x < static_cast<unsigned>(off)) lds[threadIdx.x] += lds[threadIdx.x + off];
-      __syncthreads();
-    }
-    if (threadIdx.x == 0) {
-      if (target == 0) I->h1[q] = lds[0];
-      else if (target == 1) I->h2[q] = lds[0];
-      else I->beta[j] = sqrt(lds[0]);
-    }
+    // wave-uniform (scalar) slot, so the instance's arguments load into
+    // scalar registers
+    const uint32_t sl = __builtin_amdgcn_readfirstlane(s_slot);
     __syncthreads();
+    if (sl >= nslots) return;
+    const uint32_t inst = __builtin_amdgcn_readfirstlane(slots[sl].inst);
+    const uint32_t member = __builtin_amdgcn_readfirstlane(slots[sl].member);
+    solve_instance<METHOD, NW>(insts[inst], inst, member, m, keep, tol, max_cycles, seed, abort,
+                               t_start);
   }
 }
 
-// w -= sum_q h V_q  (h = h1 or h2)
-__global__ void __launch_bounds__(kTile)
-    batch_gemv_sub_kernel(const BatchInst *insts, const Tile *tiles, int nvec, int which) {
-  const BatchInst *I;
-  uint64_t r;
-  if (!tile_row(insts, tiles, I, r)) return;
-  const double *h = which == 0 ? I->h1 : I->h2;
-  double acc = 0.0;
-  for (int q = 0; q < nvec; ++q) acc = fma(h[q], I->V[static_cast<uint64_t>(q) * I->D + r], acc);
-  I->w[r] -= acc;
-}
-
-__global__ void batch_store_column_kernel(const BatchInst *insts, int j) {
-  const BatchInst *I = insts + blockIdx.x;
-  if (*I->flag != 0) return;
-  const int i = threadIdx.x;
-  if (i <= j) {
-    const double v = I->h1[i] + I->h2[i];
-    I->T[i * kMaxM + j] = v;
-    I->T[j * kMaxM + i] = v;
+__global__ void batch_init_kernel(unsigned *ticket, int *abort, uint64_t *t_start, unsigned *bars,
+                                  uint64_t nbar) {
+  const uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+  if (i == 0) {
+    *ticket = 0;
+    *abort = 0;
+    *t_start = ~0ull;
   }
+  if (i < nbar) bars[i] = 0;
 }
 
-// V_dst = w / beta[bidx]
-__global__ void __launch_bounds__(kTile)
-    batch_scale_store_kernel(const BatchInst *insts, const Tile *tiles, int bidx, int dst) {
-  const BatchInst *I;
-  uint64_t r;
-  if (!tile_row(insts, tiles, I, r)) return;
-  const double b = I->beta[bidx];
-  const double inv = b > 1e-300 ? 1.0 / b : 0.0;
-  I->V[static_cast<uint64_t>(dst) * I->D + r] = I->w[r] * inv;
+template <int METHOD, int NW>
+int resident_blocks() {
+  int per_cu = 0;
+  SPINED_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+      &per_cu, reinterpret_cast<const void *>(&batch_persistent_kernel<METHOD, NW>), kTile, 0));
+  int dev = 0, cus = 0;
+  SPINED_HIP(hipGetDevice(&dev));
+  SPINED_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  return std::max(1, per_cu) * std::max(1, cus);
 }
-
-__global__ void __launch_bounds__(256) batch_jacobi_kernel(const BatchInst *insts, int m) {
-  const BatchInst *I = insts + blockIdx.x;
-  if (*I->flag != 0) return;
-  const int mm = static_cast<uint64_t>(m) < I->D ? m : static_cast<int>(I->D);
-  jacobi_eigh_device(I->T, kMaxM, mm, I->theta, I->S, 40);
-}
-
-// Converged when |beta_{m-1} S[m-1, 0]| <= tol (or the Krylov space is
-// exhausted); the flag records the cycle number.
-__global__ void batch_check_kernel(const BatchInst *insts, int n_inst, int m, double tol,
-                                   const int *cycle_counter, int *remaining) {
-  const int cycle = *cycle_counter;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n_inst) return;
-  const BatchInst *I = insts + i;
-  if (*I->flag != 0) return;
-  const int mm = static_cast<uint64_t>(m) < I->D ? m : static_cast<int>(I->D);
-  const double est = fabs(I->beta[mm - 1] * I->S[mm - 1]);
-  if (est <= tol || mm < m) {
-    *I->flag = cycle;
-  } else {
-    atomicAdd(remaining, 1);
-  }
-}
-
-// in place, row-local: V_i <- sum_q S[i m + q] V_q for i < keep; V_keep <- V_m
-__global__ void __launch_bounds__(kTile)
-    batch_restart_kernel(const BatchInst *insts, const Tile *tiles, int m, int keep) {
-  const BatchInst *I;
-  uint64_t r;
-  if (!tile_row(insts, tiles, I, r)) return;
-  double row[kMaxM];
-  for (int q = 0; q < m; ++q) row[q] = I->V[static_cast<uint64_t>(q) * I->D + r];
-  for (int i = 0; i < keep; ++i) {
-    double acc = 0.0;
-    for (int q = 0; q < m; ++q) acc = fma(I->S[i * m + q], row[q], acc);
-    I->V[static_cast<uint64_t>(i) * I->D + r] = acc;
-  }
-  I->V[static_cast<uint64_t>(keep) * I->D + r] = I->V[static_cast<uint64_t>(m) * I->D + r];
-}
-
-__global__ void batch_reset_kernel(const BatchInst *insts, int keep) {
-  const BatchInst *I = insts + blockIdx.x;
-  if (*I->flag != 0) return;
-  for (int e = threadIdx.x; e < kMaxM * kMaxM; e += blockDim.x) {
-    const int i = e / kMaxM, j = e % kMaxM;
-    I->T[e] = (i == j && i < keep) ? I->theta[i] : 0.0;
-  }
-}
-
-__global__ void batch_zero_kernel(int *p, int n) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = 0;
-}
-
-// start of a cycle (inside the captured graph): count it, clear `remaining`
-__global__ void batch_cycle_begin_kernel(int *cycle_counter, int *remaining) {
-  *cycle_counter += 1;
-  *remaining = 0;
-}
-
-// instances still active at the end: report their current lowest Ritz pair
-// (after the last restart it is V_0, i.e. S = e_0)
-__global__ void batch_mark_unconverged_kernel(const BatchInst *insts, int n_inst) {
-  const int i = blockIdx.x;
-  if (i >= n_inst) return;
-  const BatchInst *I = insts + i;
-  if (*I->flag != 0) return;
-  __syncthreads();
-  for (int q = threadIdx.x; q < kMaxM; q += blockDim.x) I->S[q] = q == 0 ? 1.0 : 0.0;
-  __syncthreads();
-  if (threadIdx.x == 0) *I->flag = -1;
-}
-
-// ---- finalisation (converged instances: flag > 0) ----------------------------
-// w = Ritz vector of the lowest Ritz value (FP64)
-__global__ void __launch_bounds__(kTile)
-    batch_ritz_kernel(const BatchInst *insts, const Tile *tiles, int m) {
-  const Tile t = tiles[blockIdx.x];
-  const BatchInst *I = insts + t.inst;
-  const uint64_t r = t.row0 + threadIdx.x;
-  if (*I->flag == 0 || r >= I->D) return;
-  const int mm = static_cast<uint64_t>(m) < I->D ? m : static_cast<int>(I->D);
-  double acc = 0.0;
-  for (int q = 0; q < mm; ++q) acc = fma(I->S[q], I->V[static_cast<uint64_t>(q) * I->D + r], acc);
-  I->w[r] = acc;
-}
-
-// partials of (y.Hy, y.y) with y = w, Hy = V_m; or (mode 1) |Hy - lambda y|^2
-__global__ void __launch_bounds__(kTile)
-    batch_final_dots_kernel(const BatchInst *insts, const Tile *tiles, int m, int mode,
-                            double *partial) {
-  __shared__ double lds[2][kTile / 64];
-  const Tile t = tiles[blockIdx.x];
-  const BatchInst *I = insts + t.inst;
-  const uint64_t r = t.row0 + threadIdx.x;
-  double a = 0.0, b = 0.0;
-  if (*I->flag != 0 && r < I->D) {
-    const double y = I->w[r], hy = I->V[static_cast<uint64_t>(m) * I->D + r];
-    if (mode == 0) {
-      a = y * hy;
-      b = y * y;
-    } else {
-      const double d = fma(-I->scal[0], y, hy);
-      a = d * d;
-    }
-  }
-  for (int off = 32; off > 0; off >>= 1) {
-    a += __shfl_down(a, off, 64);
-    b += __shfl_down(b, off, 64);
-  }
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  if (lane == 0) {
-    lds[0][wid] = a;
-    lds[1][wid] = b;
-  }
-  __syncthreads();
-  if (threadIdx.x < 2) {
-    double s = 0.0;
-    for (int k = 0; k < kTile / 64; ++k) s += lds[threadIdx.x][k];
-    partial[static_cast<uint64_t>(blockIdx.x) * kPS + threadIdx.x] = s;
-  }
-}
-
-__global__ void __launch_bounds__(256)
-    batch_final_reduce_kernel(const BatchInst *insts, const double *partial, int mode) {
-  const BatchInst *I = insts + blockIdx.x;
-  if (*I->flag == 0) return;
-  __shared__ double lds[2][256];
-  double a = 0.0, b = 0.0;
-  for (uint32_t t = I->tile_begin + threadIdx.x; t < I->tile_end; t += 256) {
-    a += partial[static_cast<uint64_t>(t) * kPS];
-    b += partial[static_cast<uint64_t>(t) * kPS + 1];
-  }
-  lds[0][threadIdx.x] = a;
-  lds[1][threadIdx.x] = b;
-  __syncthreads();
-  for (int off = 128; off > 0; off >>= 1) {
-    if (threadIdx.x < static_cast<unsigned>(off)) {
-      lds[0][threadIdx.x] += lds[0][threadIdx.x + off];
-      lds[1][threadIdx.x] += lds[1][threadIdx.x + off];
-    }
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) {
-    if (mode == 0) {
-      I->scal[0] = lds[0][0] / lds[1][0]; // Rayleigh quotient
-      I->scal[1] = lds[1][0];             // |y|^2
-    } else {
-      I->scal[2] = sqrt(lds[0][0] / I->scal[1]); // |Hy - lambda y| / |y|
-    }
-  }
-}
-
-__global__ void __launch_bounds__(kTile)
-    batch_random_kernel(const BatchInst *insts, const Tile *tiles, uint64_t seed) {
-  const Tile t = tiles[blockIdx.x];
-  const BatchInst *I = insts + t.inst;
-  const uint64_t r = t.row0 + threadIdx.x;
-  if (r >= I->D) return;
-  I->w[r] = hash_uniform(seed + t.inst, r);
-}
-
-unsigned blocks_for(uint64_t n, unsigned b) { return static_cast<unsigned>((n + b - 1) / b); }
 
 } // namespace
 
 // One batched solve; see the file comment.
 int batch_eigsh(const std::vector<Operator *> &ops, double tol, int m_req, uint64_t seed,
                 int max_cycles, double *evals, double *rnorms, int32_t *cycles_out,
-                double *seconds_out, spined_batch_stats *stats) {
-  const auto t_start = std::chrono::steady_clock::now();
+                double *seconds_out, int64_t *matvecs_out, spined_batch_stats *stats) {
+  const auto t_begin = std::chrono::steady_clock::now();
   const int n_inst = static_cast<int>(ops.size());
   if (n_inst == 0) return LS_SUCCESS;
   hipStream_t s = stream();
-  // all instances must share the kernel variant (orbit method, bond form)
   const int method = ops[0]->orbit_method();
   int nw = 1;
   for (auto *op : ops) {
@@ -372,27 +498,40 @@ int batch_eigsh(const std::vector<Operator *> &ops, double tol, int m_req, uint6
   }
   const int m = std::max(4, std::min(m_req > 0 ? m_req : 20, kMaxM));
   const int keep = m / 2;
-  // tiles and per-instance storage
-  std::vector<Tile> htiles;
+  const int resident = method == 1 ? (nw == 1 ? resident_blocks<1, 1>() : resident_blocks<1, 2>())
+                                   : (nw == 1 ? resident_blocks<0, 1>() : resident_blocks<0, 2>());
+  // team sizing: about rows_per_member rows per workgroup, at most max_team
+  // workgroups (SPINED_BATCH_ROWS_PER_MEMBER / SPINED_BATCH_MAX_TEAM override
+  // them, for tuning)
+  uint64_t rows_per_member = kRowsPerMember;
+  int team_cap = kMaxTeam;
+  if (const char *e = std::getenv("SPINED_BATCH_ROWS_PER_MEMBER"))
+    rows_per_member = std::max<uint64_t>(kTile, std::strtoull(e, nullptr, 10));
+  if (const char *e = std::getenv("SPINED_BATCH_MAX_TEAM"))
+    team_cap = std::max(1, std::min(kMaxTeam, std::atoi(e)));
+  const uint32_t max_team = static_cast<uint32_t>(std::max(1, std::min(team_cap, resident)));
+
+  // per-instance storage
   std::vector<uint64_t> voff(n_inst);
-  uint64_t total_rows = 0;
+  uint64_t total_rows = 0, total_vec = 0;
   for (int i = 0; i < n_inst; ++i) {
     const uint64_t D = ops[i]->basis->D;
     if (D == 0) throw Error(LS_INVALID_ARGUMENT, "empty basis in batch");
-    voff[i] = total_rows;
+    voff[i] = total_vec;
+    total_vec += D * static_cast<uint64_t>(m + 3); // m + 2 basis slots and w
     total_rows += D;
   }
-  DeviceBuffer<double> Vpool(total_rows * static_cast<uint64_t>(m + 1));
-  DeviceBuffer<double> wpool(total_rows);
-  const size_t per_small = kMaxM * kMaxM * 2 + kMaxM + 3 * (kMaxM + 1) + 4;
-  DeviceBuffer<double> small(per_small * n_inst);
-  DeviceBuffer<int> flags(n_inst + 1);
-  std::vector<BatchInst> hinst(n_inst);
+  DeviceBuffer<double> vpool(total_vec);
+  const uint64_t pm_per = 2ull * kMaxTeam * kPS;
+  DeviceBuffer<double> pm(pm_per * n_inst);
+  DeviceBuffer<double> outs(8ull * n_inst);
+  DeviceBuffer<unsigned> bars(2ull * n_inst);
+  std::vector<PInst> hinst(n_inst);
   for (int i = 0; i < n_inst; ++i) {
     Operator &op = *ops[i];
     Basis &b = *op.basis;
     const auto &t = op.tables;
-    BatchInst &I = hinst[i];
+    PInst &I = hinst[i];
     I.o = OpArgs{};
     I.bd = BondArgs{t.bond_sites.ptr, t.bond_dp.ptr, t.bond_da.ptr, t.bond_o.ptr, t.bond_flip.ptr,
                     static_cast<int32_t>(t.h_bond_sites.size()), nullptr, 0};
@@ -402,157 +541,99 @@ int batch_eigsh(const std::vector<Operator *> &ops, double tol, int m_req, uint6
     I.ix = b.index_args(t.keeps_weight);
     I.stab = b.stab.ptr;
     I.D = b.D;
-    I.V = Vpool.ptr + voff[i] * static_cast<uint64_t>(m + 1);
-    I.w = wpool.ptr + voff[i];
-    double *sm = small.ptr + per_small * i;
-    I.T = sm;
-    I.S = sm + kMaxM * kMaxM;
-    I.theta = sm + 2 * kMaxM * kMaxM;
-    I.beta = I.theta + kMaxM;
-    I.h1 = I.beta + kMaxM + 1;
-    I.h2 = I.h1 + kMaxM + 1;
-    I.scal = I.h2 + kMaxM + 1;
-    I.flag = flags.ptr + i;
-    I.tile_begin = static_cast<uint32_t>(htiles.size());
-    for (uint64_t r0 = 0; r0 < b.D; r0 += kTile)
-      htiles.push_back(Tile{static_cast<uint32_t>(i), 0u, r0});
-    I.tile_end = static_cast<uint32_t>(htiles.size());
+    I.V = vpool.ptr + voff[i];
+    I.w = I.V + b.D * static_cast<uint64_t>(m + 2);
+    I.pm = pm.ptr + pm_per * i;
+    I.bar = bars.ptr + 2ull * i;
+    I.out = outs.ptr + 8ull * i;
+    I.ntiles = static_cast<uint32_t>((b.D + kTile - 1) / kTile);
+    const uint64_t want = (b.D + rows_per_member - 1) / rows_per_member;
+    I.team = static_cast<uint32_t>(std::max<uint64_t>(1, std::min<uint64_t>({want, max_team, I.ntiles})));
   }
-  // instances whose dimension is at most m would need m > D vectors; they
-  // run with m' = D (exhausted Krylov space converges in the first cycle)
-  const unsigned ntiles = static_cast<unsigned>(htiles.size());
-  DeviceBuffer<BatchInst> dinst(n_inst);
-  DeviceBuffer<Tile> dtiles(ntiles);
-  SPINED_HIP(hipMemcpy(dinst.ptr, hinst.data(), sizeof(BatchInst) * n_inst, hipMemcpyHostToDevice));
-  SPINED_HIP(hipMemcpy(dtiles.ptr, htiles.data(), sizeof(Tile) * ntiles, hipMemcpyHostToDevice));
-  SPINED_HIP(hipMemsetAsync(small.ptr, 0, small.bytes(), s));
-  DeviceBuffer<double> partial(static_cast<uint64_t>(ntiles) * kPS);
-  batch_zero_kernel<<<blocks_for(n_inst + 1, 256), 256, 0, s>>>(flags.ptr, n_inst + 1);
+  // queue: largest instances first (their teams form first), members in order
+  std::vector<int> order(n_inst);
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int a, int b) { return hinst[a].D > hinst[b].D; });
+  std::vector<Slot> hslots;
+  for (int i : order)
+    for (uint32_t k = 0; k < hinst[i].team; ++k) hslots.push_back(Slot{static_cast<uint32_t>(i), k});
+  const uint32_t nslots = static_cast<uint32_t>(hslots.size());
+  DeviceBuffer<PInst> dinst(n_inst);
+  DeviceBuffer<Slot> dslots(nslots);
+  SPINED_HIP(hipMemcpyAsync(dinst.ptr, hinst.data(), sizeof(PInst) * n_inst, hipMemcpyHostToDevice, s));
+  SPINED_HIP(hipMemcpyAsync(dslots.ptr, hslots.data(), sizeof(Slot) * nslots, hipMemcpyHostToDevice, s));
+  DeviceBuffer<unsigned> ticket(1);
+  DeviceBuffer<int> abort_flag(1);
+  DeviceBuffer<uint64_t> t_start(1);
+  const uint64_t nbar = 2ull * n_inst;
+  batch_init_kernel<<<static_cast<unsigned>((nbar + 255) / 256), 256, 0, s>>>(
+      ticket.ptr, abort_flag.ptr, t_start.ptr, bars.ptr, nbar);
   SPINED_CHECK_LAUNCH();
-
-  const unsigned ib = static_cast<unsigned>(n_inst);
-  auto matvec = [&](int src, int dst, int final_pass) {
-    if (method == 1) {
-      if (nw == 1) batch_matvec_kernel<1, 1><<<ntiles, kTile, 0, s>>>(dinst.ptr, dtiles.ptr, src, dst, final_pass);
-      else batch_matvec_kernel<1, 2><<<ntiles, kTile, 0, s>>>(dinst.ptr, dtiles.ptr, src, dst, final_pass);
-    } else {
-      if (nw == 1) batch_matvec_kernel<0, 1><<<ntiles, kTile, 0, s>>>(dinst.ptr, dtiles.ptr, src, dst, final_pass);
-      else batch_matvec_kernel<0, 2><<<ntiles, kTile, 0, s>>>(dinst.ptr, dtiles.ptr, src, dst, final_pass);
-    }
-    SPINED_CHECK_LAUNCH();
-  };
-  auto multidot = [&](int nvec, int mode, int target, int j) {
-    const unsigned chunks = mode == 1 ? 1u : static_cast<unsigned>((nvec + kChunk - 1) / kChunk);
-    batch_multidot_kernel<<<dim3(ntiles, chunks), kTile, 0, s>>>(dinst.ptr, dtiles.ptr, nvec, mode,
-                                                                 partial.ptr);
-    SPINED_CHECK_LAUNCH();
-    batch_reduce_kernel<<<ib, 256, 0, s>>>(dinst.ptr, partial.ptr, mode == 1 ? 1 : nvec, target, j);
-    SPINED_CHECK_LAUNCH();
-  };
-  auto step = [&](int j) {
-    matvec(j, -1, 0);
-    multidot(j + 1, 0, 0, 0);
-    batch_gemv_sub_kernel<<<ntiles, kTile, 0, s>>>(dinst.ptr, dtiles.ptr, j + 1, 0);
-    SPINED_CHECK_LAUNCH();
-    multidot(j + 1, 0, 1, 0);
-    batch_gemv_sub_kernel<<<ntiles, kTile, 0, s>>>(dinst.ptr, dtiles.ptr, j + 1, 1);
-    SPINED_CHECK_LAUNCH();
-    batch_store_column_kernel<<<ib, kMaxM, 0, s>>>(dinst.ptr, j);
-    SPINED_CHECK_LAUNCH();
-    multidot(1, 1, 2, j);
-    batch_scale_store_kernel<<<ntiles, kTile, 0, s>>>(dinst.ptr, dtiles.ptr, j, j + 1);
-    SPINED_CHECK_LAUNCH();
-  };
-  // one restart cycle from step j0 (captured as a graph)
-  DeviceBuffer<int> counters(2); // [0] cycle number, [1] instances still active
-  auto cycle = [&](int j0) {
-    batch_cycle_begin_kernel<<<1, 1, 0, s>>>(counters.ptr, counters.ptr + 1);
-    SPINED_CHECK_LAUNCH();
-    for (int j = j0; j < m; ++j) step(j);
-    batch_jacobi_kernel<<<ib, 256, 0, s>>>(dinst.ptr, m);
-    SPINED_CHECK_LAUNCH();
-    batch_check_kernel<<<blocks_for(n_inst, 256), 256, 0, s>>>(dinst.ptr, n_inst, m, tol,
-                                                               counters.ptr, counters.ptr + 1);
-    SPINED_CHECK_LAUNCH();
-    batch_restart_kernel<<<ntiles, kTile, 0, s>>>(dinst.ptr, dtiles.ptr, m, keep);
-    SPINED_CHECK_LAUNCH();
-    batch_reset_kernel<<<ib, 256, 0, s>>>(dinst.ptr, keep);
-    SPINED_CHECK_LAUNCH();
-  };
-  auto capture = [&](int j0) {
-    hipGraph_t g = nullptr;
-    hipGraphExec_t e = nullptr;
-    SPINED_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-    cycle(j0);
-    SPINED_HIP(hipStreamEndCapture(s, &g));
-    SPINED_HIP(hipGraphInstantiate(&e, g, nullptr, nullptr, 0));
-    SPINED_HIP(hipGraphDestroy(g));
-    return e;
-  };
-
-  // start vectors: w = hash(seed + instance, r), v_0 = w / |w|
-  batch_zero_kernel<<<1, 2, 0, s>>>(counters.ptr, 2);
-  SPINED_CHECK_LAUNCH();
-  batch_random_kernel<<<ntiles, kTile, 0, s>>>(dinst.ptr, dtiles.ptr, seed);
-  SPINED_CHECK_LAUNCH();
-  multidot(1, 1, 2, kMaxM);
-  batch_scale_store_kernel<<<ntiles, kTile, 0, s>>>(dinst.ptr, dtiles.ptr, kMaxM, 0);
-  SPINED_CHECK_LAUNCH();
+  // one workgroup per slot at most, no more than fit on the device at once
+  const unsigned grid = static_cast<unsigned>(std::min<uint64_t>(nslots, static_cast<uint64_t>(resident)));
   SPINED_HIP(hipStreamSynchronize(s));
-
-  hipGraphExec_t first = capture(0), steady = capture(keep);
-  std::vector<double> t_cycle;
-  int done_cycles = 0;
-  int active = n_inst;
-  const auto t0 = std::chrono::steady_clock::now();
-  for (int c = 0; c < max_cycles && active > 0; ++c) {
-    SPINED_HIP(hipGraphLaunch(c == 0 ? first : steady, s));
-    SPINED_HIP(hipMemcpyAsync(&active, counters.ptr + 1, sizeof(int), hipMemcpyDeviceToHost, s));
-    SPINED_HIP(hipStreamSynchronize(s));
-    t_cycle.push_back(std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count());
-    ++done_cycles;
-    if (logging_enabled()) log_line("batch cycle %d: %d instances active", done_cycles, active);
+  hipEvent_t ev0, ev1;
+  SPINED_HIP(hipEventCreate(&ev0));
+  SPINED_HIP(hipEventCreate(&ev1));
+  SPINED_HIP(hipEventRecord(ev0, s));
+  auto launch = [&](auto meth, auto nwc) {
+    constexpr int M = decltype(meth)::value, W = decltype(nwc)::value;
+    batch_persistent_kernel<M, W><<<grid, kTile, 0, s>>>(dinst.ptr, dslots.ptr, nslots, ticket.ptr,
+                                                         m, keep, tol, max_cycles, seed,
+                                                         abort_flag.ptr, t_start.ptr);
+  };
+  if (method == 1) {
+    if (nw == 1) launch(std::integral_constant<int, 1>{}, std::integral_constant<int, 1>{});
+    else launch(std::integral_constant<int, 1>{}, std::integral_constant<int, 2>{});
+  } else {
+    if (nw == 1) launch(std::integral_constant<int, 0>{}, std::integral_constant<int, 1>{});
+    else launch(std::integral_constant<int, 0>{}, std::integral_constant<int, 2>{});
   }
-  (void)hipGraphExecDestroy(first);
-  (void)hipGraphExecDestroy(steady);
-
-  // finalisation: Ritz vector, H y, Rayleigh quotient, explicit residual
-  batch_mark_unconverged_kernel<<<ib, 64, 0, s>>>(dinst.ptr, n_inst);
   SPINED_CHECK_LAUNCH();
-  batch_ritz_kernel<<<ntiles, kTile, 0, s>>>(dinst.ptr, dtiles.ptr, m);
-  SPINED_CHECK_LAUNCH();
-  matvec(-1, m, 1);
-  batch_final_dots_kernel<<<ntiles, kTile, 0, s>>>(dinst.ptr, dtiles.ptr, m, 0, partial.ptr);
-  SPINED_CHECK_LAUNCH();
-  batch_final_reduce_kernel<<<ib, 256, 0, s>>>(dinst.ptr, partial.ptr, 0);
-  SPINED_CHECK_LAUNCH();
-  batch_final_dots_kernel<<<ntiles, kTile, 0, s>>>(dinst.ptr, dtiles.ptr, m, 1, partial.ptr);
-  SPINED_CHECK_LAUNCH();
-  batch_final_reduce_kernel<<<ib, 256, 0, s>>>(dinst.ptr, partial.ptr, 1);
-  SPINED_CHECK_LAUNCH();
-  std::vector<int> hflags(n_inst);
-  std::vector<double> hsmall(per_small * n_inst);
-  SPINED_HIP(hipMemcpyAsync(hflags.data(), flags.ptr, sizeof(int) * n_inst, hipMemcpyDeviceToHost, s));
-  SPINED_HIP(hipMemcpyAsync(hsmall.data(), small.ptr, small.bytes(), hipMemcpyDeviceToHost, s));
+  SPINED_HIP(hipEventRecord(ev1, s));
   SPINED_HIP(hipStreamSynchronize(s));
-  const double total = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
-  int nconv = 0;
+  float ms = 0.f;
+  SPINED_HIP(hipEventElapsedTime(&ms, ev0, ev1));
+  (void)hipEventDestroy(ev0);
+  (void)hipEventDestroy(ev1);
+  const double t_solve = ms * 1e-3;
+  int aborted = 0;
+  std::vector<double> hout(8ull * n_inst);
+  SPINED_HIP(hipMemcpy(&aborted, abort_flag.ptr, sizeof(int), hipMemcpyDeviceToHost));
+  SPINED_HIP(hipMemcpy(hout.data(), outs.ptr, 8ull * n_inst * sizeof(double), hipMemcpyDeviceToHost));
+  if (aborted)
+    throw Error(LS_DEVICE_ERROR, "batched solve abandoned: a team barrier waited longer than its limit");
+  int dev = 0, khz = 0;
+  SPINED_HIP(hipGetDevice(&dev));
+  SPINED_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
+  const double tick_s = khz > 0 ? 1.0 / (1e3 * khz) : 1e-8;
+  int nconv = 0, max_cycles_seen = 0;
+  int64_t max_mv = 0;
   for (int i = 0; i < n_inst; ++i) {
-    const double *sc = hsmall.data() + per_small * i + (hinst[i].scal - hinst[i].T);
-    evals[i] = sc[0];
-    rnorms[i] = sc[2];
-    const int f = hflags[i];
-    if (cycles_out) cycles_out[i] = f;
-    if (seconds_out) seconds_out[i] = f > 0 ? t_cycle[f - 1] : (t_cycle.empty() ? 0.0 : t_cycle.back());
-    if (f > 0 && rnorms[i] <= tol) ++nconv;
+    const double *o = hout.data() + 8ull * i;
+    evals[i] = o[0];
+    rnorms[i] = o[1];
+    const int cyc = static_cast<int>(o[2]);
+    const bool ok = o[5] == 1.0 && o[1] <= tol;
+    if (ok) ++nconv;
+    if (cycles_out) cycles_out[i] = ok ? cyc : -1;
+    if (seconds_out) seconds_out[i] = o[4] * tick_s;
+    if (matvecs_out) matvecs_out[i] = static_cast<int64_t>(o[3]);
+    max_cycles_seen = std::max(max_cycles_seen, cyc);
+    max_mv = std::max<int64_t>(max_mv, static_cast<int64_t>(o[3]));
   }
+  if (logging_enabled())
+    log_line("batch: %d instances in %u slots on %u workgroups (largest team %u): %.3f s",
+             n_inst, nslots, grid, max_team, t_solve);
   if (stats) {
-    stats->cycles = done_cycles;
+    stats->cycles = max_cycles_seen;
     stats->converged = nconv;
-    stats->seconds_total = total;
-    stats->seconds_cycles = t_cycle.empty() ? 0.0 : t_cycle.back();
+    stats->seconds_total =
+        std::chrono::duration<double>(std::chrono::steady_clock::now() - t_begin).count();
+    stats->seconds_cycles = t_solve;
     stats->total_rows = total_rows;
-    stats->matvecs_per_instance_max = done_cycles ? m + (done_cycles - 1) * (m - keep) + 1 : 0;
+    stats->matvecs_per_instance_max = max_mv;
   }
   return nconv == n_inst ? LS_SUCCESS : LS_NOT_CONVERGED;
 }

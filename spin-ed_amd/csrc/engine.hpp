@@ -257,7 +257,8 @@ struct MatvecCall {
 };
 void launch_matvec(const MatvecCall &c);
 // out[0], out[1] (device) = re, im of sum_r conj(x_r) y_r in FP64, in a fixed
-// order; partial holds 2 * nb doubles, nb = vdot_blocks(n).
+// order; partial holds 2 * nb doubles, nb = vdot_blocks(n) <= kVdotMaxBlocks.
+constexpr unsigned kVdotMaxBlocks = 2048;
 unsigned vdot_blocks(uint64_t n);
 void launch_vdot(VecType t, const void *x, const void *y, uint64_t n, double *partial,
                  unsigned nb, double *out, hipStream_t s);
@@ -291,7 +292,7 @@ int eigsh(Operator &op, int dtype, uint64_t k, double tol, const spined_eigsh_op
 // ---- batched solver (batch.hip) ---------------------------------------------
 int batch_eigsh(const std::vector<Operator *> &ops, double tol, int m_req, uint64_t seed,
                 int max_cycles, double *evals, double *rnorms, int32_t *cycles_out,
-                double *seconds_out, spined_batch_stats *stats);
+                double *seconds_out, int64_t *matvecs_out, spined_batch_stats *stats);
 
 // ---- misc kernels (solver.hip) ----------------------------------------------
 void launch_random_fill(double *v, uint64_t n, uint64_t seed, hipStream_t s);

@@ -10,14 +10,17 @@ namespace spined {
 constexpr int kJacobiMax = 64;
 
 // Parallel cyclic Jacobi (round-robin pairing) for the symmetric m x m
-// projected matrix, m <= 64, in LDS.  evals ascending; S column-major.
+// projected matrix, m <= MAXM (even, <= 64), in LDS.  evals ascending; S
+// column-major.  MAXM sizes the LDS workspace (2 MAXM (MAXM + 1) doubles).
+template <int MAXM = kJacobiMax>
 __device__ inline void jacobi_eigh_device(const double *__restrict__ T, int ldt, int m,
                                           double *__restrict__ evals, double *__restrict__ S,
                                           int max_sweeps) {
-  __shared__ double A[kJacobiMax][kJacobiMax + 1];
-  __shared__ double V[kJacobiMax][kJacobiMax + 1];
-  __shared__ double cs[kJacobiMax / 2], sn[kJacobiMax / 2];
-  __shared__ int pp[kJacobiMax / 2], qq[kJacobiMax / 2];
+  static_assert(MAXM % 2 == 0 && MAXM <= kJacobiMax, "even size up to kJacobiMax");
+  __shared__ double A[MAXM][MAXM + 1];
+  __shared__ double V[MAXM][MAXM + 1];
+  __shared__ double cs[MAXM / 2], sn[MAXM / 2];
+  __shared__ int pp[MAXM / 2], qq[MAXM / 2];
   __shared__ double red[256];
   __shared__ int done;
   const int tid = threadIdx.x;
@@ -111,9 +114,9 @@ __device__ inline void jacobi_eigh_device(const double *__restrict__ T, int ldt,
     }
   }
   __syncthreads();
+  __shared__ int order[MAXM];
   if (tid == 0) {
     // selection sort of the m real eigenpairs (the padding index is dropped)
-    int order[kJacobiMax];
     for (int i = 0; i < m; ++i) order[i] = i;
     for (int i = 0; i < m; ++i) {
       int best = i;

@@ -1059,7 +1059,7 @@ void launch_vdot(VecType t, const void *x, const void *y, uint64_t n, double *pa
   SPINED_CHECK_LAUNCH();
 }
 
-unsigned vdot_blocks(uint64_t n) { return grid_for(n); }
+unsigned vdot_blocks(uint64_t n) { return std::max(1u, std::min(grid_for(n), kVdotMaxBlocks)); }
 
 uint64_t count_offdiag_device(const Operator &op) {
   const Basis &b = *op.basis;

@@ -100,8 +100,6 @@ class SpinedStats(C.Structure):
         ("seconds_matvec", C.c_double),
         ("seconds_start", C.c_double),
         ("seconds_matvec_mixed", C.c_double),
-        ("matvecs_anchor", C.c_int64),
-        ("seconds_matvec_anchor", C.c_double),
     ]
 
 

@@ -14,6 +14,7 @@ import spined
 from spined import lattice
 from spined.internal import LatticeSymmetriesException
 from helpers import Spec, engine, oracle, ring
+from test_oracle_pins import bethe_ground_energy
 
 pytestmark = pytest.mark.gpu
 
@@ -218,3 +219,27 @@ def test_expectation_on_device_matches_oracle(spec, dtype):
     spined.check_status(_lib.load().spined_operator_expectation_device(
         op.ptr, code, D, 3, dx.ptr, D, dy.ptr, out.ctypes.data))
     np.testing.assert_array_equal(out[0::2] + 1j * out[1::2], np.asarray(got, dtype=np.complex128))
+
+
+@pytest.mark.parametrize("want_vectors", [True, False])
+def test_eigsh_mixed_ring32(want_vectors):
+    """FP32 storage on a basis of more than 2^20 states (the FP32 phase runs
+    to its floor, then the FP64 phase finishes): E0 equals the FP64 solve and
+    the Bethe-ansatz value, the residual is explicit, and the eigenvector
+    overlaps the FP64 one to 1 - 1e-8."""
+    spec = ring(32)
+    basis, op = engine(spec)
+    rm = spined.eigsh(op, k=1, tol=1e-10, precision="mixed", want_vectors=want_vectors,
+                      verbose=True)
+    r64 = spined.eigsh(op, k=1, tol=1e-10, want_vectors=want_vectors)
+    print("fp64", r64.evals, r64.rnorms, r64.stats)
+    print("mixed", rm.evals, rm.rnorms, rm.stats)
+    assert rm.status == 0 and r64.status == 0
+    assert rm.rnorms[0] <= 1e-10
+    e_bethe = bethe_ground_energy(32)
+    assert abs(rm.evals[0] - r64.evals[0]) <= 1e-10 * abs(r64.evals[0])
+    assert abs(rm.evals[0] - e_bethe) <= 1e-10 * abs(e_bethe)
+    assert rm.stats["matvecs_mixed"] > 0
+    if want_vectors:
+        assert abs(np.linalg.norm(rm.evecs[:, 0]) - 1) <= 1e-12
+        assert abs(np.dot(rm.evecs[:, 0], r64.evecs[:, 0])) >= 1 - 1e-8

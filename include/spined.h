@@ -133,6 +133,19 @@ int spined_operator_matmat_device(const ls_operator *op, int dtype, uint64_t siz
 int spined_operator_expectation_device(const ls_operator *op, int dtype, uint64_t size,
                                        uint64_t block_size, const void *x, uint64_t x_stride,
                                        void *y_scratch, double *out);
+/* The same as spined_operator_expectation_device, but allocating no device
+ * memory: no counterpart in the reference; for callers that cannot allocate
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The dot's partial sums live in a fixed device array of
+ * 2 * 2048 + 2 * 64 doubles (33 KB) that each GPU holds until the process
+ * ends and every call on that GPU shares: calls that overlap on one GPU race.
+ * At most 64 columns per call (more: LS_INVALID_ARGUMENT).  The operator must
+ * have been applied once on this device (its tables, and for a complex
+ * operator its phase table, must be there already; else LS_INVALID_ARGUMENT);
+ * an existing gather cache is used, none is built. */
+int spined_operator_expectation_device_noalloc(const ls_operator *op, int dtype, uint64_t size,
+                                               uint64_t block_size, const void *x,
+                                               uint64_t x_stride, void *y_scratch, double *out);
 /* Y = H X for a row-interleaved block of block_size in {1, 2, 4, 8} real
  * vectors (element (row, c) at row * block_size + c; DEVICE pointers): one
  * orbit scan and index search per term serve the whole block, and every

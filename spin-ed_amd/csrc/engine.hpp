@@ -254,6 +254,9 @@ struct MatvecCall {
   void *y;
   uint64_t size;
   hipStream_t stream;
+  // no device allocation: the gather cache is read when it is built but not
+  // built, and a complex operator's phase table must exist already
+  bool no_alloc = false;
 };
 void launch_matvec(const MatvecCall &c);
 // out[0], out[1] (device) = re, im of sum_r conj(x_r) y_r in FP64, in a fixed

@@ -641,7 +641,7 @@ void launch_cached(const Ctx &c, const GatherCache &gc, const XT *x, YT *y, hipS
 template <int METHOD> bool try_cached(const MatvecCall &m, const Ctx &c) {
   GatherCache &gc = *m.op->cache;
   std::lock_guard<std::mutex> lk(gc.mu);
-  if (!ensure_cache<METHOD>(*m.op, c, gc, m.stream)) return false;
+  if (m.no_alloc ? !gc.built : !ensure_cache<METHOD>(*m.op, c, gc, m.stream)) return false;
   using VT = VecType;
   if (m.xt == VT::F64 && m.yt == VT::F64)
     launch_cached(c, gc, static_cast<const double *>(m.x), static_cast<double *>(m.y), m.stream);
@@ -768,13 +768,17 @@ Ctx make_ctx(const Operator &op, const double *pc, const double *ps) {
 
 std::mutex g_phase_mu;
 
-const PhaseTable &phase_table(const Basis &b) {
+const PhaseTable &phase_table(const Basis &b, bool no_alloc = false) {
   // one table per (device, L); small and kept for the process lifetime
   static std::vector<std::pair<std::pair<int, int>, std::unique_ptr<PhaseTable>>> cache;
   std::lock_guard<std::mutex> lk(g_phase_mu);
   const int dev = current_device();
   for (auto &kv : cache)
     if (kv.first.first == dev && kv.first.second == b.tables.L) return *kv.second;
+  if (no_alloc)
+    throw Error(LS_INVALID_ARGUMENT, "complex phase table not on the device yet: apply the "
+                                     "operator once (ls_operator_matmat) before a call that "
+                                     "allocates nothing");
   auto t = std::make_unique<PhaseTable>();
   const int L = b.tables.L;
   if (L > (1 << 20)) throw Error(LS_NOT_IMPLEMENTED, "complex sectors with huge periodicity");
@@ -801,7 +805,7 @@ void launch_matvec(const MatvecCall &m) {
     throw Error(LS_INVALID_DATATYPE, "operator is complex; use a complex datatype");
   const double *pc = nullptr, *ps = nullptr;
   if (cplx) {
-    const auto &pt = phase_table(b);
+    const auto &pt = phase_table(b, m.no_alloc);
     pc = pt.cos_t.ptr;
     ps = pt.sin_t.ptr;
   }

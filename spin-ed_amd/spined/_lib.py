@@ -31,7 +31,7 @@ SPINED_SYMBOLS = (
     "spined_eigsh_options_init", "spined_eigsh", "spined_get_last_stats", "spined_stream",
     "spined_synchronize", "spined_basis_device_states", "spined_basis_get_stabilizers",
     "spined_operator_matmat_device", "spined_operator_matmat_rows_device",
-    "spined_operator_expectation_device",
+    "spined_operator_expectation_device", "spined_operator_expectation_device_noalloc",
     "spined_operator_count_offdiag", "spined_operator_orbit_method",
     "spined_operator_set_orbit_method", "spined_operator_set_schedule",
     "spined_operator_get_schedule", "spined_basis_set_bucket_bits",
@@ -162,6 +162,7 @@ def load():
         "spined_operator_matmat_device": (I, [P, I, U64, U64, P, U64, P, U64, I]),
         "spined_operator_matmat_rows_device": (I, [P, I, U64, U64, P, P, I]),
         "spined_operator_expectation_device": (I, [P, I, U64, U64, P, U64, P, P]),
+        "spined_operator_expectation_device_noalloc": (I, [P, I, U64, U64, P, U64, P, P]),
         "spined_operator_count_offdiag": (I, [P, C.POINTER(U64)]),
         "spined_operator_orbit_method": (I, [P]),
         "spined_operator_set_orbit_method": (I, [P, I]),

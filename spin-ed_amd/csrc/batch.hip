@@ -54,7 +54,7 @@ constexpr int kPS = kMaxM + 4;  // partial slots per member and reduction
 constexpr int kChunk = 8;       // dots per pass
 constexpr int kMaxKeep = kMaxM / 2; // kept Ritz vectors (keep = m / 2)
 constexpr int kMaxTeam = 32;
-constexpr uint64_t kRowsPerMember = 131072;
+constexpr uint64_t kRowsPerMember = 32768;
 // s_memrealtime runs at 100 MHz: 60 s
 constexpr uint64_t kWaitLimitTicks = 6000000000ull;
 constexpr double kEta = 1e-2; // second Gram-Schmidt pass below this |w'| / |w|
@@ -86,11 +86,15 @@ __device__ inline uint64_t now_ticks() { return wall_clock64(); }
 
 // Team barrier: everything this workgroup wrote before it is visible to the
 // teammates after it, on any XCD.  False when the wait was abandoned (abort).
+// The workgroup barrier orders every wave's stores before thread 0's
+// agent-scope release (which writes the XCD's L2 back), and thread 0's
+// agent-scope acquire (which invalidates the CU's and XCD's caches) before
+// every wave's later loads; one fence pair per workgroup suffices.
 __device__ bool team_barrier(unsigned *bar, uint32_t t, int *abort) {
   __shared__ int s_ok;
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
   __syncthreads();
   if (threadIdx.x == 0) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
     int ok = 1;
     if (t > 1) {
       const unsigned g = __hip_atomic_load(bar + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -114,10 +118,10 @@ __device__ bool team_barrier(unsigned *bar, uint32_t t, int *abort) {
         }
       }
     }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     s_ok = ok;
   }
   __syncthreads();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
   return s_ok != 0;
 }
 

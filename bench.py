@@ -379,6 +379,11 @@ def main():
         "data": "synthetic (deterministic YAML-equivalent deck; random start vector seed 0x5EED)",
         "config": {"workload": desc, "dimension": D, "precision": a.precision,
                    "start": a.start, "tolerance": a.tol, "basis_build_s": round(t_build, 3),
+                   # deck to E0 in one process: basis build, the training data
+                   # (engine ground states of the small rings), training, one solve
+                   # (the HDF5 writes of the representatives and eigenvector not counted)
+                   "end_to_end_s": round(t_build + (train_info["data_s"] + train_info["train_s"]
+                                                     if train_info else 0.0) + per_step, 3),
                    "autotune": tune,
                    "learned_start": train_info},
         "comparison": compare,

@@ -14,7 +14,8 @@ Each fixture <deck>.npz holds only arrays (numpy.load with allow_pickle=False):
   x_seeds       counter-hash seeds of x (pyoracle.random_vector): x = re (+ i im)
   y_idx, y_re, y_im, y_sum_re, y_sum_im, y_norm2   sampled rows and sums of H x
   evals         lowest eigenvalues (dense LAPACK for D <= 6000, else the
-                oracle's Lanczos for real sectors; empty when neither applies)
+                oracle's Lanczos for real sectors; empty when neither applies
+                or the deck is in NO_EVALS)
 
 Run from the repository root:  python tests/golden/make_golden.py
 """
@@ -35,6 +36,10 @@ DECKS = ["heisenberg_chain_4", "heisenberg_chain_10", "heisenberg_kagome_12",
          "heisenberg_square_5x5", "heisenberg_pyrochlore_32"]
 OUT = Path(__file__).resolve().parent
 SAMPLE = 4096
+# decks whose oracle eigensolve is out of reach: every oracle matvec on the
+# pyrochlore deck evaluates 768-element orbits for 3.8e7 terms (minutes), so
+# its fixture pins the basis and H x, not the spectrum
+NO_EVALS = {"heisenberg_pyrochlore_32"}
 SEEDS = (20261202, 777)
 
 
@@ -83,7 +88,9 @@ def make(name):
         x = x + 1j * pyoracle.random_vector(SEEDS[1], D)
     y = o.matvec(x)
     yi = y[idx]
-    if D <= 6000:
+    if name in NO_EVALS:
+        evals = np.empty(0)
+    elif D <= 6000:
         evals = np.linalg.eigvalsh(dense(o))[:min(4, D)]
     elif o.is_real:
         evals = o.lanczos(k=1, tol=1e-12, max_iter=600)["evals"]

@@ -38,7 +38,7 @@ CONFIGS = {
     "ring28": (28, "Heisenberg ring N=28, w=14, k=0, p=+1, z=+1"),
     "ring24": (24, "Heisenberg ring N=24, w=12, k=0, p=+1, z=+1 (28,968 states)"),
     "xxz512": (0, "512 XXZ rings, N uniform over {12..26}, Delta~U(0,2), J_b~U(0.8,1.2), "
-                  "w=N/2, z=+1, residual 1e-9, one batched graph-replayed solve"),
+                  "w=N/2, z=+1, residual 1e-9, one persistent batched solver launch"),
 }
 PEAK_HBM = 8.0e12
 

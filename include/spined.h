@@ -192,7 +192,12 @@ int spined_basis_get_bucket_bits(const ls_spin_basis *basis);
 /* Index of representatives used by every lookup: 0 = bucket table on the
  * top bits + binary search; 1 = open-addressing hash table (2^H slots of
  * 8 bytes, H = ceil(log2 D) + 1, one slot line + one confirming load per
- * lookup); -1 = automatic.  Setting it on a built basis rebuilds the index. */
+ * lookup); 2 = combinatorial rank, for complete bases whose symmetry group
+ * has no permutation but the identity (with or without spin inversion):
+ * index = sum of C(p_i, i + 1) over the up sites p_0 < p_1 < ..., or the
+ * state itself without a Hamming weight, no table beyond w x 64 binomials;
+ * -1 = automatic (2 where it applies, else 0).  Setting it on a built basis
+ * rebuilds the index; 2 on a basis it does not apply to is an error. */
 int spined_basis_set_index_kind(ls_spin_basis *basis, int kind);
 int spined_basis_get_index_kind(const ls_spin_basis *basis);
 /* 1 when the built basis holds every valid orbit minimum of its sector (an
@@ -205,14 +210,15 @@ int spined_basis_is_complete(const ls_spin_basis *basis);
 /* Autotuner: times the operator's real matvec on its own basis for every
  * candidate setting and keeps the fastest.  Candidates: bucket-table sizes
  * (bits ceil(log2 D) + 1 .. + 4) with the row schedule, the default size
- * without it, and the hash table when the bucket search must confirm its
- * hits (spined_basis_is_complete); then the per-vector time of the block
+ * without it, the hash table when the bucket search must confirm its
+ * hits (spined_basis_is_complete), and the rank index where it applies;
+ * then the per-vector time of the block
  * matvec for K = 1, 2, 4, 8.  Index tables larger than a quarter of the
  * free device memory are skipped, leaving room for the eigensolver's
  * vectors.  Each timing is the mean of `repeats` launches after one warm-up.
  * The chosen setting stays on the basis / operator. */
 typedef struct spined_autotune_report {
-  int32_t index_kind;         /* chosen: 0 bucket table, 1 hash */
+  int32_t index_kind;         /* chosen: 0 bucket table, 1 hash, 2 rank */
   int32_t bucket_bits;        /* chosen (bucket table) */
   int32_t schedule;           /* chosen: 1 row schedule on */
   int32_t candidates;         /* settings timed */

@@ -784,6 +784,7 @@ int spined_operator_autotune(ls_operator *op, int32_t repeats, spined_autotune_r
       cands.push_back({0, bits, 1});
     cands.push_back({0, lg + 3, 0});
     if (!exact) cands.push_back({1, -1, 1});
+    if (rank_index_mode(b)) cands.push_back({2, -1, 1});
     // an index may take a quarter of the memory free outside the tuner's
     // own vectors and the current index: the rest is the eigensolver's
     const uint64_t budget =
@@ -896,9 +897,12 @@ int spined_basis_get_bucket_bits(const ls_spin_basis *basis) {
 
 int spined_basis_set_index_kind(ls_spin_basis *basis, int kind) {
   return guarded([&] {
-    if (!basis || kind < -1 || kind > 1) throw Error(LS_INVALID_ARGUMENT, "bad argument");
+    if (!basis || kind < -1 || kind > 2) throw Error(LS_INVALID_ARGUMENT, "bad argument");
     Basis &b = *basis->p;
     std::lock_guard<std::mutex> lk(b.mu);
+    if (kind == 2 && b.built && !rank_index_mode(b))
+      throw Error(LS_INVALID_ARGUMENT, "the rank index needs a complete basis whose symmetry "
+                                       "group has no permutation but the identity");
     b.index_kind_request = kind;
     if (b.built) build_index(b);
   });

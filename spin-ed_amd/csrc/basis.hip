@@ -293,11 +293,63 @@ void build_hash_index(Basis &b, hipStream_t s) {
   SPINED_HIP(hipStreamSynchronize(s));
 }
 
+// Rank mode of the combinatorial-rank index (IndexArgs): 1 (colex rank of
+// weight-w states) or 2 (index = state) for a complete basis whose group
+// has no permutation but the identity, with or without the spin flip; 0
+// when the rank does not give the basis order.  The representatives are
+// then every state of the sector: all 2^n or C(n, w) states, or with the
+// flip the orbit minima min(v, ~v), i.e. the states with site n - 1 down
+// (2^(n-1), or C(n - 1, w) for w = n / 2; other weights have no flip
+// sector).  The dimension is checked against that count.  Weights above
+// kRankMaxW keep the bucket index.
+int rank_index_mode(const Basis &b) {
+  if (!b.complete || b.tables.nperm != 1 || b.n < 1 || b.n > 63) return 0;
+  const bool flip = b.tables.has_flip != 0;
+  if (b.w < 0) return b.D == (1ull << (flip ? b.n - 1 : b.n)) ? 2 : 0;
+  if ((flip && 2 * b.w != b.n) || b.w > kRankMaxW) return 0;
+  return b.D == (flip ? binomial(b.n - 1, b.w) : binomial(b.n, b.w)) ? 1 : 0;
+}
+
+static void build_rank_index(Basis &b, int mode, hipStream_t s) {
+  b.rank_mode = mode;
+  b.rank_tab.release();
+  const int w = mode == 1 ? b.w : 0;
+  std::vector<uint32_t> tab(static_cast<size_t>(std::max(w, 1)) * 64, 0);
+  for (int i = 0; i < w; ++i)
+    for (int p = 0; p < 64; ++p)
+      tab[i * 64 + p] = static_cast<uint32_t>(std::min<uint64_t>(binomial(p, i + 1), 0xffffffffull));
+  b.rank_tab.allocate(tab.size());
+  SPINED_HIP(hipMemcpyAsync(b.rank_tab.ptr, tab.data(), tab.size() * 4, hipMemcpyHostToDevice, s));
+  SPINED_HIP(hipStreamSynchronize(s));
+}
+
 void build_index(Basis &b) {
   hipStream_t s = stream();
   if (b.D >= 0xffffffffull)
     throw Error(LS_NOT_IMPLEMENTED, "dimensions of 2^32 - 1 or more are not supported");
-  b.index_kind = b.index_kind_request >= 0 ? b.index_kind_request : 0;
+  const int rmode = rank_index_mode(b);
+  const int kind = b.index_kind_request >= 0 ? b.index_kind_request : (rmode ? 2 : 0);
+  if (kind == 2 && !rmode)
+    throw Error(LS_INVALID_ARGUMENT, "the rank index needs a complete basis whose symmetry group "
+                                     "has no permutation but the identity");
+  b.index_kind = kind;
+  if (kind != 2) {
+    b.rank_mode = 0;
+    b.rank_tab.release();
+  }
+  if (b.index_kind == 2) {
+    b.slots.release();
+    b.hmask = 0;
+    b.buckets.release();
+    b.dir.release();
+    b.nbuckets = 0;
+    b.nseg = 0;
+    b.shift = 0;
+    b.bucket_bits = 0;
+    b.h_segstart.clear();
+    build_rank_index(b, rmode, s);
+    return;
+  }
   if (b.index_kind == 1) {
     b.buckets.release();
     b.dir.release();

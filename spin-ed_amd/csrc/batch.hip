@@ -174,8 +174,9 @@ struct Team {
 };
 
 template <int METHOD, int NW>
-__device__ inline double matvec_row(const PInst &I, uint64_t r, const double *x) {
-  return gather_row_real<METHOD, NW, true>(I.o, I.bd, I.rt, I.ga, I.ra, I.ix, I.stab, r, x);
+__device__ inline double matvec_row(const PInst &I, const IndexArgs &ix, uint64_t r,
+                                    const double *x) {
+  return gather_row_real<METHOD, NW, true>(I.o, I.bd, I.rt, I.ga, I.ra, ix, I.stab, r, x);
 }
 
 // partials of V_q . w for q in [0, nv) and (with ww) w . w in slot nv, over
@@ -248,6 +249,21 @@ __device__ void solve_instance(const PInst &I, uint32_t inst_id, uint32_t member
   __shared__ double sbeta[kMaxM + 1];
   __shared__ double sh1[kPS], sh2[kPS], sred[kPS];
   __shared__ double sC[kMaxM * kMaxKeep]; // restart coefficients [q][i]
+  // the rank index (bases without permutations, orbit method 0): its
+  // binomial table is read from LDS by every lookup
+  __shared__ uint32_t sRank[METHOD == 0 ? kRankMaxW * 64 : 1];
+  IndexArgs ix = I.ix;
+  if constexpr (METHOD == 0) {
+    if (ix.rank_mode == 1) {
+      for (int e = threadIdx.x; e < ix.rank_w * 64; e += kTile) sRank[e] = ix.rank_tab[e];
+      __syncthreads();
+    }
+    ix.rank_tab = sRank;
+  }
+  auto mv = [&](uint64_t r, const double *x) {
+    if constexpr (METHOD == 0) return matvec_row<METHOD, NW>(I, ix, r, x);
+    else return matvec_row<METHOD, NW>(I, I.ix, r, x);
+  };
   Team tm(I, member, abort);
   const uint64_t D = I.D;
   const int mm = static_cast<int>(std::min<uint64_t>(static_cast<uint64_t>(m), D));
@@ -290,7 +306,7 @@ __device__ void solve_instance(const PInst &I, uint32_t inst_id, uint32_t member
       const double *x = vec(j);
       for (uint32_t tile = member; tile < I.ntiles; tile += tm.t) {
         const uint64_t r = static_cast<uint64_t>(tile) * kTile + threadIdx.x;
-        if (r < D) I.w[r] = matvec_row<METHOD, NW>(I, r, x);
+        if (r < D) I.w[r] = mv(r, x);
       }
       ++matvecs;
       dots_pass(tm, j + 1, true);
@@ -361,7 +377,7 @@ __device__ void solve_instance(const PInst &I, uint32_t inst_id, uint32_t member
       for (uint32_t tile = member; tile < I.ntiles; tile += tm.t) {
         const uint64_t r = static_cast<uint64_t>(tile) * kTile + threadIdx.x;
         if (r >= D) continue;
-        const double h = matvec_row<METHOD, NW>(I, r, I.w);
+        const double h = mv(r, I.w);
         hy[r] = h;
         const double y = I.w[r];
         acc2[0] = fma(y, h, acc2[0]);

@@ -101,11 +101,14 @@ struct Basis {
   int shift = 0;
   uint64_t nseg = 0;
   uint64_t nbuckets = 0;
-  // index form: 0 = bucket table + binary search, 1 = hash table
+  // index form: 0 = bucket table + binary search, 1 = hash table,
+  // 2 = combinatorial rank (rank_index_mode)
   int index_kind_request = -1; // -1: automatic
   int index_kind = 0;
   DeviceBuffer<uint64_t> slots;
   uint64_t hmask = 0;
+  int rank_mode = 0;                  // IndexArgs::rank_mode
+  DeviceBuffer<uint32_t> rank_tab;    // w x 64 binomials (IndexArgs)
   std::shared_ptr<std::vector<uint64_t>> host_reps; // lazily downloaded
   double seconds_build = 0.0;
 
@@ -118,6 +121,7 @@ struct Basis {
   // images_keep_weight: the operator maps weight-w states to weight-w states
   IndexArgs index_args(bool images_keep_weight) const {
     const bool exact = index_kind == 0 && complete && (w < 0 || images_keep_weight);
+    const bool ranked = index_kind == 2;
     return IndexArgs{reps.ptr,
                      buckets.ptr,
                      dir.ptr,
@@ -126,7 +130,12 @@ struct Basis {
                      shift,
                      index_kind == 1 ? slots.ptr : nullptr,
                      hmask,
-                     exact ? 1 : 0};
+                     exact ? 1 : 0,
+                     ranked ? rank_mode : 0,
+                     w,
+                     ranked && rank_mode == 1 && !images_keep_weight ? 1 : 0,
+                     ranked ? rank_tab.ptr : nullptr,
+                     D};
   }
 };
 
@@ -135,6 +144,7 @@ void build_basis(Basis &b);                                  // basis.hip
 void adopt_basis(Basis &b, uint64_t n, const uint64_t *reps); // basis.hip (ls_build_unsafe)
 void validate_representatives(const Basis &b, uint64_t n, const uint64_t *reps);
 void build_index(Basis &b);                                   // basis.hip
+int rank_index_mode(const Basis &b);                          // basis.hip
 // level-2 bucket count for about 2^bits buckets (fills dir when given)
 uint64_t level2_size(const Basis &b, int bits, uint2 *dir);   // basis.hip
 

@@ -92,6 +92,7 @@ __global__ void __launch_bounds__(kRowBlock)
                       const uint32_t *__restrict__ stab, uint64_t D,
                       const uint16_t *__restrict__ sched, const uint64_t *__restrict__ woff,
                       uint32_t *__restrict__ entries) {
+  ix = index_for_method<METHOD>(ix);
   const uint64_t slot = static_cast<uint64_t>(blockIdx.x) * kRowBlock + threadIdx.x;
   if (slot >= D) return;
   const uint64_t i = scheduled_row(sched, slot);
@@ -337,6 +338,7 @@ __global__ void __launch_bounds__(kBlock)
     matvec_real_direct_kernel(OpArgs o, GenericOrbitArgs ga, RingOrbitArgs ra, IndexArgs ix,
                               const uint32_t *__restrict__ stab, uint64_t D,
                               const XT *__restrict__ x, YT *__restrict__ y) {
+  ix = index_for_method<METHOD>(ix);
   const uint64_t lb = xcd_block(blockIdx.x, gridDim.x);
   const uint64_t i = lb * kBlock + threadIdx.x;
   if (i >= D) return;
@@ -371,6 +373,7 @@ __global__ void __launch_bounds__(kBlock)
     matvec_complex_kernel(OpArgs o, GenericOrbitArgs ga, RingOrbitArgs ra, IndexArgs ix,
                           const uint32_t *__restrict__ stab, uint64_t D,
                           const XT *__restrict__ x, YT *__restrict__ y) {
+  ix = index_for_method<METHOD>(ix);
   const uint64_t lb = xcd_block(blockIdx.x, gridDim.x);
   const uint64_t i = lb * kBlock + threadIdx.x;
   if (i >= D) return;
@@ -414,6 +417,7 @@ template <int METHOD>
 __global__ void __launch_bounds__(kBlock)
     count_offdiag_kernel(OpArgs o, GenericOrbitArgs ga, RingOrbitArgs ra, IndexArgs ix,
                          uint64_t D, unsigned long long *__restrict__ partial) {
+  ix = index_for_method<METHOD>(ix);
   __shared__ unsigned long long lds[kBlock / 64];
   const uint64_t i = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
   unsigned long long c = 0;

@@ -441,6 +441,17 @@ __host__ __device__ inline uint64_t mix64(uint64_t z) {
 //    h = mix64(rep); a slot holds (h >> 32) << 32 | index, or kEmptySlot.  A
 //    lookup reads one slot line for the fingerprint, then reps[index] to
 //    confirm, which the row gather issues together with the x gather.
+//  * combinatorial rank (rank_mode != 0), for complete bases whose group has
+//    no permutation (the identity, with or without the global spin flip).
+//    Their representatives are every state of the sector in ascending
+//    order: all states (index = v); all weight-w states; or, with the flip
+//    (w = n/2 or no weight), the states with site n-1 down.  Ascending order
+//    of weight-w states is colexicographic order of their up-site sets, so
+//    index(v) = sum_i C(p_i, i + 1) over the up sites p_0 < p_1 < ... (the
+//    combinatorial number system; Knuth, TAOCP 7.2.1.3, Theorem L).  A
+//    lookup is w table reads (C(p, i + 1) at rank_tab[i * 64 + p], w <= 32)
+//    with no dependent loads, where the searches above chain three to five;
+//    kernels that can spare 8 KB of LDS stage the table there.
 // ---------------------------------------------------------------------------
 constexpr uint64_t kEmptySlot = ~0ull;
 
@@ -471,7 +482,62 @@ struct IndexArgs {
   // Hamming weight): a bucket search then narrows to the entry without the
   // confirming load of reps.
   int32_t exact;
+  // combinatorial rank: 0 off, 1 colex rank of a weight-rank_w state, 2 the
+  // state itself; rank_check: reject states whose weight is not rank_w
+  int32_t rank_mode;
+  int32_t rank_w;
+  int32_t rank_check;
+  // C(p, i + 1) at [i * 64 + p], i < rank_w, clamped to 2^32 - 1: a state
+  // of the sector only meets entries below its index, and indices are below
+  // the dimension (< 2^32 - 1)
+  const uint32_t *__restrict__ rank_tab;
+  uint64_t rank_dim; // dimension: a sum at or above it is no index
 };
+
+// Largest weight of the rank index (its table: rank_w x 64 entries).
+constexpr int kRankMaxW = 32;
+
+// Up site of v with the lowest index; 63 once v is empty, so that a state
+// of the wrong weight can never read outside the table.
+__host__ __device__ inline int rank_site(uint64_t v) { return ctz64(v | (1ull << 63)); }
+
+// Indices of va and vb in a rank-indexed basis (-1: not a representative,
+// or use_x false).  The two sums run in one loop, so the table reads of
+// both terms are in flight together.
+__host__ __device__ inline void rank_of2(const IndexArgs &t, uint64_t va, bool use_a, uint64_t vb,
+                                         bool use_b, int64_t &ka, int64_t &kb) {
+  if (t.rank_mode == 2) {
+    ka = use_a && va < t.rank_dim ? static_cast<int64_t>(va) : -1;
+    kb = use_b && vb < t.rank_dim ? static_cast<int64_t>(vb) : -1;
+    return;
+  }
+  if (t.rank_check) {
+    use_a = use_a && __builtin_popcountll(va) == t.rank_w;
+    use_b = use_b && __builtin_popcountll(vb) == t.rank_w;
+  }
+  uint64_t ra = 0, rb = 0;
+  for (int i = 0; i < t.rank_w; ++i) {
+    ra += t.rank_tab[i * 64 + rank_site(va)];
+    rb += t.rank_tab[i * 64 + rank_site(vb)];
+    va &= va - 1;
+    vb &= vb - 1;
+  }
+  ka = use_a && ra < t.rank_dim ? static_cast<int64_t>(ra) : -1;
+  kb = use_b && rb < t.rank_dim ? static_cast<int64_t>(rb) : -1;
+}
+
+// The rank index serves only bases without permutations (orbit method 0):
+// kernels of the ring method leave its code out.
+template <int METHOD> __host__ __device__ inline IndexArgs index_for_method(IndexArgs t) {
+  if (METHOD != 0) t.rank_mode = 0;
+  return t;
+}
+
+__host__ __device__ inline int64_t rank_of(const IndexArgs &t, uint64_t v) {
+  int64_t k = -1, unused = -1;
+  rank_of2(t, v, true, v, false, k, unused);
+  return k;
+}
 
 // Level-2 bucket of v, or false when v lies beyond the last segment.
 __host__ __device__ inline bool bucket_of(const IndexArgs &t, uint64_t v, uint64_t &b) {
@@ -514,6 +580,7 @@ __host__ __device__ inline int64_t hash_find(const IndexArgs &t, uint64_t v) {
 }
 
 __host__ __device__ inline int64_t find_index(const IndexArgs &t, uint64_t v) {
+  if (t.rank_mode) return rank_of(t, v);
   if (t.slots) return hash_find(t, v);
   uint64_t b = 0;
   if (!bucket_of(t, v, b)) return -1;
@@ -531,9 +598,13 @@ __host__ __device__ inline int64_t find_index(const IndexArgs &t, uint64_t v) {
 
 // Two independent searches advanced in lockstep, so a lane keeps two
 // dependent load chains in flight instead of one.  use_a / use_b switch a
-// search off (its result is then -1).  Exact for both index forms.
+// search off (its result is then -1).  Exact for every index form.
 __host__ __device__ inline void find_index2(const IndexArgs &t, uint64_t va, bool use_a,
                                             uint64_t vb, bool use_b, int64_t &ka, int64_t &kb) {
+  if (t.rank_mode) {
+    rank_of2(t, va, use_a, vb, use_b, ka, kb);
+    return;
+  }
   if (t.slots) {
     uint64_t pa = 0, pb = 0;
     ka = use_a ? hash_probe(t, va, &pa) : -1;

@@ -227,9 +227,10 @@ __device__ inline double row_pass_a(const OpArgs &o, const BondArgs &bd, uint64_
 template <int METHOD, int NW, bool BONDS, class XT>
 __device__ inline double gather_row_real(const OpArgs &o, const BondArgs &bd, const RatioArgs &rt,
                                          const GenericOrbitArgs &ga, const RingOrbitArgs &ra,
-                                         const IndexArgs &ix, const uint32_t *__restrict__ stab,
+                                         const IndexArgs &ix_in, const uint32_t *__restrict__ stab,
                                          uint64_t i, const XT *__restrict__ x,
                                          const double *__restrict__ bond_o = nullptr) {
+  const IndexArgs ix = index_for_method<METHOD>(ix_in);
   if (!bond_o) bond_o = bd.o;
   const uint64_t r = ix.reps[i];
   uint64_t act[NW];
@@ -281,9 +282,10 @@ template <class XT, int K> struct alignas(K * sizeof(XT) < 16 ? K * sizeof(XT) :
 template <int METHOD, int NW, bool BONDS, int K, class XT>
 __device__ inline void gather_row_real_block(const OpArgs &o, const BondArgs &bd,
                                              const RatioArgs &rt, const GenericOrbitArgs &ga,
-                                             const RingOrbitArgs &ra, const IndexArgs &ix,
+                                             const RingOrbitArgs &ra, const IndexArgs &ix_in,
                                              const uint32_t *__restrict__ stab, uint64_t i,
                                              const XT *__restrict__ x, double (&acc)[K]) {
+  const IndexArgs ix = index_for_method<METHOD>(ix_in);
   using Row = RowK<XT, K>;
   const Row *__restrict__ xr = reinterpret_cast<const Row *>(x);
   const uint64_t r = ix.reps[i];

@@ -1,5 +1,6 @@
 """Hash index of representatives (spined_basis_set_index_kind(basis, 1))
-against the bucket index: every lookup is exact in both forms, so matvecs
+against the bucket index (and, for the basis without permutations, the
+rank index): every lookup is exact in each form, so matvecs
 (single, block, complex), off-diagonal counts and eigenpairs are
 bit-identical."""
 import numpy as np
@@ -33,7 +34,8 @@ def test_hash_index_matvec_bit_identical(spec):
     x = rng.uniform(-1, 1, D)
     if not o.is_real:
         x = x + 1j * rng.uniform(-1, 1, D)
-    assert spined.index_kind(basis) == 0
+    # a basis with no permutation symmetry gets the rank index by default
+    assert spined.index_kind(basis) == (2 if not spec.gens else 0)
     y0 = spined.apply(op, x)
     c0 = spined.count_offdiag(op)
     spined.set_index_kind(basis, 1)

@@ -5,7 +5,8 @@
 // Network (2,722 parameters; the oracle's cnn_oracle.c is its FP64 twin):
 //   h0[i] = +1 if bit i is set else -1
 //   z_l[c][i] = b_l[c] + sum_{c', d<5} W_l[c][c'][d] h_{l-1}[c'][(i+d-2) mod N]
-//   h_l = GELU_tanh(z_l), l = 1..3, 16 channels;  f[c] = sum_i h3[c][i]
+//   h_l = act(z_l) = z_l clamp(1/2 + z_l/4, 0, 1), l = 1..3, 16 channels;
+//   f[c] = sum_i h3[c][i]
 //   a = u0 + u.f  (log|psi|),  t = v0 + v.f  (sign logit)
 //
 // MFMA mapping (v_mfma_f32_16x16x32_bf16, one wave = 16 samples of equal N):
@@ -43,26 +44,21 @@ constexpr int OFF_W1 = 0, OFF_B1 = OFF_W1 + CH * KW, OFF_W2 = OFF_B1 + CH,
 constexpr int kMaxN = 64;
 constexpr int kInPlaceMinN = 8; // layer 2 in place for n >= 8 (see cnn_forward_kernel)
 
-constexpr float GELU_A = 0.7978845608028654f;
-constexpr float GELU_B = 0.044715f;
-
-// GELU in its tanh form, 0.5 x (1 + tanh(u)) with u = A (x + B x^3), is
-// x s(2u) with the logistic s(y) = 1 / (1 + exp(-y)): one native base-2
-// exponential and one native reciprocal (no division sequence).  s
-// saturates to 0 / 1 through the exponential's overflow / underflow; the
-// error is a few FP32 ulp (the network's operands are BF16).
-constexpr float LOG2E = 1.4426950408889634f;
-__device__ inline float logistic2(float u) {
-  return __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(-2.0f * LOG2E * u));
+// Activation: the hard-sigmoid gate h(x) = x g(x), g(x) = clamp(1/2 + x/4,
+// 0, 1): zero below -2, the identity above 2 and x/2 + x^2/4 between --
+// GELU's shape (a small negative lobe, slope 1/2 at 0) from three VALU
+// operations (an fma whose result the hardware clamps to [0, 1], a mul) and
+// no transcendental.  GELU's exponential and reciprocal, 8 per lane per
+// position at a quarter of the VALU rate, made the inference kernel
+// VALU-bound.  x / 4 is exact, so g is the correctly rounded 1/2 + x/4.
+// h'(x) = g(x) + x/4 inside (-2, 2), 0 below and 1 above (the one-sided
+// values at the two kinks).
+__device__ inline float act_f(float x) {
+  return x * __builtin_amdgcn_fmed3f(fmaf(x, 0.25f, 0.5f), 0.0f, 1.0f);
 }
-
-__device__ inline float gelu_f(float x) {
-  return x * logistic2(GELU_A * (x + GELU_B * x * x * x));
-}
-// d/dx [x s(2u)] = s + 2 x s (1 - s) A (1 + 3 B x^2)
-__device__ inline float gelu_grad_f(float x) {
-  const float sg = logistic2(GELU_A * (x + GELU_B * x * x * x));
-  return sg + 2.0f * x * sg * (1.0f - sg) * GELU_A * (1.0f + 3.0f * GELU_B * x * x);
+__device__ inline float act_grad_f(float x) {
+  const float g = fmaf(x, 0.25f, 0.5f);
+  return g <= 0.0f ? 0.0f : (g >= 1.0f ? 1.0f : g + 0.25f * x);
 }
 
 __device__ inline floatx4 mfma(bf16x8 a, bf16x8 b, floatx4 c) {
@@ -205,7 +201,7 @@ __global__ void __launch_bounds__(64)
         const float wk = static_cast<float>(static_cast<__bf16>(P[OFF_W1 + ch * KW + k]));
         v += ((pat >> k) & 1) ? wk : -wk;
       }
-      tab[e] = static_cast<__bf16>(gelu_f(v + P[OFF_B1 + ch]));
+      tab[e] = static_cast<__bf16>(act_f(v + P[OFF_B1 + ch]));
     }
     __syncthreads();
     for (int pos = 0; pos < n; ++pos) {
@@ -227,7 +223,7 @@ __global__ void __launch_bounds__(64)
       for (int r = 0; r < 4; ++r) acc[r] += bias[r];
       if (z1) store4_f32(z1, n, pos, acc);
 #pragma unroll
-      for (int r = 0; r < 4; ++r) acc[r] = gelu_f(acc[r]);
+      for (int r = 0; r < 4; ++r) acc[r] = act_f(acc[r]);
       store4(h1, n, pos, acc);
     }
   }
@@ -251,7 +247,7 @@ __global__ void __launch_bounds__(64)
         for (int r = 0; r < 4; ++r) acc[r] += bias[r];
         if (z2) store4_f32(z2, n, pos, acc);
 #pragma unroll
-        for (int r = 0; r < 4; ++r) acc[r] = gelu_f(acc[r]);
+        for (int r = 0; r < 4; ++r) acc[r] = act_f(acc[r]);
         if (pos >= 4) store4(h1, n, pos - 2, prev2);
         if (pos == 0) o0 = acc;
         if (pos == 1) o1 = acc;
@@ -269,7 +265,7 @@ __global__ void __launch_bounds__(64)
         for (int r = 0; r < 4; ++r) acc[r] += bias[r];
         if (z2) store4_f32(z2, n, pos, acc);
 #pragma unroll
-        for (int r = 0; r < 4; ++r) acc[r] = gelu_f(acc[r]);
+        for (int r = 0; r < 4; ++r) acc[r] = act_f(acc[r]);
         store4(h2, n, pos, acc);
       }
     }
@@ -288,7 +284,7 @@ __global__ void __launch_bounds__(64)
       for (int r = 0; r < 4; ++r) acc[r] += bias[r];
       if (z3) store4_f32(z3, n, pos, acc);
 #pragma unroll
-      for (int r = 0; r < 4; ++r) f[r] += gelu_f(acc[r]);
+      for (int r = 0; r < 4; ++r) f[r] += act_f(acc[r]);
     }
   }
   // heads: reduce over the 4 lane groups holding channels 4g..4g+3
@@ -308,15 +304,17 @@ __global__ void __launch_bounds__(64)
   }
 }
 
-// Streaming inference (no stored pre-activations, n >= 8): the forward
-// kernel's outputs (same operands and MFMA sums) from 7 KB of LDS per wave
-// instead of 21 KB, so about three times the waves stay resident.
+// Streaming inference (no stored pre-activations, n >= kInferMinN): the
+// forward kernel's outputs (same operands; MFMA sums that start from the
+// bias instead of adding it last) from 7 KB of LDS per wave instead of
+// 21 KB, so about three times the waves stay resident.
 //  * layer 1 is the table of its 32 input patterns; layer 2 reads its B
 //    fragments from the table directly, so layer 1 is never stored;
 //  * layer 2's outputs go to a ring of the last 8 positions (plus a copy of
 //    positions 0..3 for the wrap-around), and layer 3 at position p - 2
 //    runs as soon as layer 2 at p is done.
 constexpr int kRing = 8;
+constexpr int kInferMinN = 9;
 
 __global__ void __launch_bounds__(64)
     cnn_infer_kernel(CnnArgs c, const uint64_t *__restrict__ states, float *__restrict__ out_a,
@@ -337,7 +335,7 @@ __global__ void __launch_bounds__(64)
       const float wk = static_cast<float>(static_cast<__bf16>(P[OFF_W1 + ch * KW + k]));
       v += ((pat >> k) & 1) ? wk : -wk;
     }
-    tab[e] = static_cast<__bf16>(gelu_f(v + P[OFF_B1 + ch]));
+    tab[e] = static_cast<__bf16>(act_f(v + P[OFF_B1 + ch]));
   }
   __syncthreads();
   bf16x8 a2[3], a3[3];
@@ -349,59 +347,69 @@ __global__ void __launch_bounds__(64)
     b2[r] = P[OFF_B2 + 4 * g + r];
     b3[r] = P[OFF_B3 + 4 * g + r];
   }
-  // B fragment q (rows k = 32 q + 8 g .. + 7: offset d = k >> 4, channels
-  // c0 = k & 15) of a conv at position pos.  Layer 2's fragment reads the
-  // layer-1 table at the 5-spin pattern around p = pos + d - 2, i.e. bits
-  // p - 2 .. p + 2 (cyclic) of the state: win[q] holds the state rotated
-  // right by pos + d - 4 (mod n, exactly n bits), advanced by one bit per
-  // position, so the pattern is its low five bits.
-  const uint64_t nmask = n == 64 ? ~0ull : ((1ull << n) - 1);
-  uint64_t win[3];
+  // B fragment q holds rows k = 32 q + 8 g .. + 7: tap offset d = k >> 4,
+  // channels c0 = k & 15 (0 or 8).  Rows k >= 80 (offset 5) meet zero
+  // weights (load_conv_weights), so their lanes read the offset-4 data
+  // instead of building zeros, and no fragment load needs a branch.
+  int dq[3];
 #pragma unroll
-  for (int q = 0; q < 3; ++q) {
-    const int d = (32 * q + 8 * g) >> 4;
-    const int st = d - 4 < 0 ? d - 4 + n : d - 4;
-    win[q] = st == 0 ? state : (((state >> st) | (state << (n - st))) & nmask);
-  }
+  for (int q = 0; q < 3; ++q) dq[q] = std::min((32 * q + 8 * g) >> 4, KW - 1);
+  const int c0 = (8 * g) & 15;
+  // cur: the state rotated right by pos - 4 (mod n, exactly n bits).  Layer
+  // 2's tap at offset d reads the layer-1 table at the 5-spin pattern around
+  // pos + d - 2, i.e. bits d .. d + 4 of cur (below n for n >= kInferMinN).
+  const uint64_t nmask = n == 64 ? ~0ull : ((1ull << n) - 1);
+  uint64_t cur = ((state >> (n - 4)) | (state << 4)) & nmask;
   auto frag2 = [&](int q) {
-    const int k0 = 32 * q + 8 * g, d = k0 >> 4, c0 = k0 & 15;
-    if (d >= KW) return zero8();
-    return *reinterpret_cast<const bf16x8 *>(tab + static_cast<int>(win[q] & 31u) * CH + c0);
+    const int pat = static_cast<int>((cur >> dq[q]) & 31u);
+    return *reinterpret_cast<const bf16x8 *>(tab + pat * CH + c0);
   };
-  auto frag3 = [&](int pos, int q, bool tail) {
-    const int k0 = 32 * q + 8 * g, d = k0 >> 4, c0 = k0 & 15;
-    if (d >= KW) return zero8();
-    int p = pos + d - 2;
+  // layer 3 at position pos - 2 inside the loop (pos >= 4) reads layer 2 at
+  // pos + d - 4, which lies in [0, n): no wrap-around, always the ring
+  auto frag3_ring = [&](int pos, int q) {
+    const int p = (pos + dq[q] - 4) & (kRing - 1);
+    return *reinterpret_cast<const bf16x8 *>(ring + (p * 16 + s) * CH + c0);
+  };
+  // the last four layer-3 positions wrap around to the head copy of 0..3
+  auto frag3_tail = [&](int pos, int q) {
+    int p = pos + dq[q] - 2;
     p = p < 0 ? p + n : (p >= n ? p - n : p);
-    const __bf16 *src = (tail && p < 4) ? head + (p * 16 + s) * CH : ring + ((p & (kRing - 1)) * 16 + s) * CH;
+    const __bf16 *src =
+        p < 4 ? head + (p * 16 + s) * CH : ring + ((p & (kRing - 1)) * 16 + s) * CH;
     return *reinterpret_cast<const bf16x8 *>(src + c0);
   };
+  // the biases start the MFMA accumulators
   float f[4] = {0.f, 0.f, 0.f, 0.f};
-  auto layer3 = [&](int q, bool tail) {
-    floatx4 acc = {0.f, 0.f, 0.f, 0.f};
+  auto layer3_out = [&](floatx4 acc) {
 #pragma unroll
-    for (int k = 0; k < 3; ++k) acc = mfma(a3[k], frag3(q, k, tail), acc);
-#pragma unroll
-    for (int r = 0; r < 4; ++r) f[r] += gelu_f(acc[r] + b3[r]);
+    for (int r = 0; r < 4; ++r) f[r] += act_f(acc[r]);
   };
   // (one wave per workgroup: its LDS accesses stay in program order)
   for (int pos = 0; pos < n; ++pos) {
-    floatx4 acc = {0.f, 0.f, 0.f, 0.f};
+    floatx4 acc = {b2[0], b2[1], b2[2], b2[3]};
 #pragma unroll
     for (int k = 0; k < 3; ++k) acc = mfma(a2[k], frag2(k), acc);
-#pragma unroll
-    for (int q = 0; q < 3; ++q) win[q] = (win[q] >> 1) | ((win[q] & 1u) << (n - 1));
+    cur = (cur >> 1) | ((cur & 1u) << (n - 1));
     bf16x4 o;
 #pragma unroll
-    for (int r = 0; r < 4; ++r) o[r] = static_cast<__bf16>(gelu_f(acc[r] + b2[r]));
+    for (int r = 0; r < 4; ++r) o[r] = static_cast<__bf16>(act_f(acc[r]));
     *reinterpret_cast<bf16x4 *>(ring + ((pos & (kRing - 1)) * 16 + s) * CH + 4 * g) = o;
     if (pos < 4) *reinterpret_cast<bf16x4 *>(head + (pos * 16 + s) * CH + 4 * g) = o;
-    if (pos >= 4) layer3(pos - 2, false);
+    if (pos >= 4) {
+      floatx4 acc3 = {b3[0], b3[1], b3[2], b3[3]};
+#pragma unroll
+      for (int k = 0; k < 3; ++k) acc3 = mfma(a3[k], frag3_ring(pos, k), acc3);
+      layer3_out(acc3);
+    }
   }
-  layer3(n - 2, true);
-  layer3(n - 1, true);
-  layer3(0, true);
-  layer3(1, true);
+  const int tail_pos[4] = {n - 2, n - 1, 0, 1};
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    floatx4 acc3 = {b3[0], b3[1], b3[2], b3[3]};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) acc3 = mfma(a3[k], frag3_tail(tail_pos[t], k), acc3);
+    layer3_out(acc3);
+  }
   float pa = 0.f, pt = 0.f;
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
@@ -503,7 +511,7 @@ __global__ void __launch_bounds__(64)
   float *out = partial + static_cast<size_t>(blockIdx.x) * NPARAMS;
   const float gas = gsh[0][s], gts = gsh[1][s];
 
-  // heads and d3 = (ga u + gt v) gelu'(z3)  -> bufA; h2 = gelu(z2) -> bufB
+  // heads and d3 = (ga u + gt v) act'(z3)  -> bufA; h2 = act(z2) -> bufB
   float f[4] = {0.f, 0.f, 0.f, 0.f};
   float coef[4];
 #pragma unroll
@@ -512,13 +520,13 @@ __global__ void __launch_bounds__(64)
     floatx4 z = load4_f32(z3, n, pos), d;
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      f[r] += gelu_f(z[r]);
-      d[r] = valid ? coef[r] * gelu_grad_f(z[r]) : 0.0f;
+      f[r] += act_f(z[r]);
+      d[r] = valid ? coef[r] * act_grad_f(z[r]) : 0.0f;
     }
     store4(bufA, n, pos, d);
     floatx4 z2v = load4_f32(z2, n, pos), h;
 #pragma unroll
-    for (int r = 0; r < 4; ++r) h[r] = gelu_f(z2v[r]);
+    for (int r = 0; r < 4; ++r) h[r] = act_f(z2v[r]);
     store4(bufB, n, pos, h);
   }
   // du[co] = sum_s ga[s] f[s][co]: reduce over the 16 samples of a lane group
@@ -577,7 +585,7 @@ __global__ void __launch_bounds__(64)
     write_wgrad(OFF_W3, OFF_B3, cacc, db, false);
   }
   __syncthreads();
-  // d2 = conv^T(d3) * gelu'(z2) -> bufC ; h1 = gelu(z1) -> bufB
+  // d2 = conv^T(d3) * act'(z2) -> bufC ; h1 = act(z1) -> bufB
   {
     bf16x8 a[3];
     load_conv_weights(P + OFF_W3, true, a);
@@ -585,7 +593,7 @@ __global__ void __launch_bounds__(64)
       floatx4 acc = conv_at(a, bufA, n, pos);
       floatx4 z = load4_f32(z2, n, pos);
 #pragma unroll
-      for (int r = 0; r < 4; ++r) acc[r] *= gelu_grad_f(z[r]);
+      for (int r = 0; r < 4; ++r) acc[r] *= act_grad_f(z[r]);
       store4(bufC, n, pos, acc);
     }
   }
@@ -593,7 +601,7 @@ __global__ void __launch_bounds__(64)
   for (int pos = 0; pos < n; ++pos) {
     floatx4 z = load4_f32(z1, n, pos), h;
 #pragma unroll
-    for (int r = 0; r < 4; ++r) h[r] = gelu_f(z[r]);
+    for (int r = 0; r < 4; ++r) h[r] = act_f(z[r]);
     store4(bufB, n, pos, h);
   }
   __syncthreads();
@@ -607,7 +615,7 @@ __global__ void __launch_bounds__(64)
     write_wgrad(OFF_W2, OFF_B2, cacc, db, false);
   }
   __syncthreads();
-  // d1 = conv^T(d2) * gelu'(z1) -> bufA
+  // d1 = conv^T(d2) * act'(z1) -> bufA
   {
     bf16x8 a[3];
     load_conv_weights(P + OFF_W2, true, a);
@@ -615,7 +623,7 @@ __global__ void __launch_bounds__(64)
       floatx4 acc = conv_at(a, bufC, n, pos);
       floatx4 z = load4_f32(z1, n, pos);
 #pragma unroll
-      for (int r = 0; r < 4; ++r) acc[r] *= gelu_grad_f(z[r]);
+      for (int r = 0; r < 4; ++r) acc[r] *= act_grad_f(z[r]);
       store4(bufA, n, pos, acc);
     }
   }
@@ -767,7 +775,7 @@ int spined_cnn_forward_device(const spined_cnn *c, int n, uint64_t count, const 
     if (count == 0) return;
     spined::CnnArgs a{c->m.params.ptr, n, count};
     const unsigned blocks = static_cast<unsigned>((count + 15) / 16);
-    if (!z_store && n >= spined::kRing)
+    if (!z_store && n >= spined::kInferMinN)
       spined::cnn_infer_kernel<<<blocks, 64, 0, spined::stream()>>>(a, states, out_a, out_t);
     else
       hipLaunchKernelGGL(spined::cnn_forward_kernel, dim3(blocks), dim3(64),
@@ -856,7 +864,7 @@ void cnn_start_vector(const spined_cnn *c, const Basis &b, uint64_t marshall_mas
   t.allocate_pooled(D, D * sizeof(double));
   CnnArgs args{c->m.params.ptr, b.n, D};
   const unsigned blocks = static_cast<unsigned>((D + 15) / 16);
-  if (b.n >= kRing)
+  if (b.n >= kInferMinN)
     cnn_infer_kernel<<<blocks, 64, 0, s>>>(args, b.reps.ptr, a.ptr, t.ptr);
   else
     hipLaunchKernelGGL(cnn_forward_kernel, dim3(blocks), dim3(64), lds_bytes_forward(b.n), s,

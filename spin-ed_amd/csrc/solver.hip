@@ -251,15 +251,22 @@ __global__ void __launch_bounds__(kBlock)
   block_partials<NQ>(acc, nvec, partial);
 }
 
-// Last update fused with the norm: w[r] -= sum_i c[i] V_i[r]; partial
-// sums of w[r]^2 (the norm_w multidot's arithmetic).  c = h2, or h1 when
-// skip is given and *skip != 0 (the first pass was the only one).
+// Last update fused with the norm: w[r] -= sum_i h2[i] V_i[r]; partial
+// sums of w[r]^2 (the norm_w multidot's arithmetic).  When *skip != 0 (the
+// first pass was the only one) the update uses h1 and stores the next basis
+// vector directly, vnext[r] = (w[r] - sum_i h1[i] V_i[r]) / *beta, with the
+// Pythagorean beta of second_pass_test_kernel (w and the partials are then
+// left alone).
 template <class VT, int NQ>
 __global__ void __launch_bounds__(kBlock)
     sub_norm_kernel(const VT *const *__restrict__ V, int nvec, const double *__restrict__ h2,
                     double *__restrict__ w, uint64_t n, double *__restrict__ partial,
-                    const double *__restrict__ h1 = nullptr, const int *__restrict__ skip = nullptr) {
-  if (skip && *skip) h2 = h1;
+                    const double *__restrict__ h1, const int *__restrict__ skip,
+                    VT *__restrict__ vnext, const double *__restrict__ beta) {
+  const bool store = *skip != 0;
+  const double *c = store ? h1 : h2;
+  const double b = store ? *beta : 0.0;
+  const double inv = b > 0.0 ? 1.0 / b : 0.0;
   const VT *vp[NQ];
 #pragma unroll
   for (int q = 0; q < NQ; ++q) vp[q] = V[q < nvec ? q : 0];
@@ -271,12 +278,16 @@ __global__ void __launch_bounds__(kBlock)
     for (int q = 0; q < NQ; ++q) vals[q] = to_d(vp[q][r]);
     double sub = 0.0;
 #pragma unroll
-    for (int q = 0; q < NQ; ++q) sub = q < nvec ? fma(h2[q], vals[q], sub) : sub;
+    for (int q = 0; q < NQ; ++q) sub = q < nvec ? fma(c[q], vals[q], sub) : sub;
     const double wr = w[r] - sub;
-    w[r] = wr;
-    acc[0] = fma(wr, wr, acc[0]);
+    if (store) {
+      vnext[r] = static_cast<VT>(wr * inv);
+    } else {
+      w[r] = wr;
+      acc[0] = fma(wr, wr, acc[0]);
+    }
   }
-  block_partials<1>(acc, 1, partial);
+  if (!store) block_partials<1>(acc, 1, partial);
 }
 
 // First Gram-Schmidt pass with the squared norm of w: partial dots of V_i
@@ -313,9 +324,11 @@ __global__ void __launch_bounds__(kBlock)
 // |w'|^2 = w.w - |h|^2; when |w'| >= eta |w| the first pass lost at most a
 // factor 1/eta of accuracy to cancellation and its result is kept (the
 // criterion of Daniel, Gragg, Kaufman & Stewart, Math. Comp. 30 (1976) 772,
-// with eta below their 1/sqrt(2)).  Sets *skip and, when set, h2 = 0.
+// with eta below their 1/sqrt(2)).  Sets *skip and, when set, h2 = 0 and
+// *beta = |w'| = sqrt(w.w - |h|^2) (relative error below eps / eta^2).
 __global__ void second_pass_test_kernel(const double *__restrict__ h, int nvec, double eta,
-                                        double *__restrict__ h2, int *__restrict__ skip) {
+                                        double *__restrict__ h2, int *__restrict__ skip,
+                                        double *__restrict__ beta) {
   __shared__ int flag;
   if (threadIdx.x == 0) {
     double s = 0.0;
@@ -323,17 +336,20 @@ __global__ void second_pass_test_kernel(const double *__restrict__ h, int nvec, 
     const double ww = h[nvec];
     flag = (ww > 0.0 && ww - s >= eta * eta * ww) ? 1 : 0;
     *skip = flag;
+    if (flag) *beta = sqrt(ww - s);
   }
   __syncthreads();
   if (flag)
     for (int i = threadIdx.x; i < nvec; i += blockDim.x) h2[i] = 0.0;
 }
 
-// v[r] = w[r] / (*beta) (beta == 0 -> zero vector)
+// v[r] = w[r] / (*beta) (beta == 0 -> zero vector); nothing when skip is
+// given and set (sub_norm_kernel stored v already)
 template <class VT>
 __global__ void __launch_bounds__(kBlock)
     scale_store_kernel(const double *__restrict__ w, const double *__restrict__ beta,
-                       VT *__restrict__ v, uint64_t n) {
+                       VT *__restrict__ v, uint64_t n, const int *__restrict__ skip = nullptr) {
+  if (skip && *skip) return;
   const double b = *beta;
   const double inv = b > 0.0 ? 1.0 / b : 0.0;
   for (uint64_t r = blockIdx.x * static_cast<uint64_t>(kBlock) + threadIdx.x; r < n;
@@ -341,7 +357,10 @@ __global__ void __launch_bounds__(kBlock)
     v[r] = static_cast<VT>(w[r] * inv);
 }
 
-__global__ void sqrt_inplace_kernel(double *v) { v[0] = sqrt(v[0]); }
+__global__ void sqrt_inplace_kernel(double *v, const int *__restrict__ skip = nullptr) {
+  if (skip && *skip) return;
+  v[0] = sqrt(v[0]);
+}
 
 // T[i*ldt + j] = T[j*ldt + i] = h[i], i <= j; beta store
 __global__ void store_column_kernel(const double *__restrict__ h, int j, double *__restrict__ T,
@@ -609,9 +628,11 @@ template <class VT, bool C = false> struct Phase {
   // kFuseMax) and the norm of the result: dots with w.w; update + dots
   // (the second pass); update + norm.  The second pass is left out on the
   // device (no host round trip) when the first update loses little to
-  // cancellation (second_pass_test_kernel).  Complex vectors take reorth()
-  // and norm_w().
-  void reorth_norm_fused(int nvec, double *h1, double *h2, double *norm_out) {
+  // cancellation (second_pass_test_kernel); the norm is then w.w - |h|^2
+  // and the update writes the next basis vector vnext = w' / |w'| itself
+  // (the caller's scale_store_kernel, given skip, does nothing).  Complex
+  // vectors take reorth() and norm_w().
+  void reorth_norm_fused(int nvec, double *h1, double *h2, double *norm_out, VT *vnext) {
     // register width: the smallest multiple of 8 holding nvec vectors (the
     // slots past nvec still issue loads, of V[0])
     const int nq = (nvec + 7) / 8 * 8;
@@ -621,7 +642,7 @@ template <class VT, bool C = false> struct Phase {
       SPINED_CHECK_LAUNCH();
       reduce_partials_kernel<<<nvec + 1, kBlock, 0, s>>>(partial.ptr, nvec + 1, G, h1, 0);
       SPINED_CHECK_LAUNCH();
-      second_pass_test_kernel<<<1, 64, 0, s>>>(h1, nvec, kSecondPassEta, h2, skip.ptr);
+      second_pass_test_kernel<<<1, 64, 0, s>>>(h1, nvec, kSecondPassEta, h2, skip.ptr, norm_out);
       SPINED_CHECK_LAUNCH();
       sub_dot_kernel<VT, NQ>
           <<<G, kBlock, 0, s>>>(vptr.ptr, nvec, h1, w.ptr, n, partial.ptr, skip.ptr);
@@ -629,7 +650,7 @@ template <class VT, bool C = false> struct Phase {
       reduce_partials_kernel<<<nvec, kBlock, 0, s>>>(partial.ptr, nvec, G, h2, 0, skip.ptr);
       SPINED_CHECK_LAUNCH();
       sub_norm_kernel<VT, NQ><<<G, kBlock, 0, s>>>(vptr.ptr, nvec, h2, w.ptr, n, partial.ptr,
-                                                   h1, skip.ptr);
+                                                   h1, skip.ptr, vnext, norm_out);
       SPINED_CHECK_LAUNCH();
     };
     if (nq <= 8)
@@ -640,9 +661,9 @@ template <class VT, bool C = false> struct Phase {
       passes(std::integral_constant<int, 24>{});
     else
       passes(std::integral_constant<int, kFuseMax>{});
-    reduce_partials_kernel<<<1, kBlock, 0, s>>>(partial.ptr, 1, G, norm_out, 0);
+    reduce_partials_kernel<<<1, kBlock, 0, s>>>(partial.ptr, 1, G, norm_out, 0, skip.ptr);
     SPINED_CHECK_LAUNCH();
-    sqrt_inplace_kernel<<<1, 1, 0, s>>>(norm_out);
+    sqrt_inplace_kernel<<<1, 1, 0, s>>>(norm_out, skip.ptr);
     SPINED_CHECK_LAUNCH();
   }
 
@@ -651,8 +672,9 @@ template <class VT, bool C = false> struct Phase {
            C ? VecType::C128 : (std::is_same<VT, float>::value ? VecType::F32 : VecType::F64),
            w.ptr);
     double *h1 = h.ptr, *h2 = h.ptr + 2 * kMaxBasis + 2;
-    if (!C && j + 1 <= kFuseMax) {
-      if constexpr (!C) reorth_norm_fused(j + 1, h1, h2, beta.ptr + j);
+    const bool fused = !C && j + 1 <= kFuseMax;
+    if (fused) {
+      if constexpr (!C) reorth_norm_fused(j + 1, h1, h2, beta.ptr + j, vecs[j + 1].ptr);
     } else {
       reorth(j + 1, h1, h2);
       norm_w(beta.ptr + j);
@@ -681,7 +703,8 @@ template <class VT, bool C = false> struct Phase {
         return;
       }
     }
-    scale_store_kernel<VT><<<G, kBlock, 0, s>>>(w.ptr, beta.ptr + j, vecs[j + 1].ptr, ne);
+    scale_store_kernel<VT><<<G, kBlock, 0, s>>>(w.ptr, beta.ptr + j, vecs[j + 1].ptr, ne,
+                                                fused ? skip.ptr : nullptr);
     SPINED_CHECK_LAUNCH();
   }
 

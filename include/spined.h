@@ -259,9 +259,8 @@ int spined_batch_eigsh(const ls_operator *const ops[], uint64_t count, double to
 
 /* ---- learned start vector (no counterpart in the reference) ------------ */
 /* Periodic 1D CNN over +-1 spins: 3 circular conv layers (16 channels,
- * kernel 5, activation x clamp(1/2 + x/4, 0, 1)), global sum, heads
- * log|psi| and sign logit; 2,722 FP32 master parameters, BF16 MFMA kernels
- * with FP32 accumulation. */
+ * kernel 5, tanh-GELU), global sum, heads log|psi| and sign logit; 2,722
+ * FP32 master parameters, BF16 MFMA kernels with FP32 accumulation. */
 typedef struct spined_cnn spined_cnn;
 int spined_cnn_num_params(void);
 int spined_cnn_create(spined_cnn **out, const float *init_params);

@@ -6,7 +6,7 @@
  * engine's design (periodic 1D CNN over +-1 spins):
  *   h0[0][i] = +1 if bit i of the state is set, else -1      (1 channel)
  *   z_l[c][i] = b_l[c] + sum_{c', d<5} W_l[c][c'][d] h_{l-1}[c'][(i + d - 2) mod N]
- *   h_l = act(z_l) = z_l clamp(1/2 + z_l/4, 0, 1)   l = 1, 2, 3; 16 channels
+ *   h_l = GELU(z_l)            l = 1, 2, 3; 16 channels; GELU in tanh form
  *   f[c] = sum_i h3[c][i]
  *   a = u0 + sum_c u[c] f[c]   (log |psi|)
  *   t = v0 + sum_c v[c] f[c]   (sign logit)
@@ -39,15 +39,17 @@ enum {
 
 int cnn_num_params(void) { return NPARAMS; }
 
-/* Activation: the hard-sigmoid gate h(x) = x clamp(1/2 + x/4, 0, 1);
- * h'(x) = 1/2 + x/2 inside (-2, 2), 0 below and 1 above (one-sided at the
- * kinks, where g = 1/2 + x/4 reaches 0 or 1). */
-static double act_gate(double x) {
-  return x * fmin(fmax(0.5 + 0.25 * x, 0.0), 1.0);
+/* GELU in its tanh form (Hendrycks & Gimpel, arXiv:1606.08415):
+ * 0.5 x (1 + tanh(sqrt(2/pi) (x + 0.044715 x^3))) */
+#define GELU_A 0.7978845608028654 /* sqrt(2/pi) */
+#define GELU_B 0.044715
+static double gelu(double x) {
+  return 0.5 * x * (1.0 + tanh(GELU_A * (x + GELU_B * x * x * x)));
 }
-static double act_grad(double x) {
-  const double g = 0.5 + 0.25 * x;
-  return g <= 0.0 ? 0.0 : (g >= 1.0 ? 1.0 : g + 0.25 * x);
+static double gelu_grad(double x) {
+  const double u = GELU_A * (x + GELU_B * x * x * x);
+  const double t = tanh(u);
+  return 0.5 * (1.0 + t) + 0.5 * x * (1.0 - t * t) * GELU_A * (1.0 + 3.0 * GELU_B * x * x);
 }
 
 /* z[co][i] = b[co] + sum W[co][ci][d] h[ci][(i+d-2) mod n] */
@@ -75,14 +77,14 @@ void cnn_forward(const double *params, int n, int64_t count, const uint64_t *sta
   for (int64_t s = 0; s < count; ++s) {
     for (int i = 0; i < n; ++i) h0[i] = ((states[s] >> i) & 1) ? 1.0 : -1.0;
     conv(n, 1, params + OFF_W1, params + OFF_B1, h0, z);
-    for (int q = 0; q < C * n; ++q) h1[q] = act_gate(z[q]);
+    for (int q = 0; q < C * n; ++q) h1[q] = gelu(z[q]);
     conv(n, C, params + OFF_W2, params + OFF_B2, h1, z);
-    for (int q = 0; q < C * n; ++q) h2[q] = act_gate(z[q]);
+    for (int q = 0; q < C * n; ++q) h2[q] = gelu(z[q]);
     conv(n, C, params + OFF_W3, params + OFF_B3, h2, z);
     double a = params[OFF_U0], t = params[OFF_V0];
     for (int c = 0; c < C; ++c) {
       double f = 0.0;
-      for (int i = 0; i < n; ++i) f += act_gate(z[c * n + i]);
+      for (int i = 0; i < n; ++i) f += gelu(z[c * n + i]);
       a += params[OFF_U + c] * f;
       t += params[OFF_V + c] * f;
     }
@@ -127,27 +129,27 @@ void cnn_backward(const double *params, int n, int64_t count, const uint64_t *st
   for (int64_t s = 0; s < count; ++s) {
     for (int i = 0; i < n; ++i) h0[i] = ((states[s] >> i) & 1) ? 1.0 : -1.0;
     conv(n, 1, params + OFF_W1, params + OFF_B1, h0, z1);
-    for (int q = 0; q < C * n; ++q) h1[q] = act_gate(z1[q]);
+    for (int q = 0; q < C * n; ++q) h1[q] = gelu(z1[q]);
     conv(n, C, params + OFF_W2, params + OFF_B2, h1, z2);
-    for (int q = 0; q < C * n; ++q) h2[q] = act_gate(z2[q]);
+    for (int q = 0; q < C * n; ++q) h2[q] = gelu(z2[q]);
     conv(n, C, params + OFF_W3, params + OFF_B3, h2, z3);
     /* heads */
     for (int c = 0; c < C; ++c) {
       double f = 0.0;
-      for (int i = 0; i < n; ++i) f += act_gate(z3[c * n + i]);
+      for (int i = 0; i < n; ++i) f += gelu(z3[c * n + i]);
       grad[OFF_U + c] += ga[s] * f;
       grad[OFF_V + c] += gt[s] * f;
       const double df = ga[s] * params[OFF_U + c] + gt[s] * params[OFF_V + c];
-      for (int i = 0; i < n; ++i) d3[c * n + i] = df * act_grad(z3[c * n + i]);
+      for (int i = 0; i < n; ++i) d3[c * n + i] = df * gelu_grad(z3[c * n + i]);
     }
     grad[OFF_U0] += ga[s];
     grad[OFF_V0] += gt[s];
     memset(d2, 0, sizeof(double) * C * n);
     conv_backward(n, C, params + OFF_W3, h2, d3, grad + OFF_W3, grad + OFF_B3, d2);
-    for (int q = 0; q < C * n; ++q) d2[q] *= act_grad(z2[q]);
+    for (int q = 0; q < C * n; ++q) d2[q] *= gelu_grad(z2[q]);
     memset(d1, 0, sizeof(double) * C * n);
     conv_backward(n, C, params + OFF_W2, h1, d2, grad + OFF_W2, grad + OFF_B2, d1);
-    for (int q = 0; q < C * n; ++q) d1[q] *= act_grad(z1[q]);
+    for (int q = 0; q < C * n; ++q) d1[q] *= gelu_grad(z1[q]);
     conv_backward(n, 1, params + OFF_W1, h0, d1, grad + OFF_W1, grad + OFF_B1, NULL);
   }
   free(h0);

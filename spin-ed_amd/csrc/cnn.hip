@@ -5,8 +5,7 @@
 // Network (2,722 parameters; the oracle's cnn_oracle.c is its FP64 twin):
 //   h0[i] = +1 if bit i is set else -1
 //   z_l[c][i] = b_l[c] + sum_{c', d<5} W_l[c][c'][d] h_{l-1}[c'][(i+d-2) mod N]
-//   h_l = act(z_l) = z_l clamp(1/2 + z_l/4, 0, 1), l = 1..3, 16 channels;
-//   f[c] = sum_i h3[c][i]
+//   h_l = GELU_tanh(z_l), l = 1..3, 16 channels;  f[c] = sum_i h3[c][i]
 //   a = u0 + u.f  (log|psi|),  t = v0 + v.f  (sign logit)
 //
 // MFMA mapping (v_mfma_f32_16x16x32_bf16, one wave = 16 samples of equal N):
@@ -44,21 +43,47 @@ constexpr int OFF_W1 = 0, OFF_B1 = OFF_W1 + CH * KW, OFF_W2 = OFF_B1 + CH,
 constexpr int kMaxN = 64;
 constexpr int kInPlaceMinN = 8; // layer 2 in place for n >= 8 (see cnn_forward_kernel)
 
-// Activation: the hard-sigmoid gate h(x) = x g(x), g(x) = clamp(1/2 + x/4,
-// 0, 1): zero below -2, the identity above 2 and x/2 + x^2/4 between --
-// GELU's shape (a small negative lobe, slope 1/2 at 0) from three VALU
-// operations (an fma whose result the hardware clamps to [0, 1], a mul) and
-// no transcendental.  GELU's exponential and reciprocal, 8 per lane per
-// position at a quarter of the VALU rate, made the inference kernel
-// VALU-bound.  x / 4 is exact, so g is the correctly rounded 1/2 + x/4.
-// h'(x) = g(x) + x/4 inside (-2, 2), 0 below and 1 above (the one-sided
-// values at the two kinks).
-__device__ inline float act_f(float x) {
-  return x * __builtin_amdgcn_fmed3f(fmaf(x, 0.25f, 0.5f), 0.0f, 1.0f);
+constexpr float GELU_A = 0.7978845608028654f;
+constexpr float GELU_B = 0.044715f;
+
+// GELU in its tanh form, 0.5 x (1 + tanh(u)) with u = A (x + B x^3), is
+// x s(2u) with the logistic s(y) = 1 / (1 + exp(-y)): one native base-2
+// exponential and one native reciprocal (no division sequence).  s
+// saturates to 0 / 1 through the exponential's overflow / underflow; the
+// error is a few FP32 ulp (the network's operands are BF16).  Training and
+// the layer-1 table use this form.
+constexpr float LOG2E = 1.4426950408889634f;
+__device__ inline float logistic2(float u) {
+  return __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(-2.0f * LOG2E * u));
 }
-__device__ inline float act_grad_f(float x) {
-  const float g = fmaf(x, 0.25f, 0.5f);
-  return g <= 0.0f ? 0.0f : (g >= 1.0f ? 1.0f : g + 0.25f * x);
+
+__device__ inline float gelu_f(float x) {
+  return x * logistic2(GELU_A * (x + GELU_B * x * x * x));
+}
+// d/dx [x s(2u)] = s + 2 x s (1 - s) A (1 + 3 B x^2)
+__device__ inline float gelu_grad_f(float x) {
+  const float sg = logistic2(GELU_A * (x + GELU_B * x * x * x));
+  return sg + 2.0f * x * sg * (1.0f - sg) * GELU_A * (1.0f + 3.0f * GELU_B * x * x);
+}
+
+// The same tanh-form GELU without a transcendental, for the inference
+// kernel: x G(x) with G(x) = clamp(1/2 + x P(min(x^2, c^2)), 0, 1), P of
+// degree 5, c = 3.75 (a minimax fit of (GELU(x) - x/2) / x^2 on (0, c]);
+// beyond c the clamp gives 0 or x.  Its largest error over all x is
+// 5.1e-4 absolute (evaluated in FP32), below the BF16 resolution of the
+// activations it produces (2^-8 relative) wherever they exceed 0.13.
+// Nine VALU operations (the last fma clamps with its output modifier)
+// instead of the exponential and reciprocal, which run at a quarter of the
+// VALU rate and made the inference kernel VALU-bound.
+__device__ inline float gelu_poly_f(float x) {
+  const float y = fminf(x * x, 3.75f * 3.75f);
+  float p = -5.1259361e-07f;
+  p = fmaf(p, y, 2.775556728e-05f);
+  p = fmaf(p, y, -6.292571424e-04f);
+  p = fmaf(p, y, 7.911820231e-03f);
+  p = fmaf(p, y, -6.266994169e-02f);
+  p = fmaf(p, y, 3.963627676e-01f);
+  return x * __builtin_amdgcn_fmed3f(fmaf(x, p, 0.5f), 0.0f, 1.0f);
 }
 
 __device__ inline floatx4 mfma(bf16x8 a, bf16x8 b, floatx4 c) {
@@ -201,7 +226,7 @@ __global__ void __launch_bounds__(64)
         const float wk = static_cast<float>(static_cast<__bf16>(P[OFF_W1 + ch * KW + k]));
         v += ((pat >> k) & 1) ? wk : -wk;
       }
-      tab[e] = static_cast<__bf16>(act_f(v + P[OFF_B1 + ch]));
+      tab[e] = static_cast<__bf16>(gelu_f(v + P[OFF_B1 + ch]));
     }
     __syncthreads();
     for (int pos = 0; pos < n; ++pos) {
@@ -223,7 +248,7 @@ __global__ void __launch_bounds__(64)
       for (int r = 0; r < 4; ++r) acc[r] += bias[r];
       if (z1) store4_f32(z1, n, pos, acc);
 #pragma unroll
-      for (int r = 0; r < 4; ++r) acc[r] = act_f(acc[r]);
+      for (int r = 0; r < 4; ++r) acc[r] = gelu_f(acc[r]);
       store4(h1, n, pos, acc);
     }
   }
@@ -247,7 +272,7 @@ __global__ void __launch_bounds__(64)
         for (int r = 0; r < 4; ++r) acc[r] += bias[r];
         if (z2) store4_f32(z2, n, pos, acc);
 #pragma unroll
-        for (int r = 0; r < 4; ++r) acc[r] = act_f(acc[r]);
+        for (int r = 0; r < 4; ++r) acc[r] = gelu_f(acc[r]);
         if (pos >= 4) store4(h1, n, pos - 2, prev2);
         if (pos == 0) o0 = acc;
         if (pos == 1) o1 = acc;
@@ -265,7 +290,7 @@ __global__ void __launch_bounds__(64)
         for (int r = 0; r < 4; ++r) acc[r] += bias[r];
         if (z2) store4_f32(z2, n, pos, acc);
 #pragma unroll
-        for (int r = 0; r < 4; ++r) acc[r] = act_f(acc[r]);
+        for (int r = 0; r < 4; ++r) acc[r] = gelu_f(acc[r]);
         store4(h2, n, pos, acc);
       }
     }
@@ -284,7 +309,7 @@ __global__ void __launch_bounds__(64)
       for (int r = 0; r < 4; ++r) acc[r] += bias[r];
       if (z3) store4_f32(z3, n, pos, acc);
 #pragma unroll
-      for (int r = 0; r < 4; ++r) f[r] += act_f(acc[r]);
+      for (int r = 0; r < 4; ++r) f[r] += gelu_f(acc[r]);
     }
   }
   // heads: reduce over the 4 lane groups holding channels 4g..4g+3
@@ -335,7 +360,7 @@ __global__ void __launch_bounds__(64)
       const float wk = static_cast<float>(static_cast<__bf16>(P[OFF_W1 + ch * KW + k]));
       v += ((pat >> k) & 1) ? wk : -wk;
     }
-    tab[e] = static_cast<__bf16>(act_f(v + P[OFF_B1 + ch]));
+    tab[e] = static_cast<__bf16>(gelu_f(v + P[OFF_B1 + ch]));
   }
   __syncthreads();
   bf16x8 a2[3], a3[3];
@@ -382,7 +407,7 @@ __global__ void __launch_bounds__(64)
   float f[4] = {0.f, 0.f, 0.f, 0.f};
   auto layer3_out = [&](floatx4 acc) {
 #pragma unroll
-    for (int r = 0; r < 4; ++r) f[r] += act_f(acc[r]);
+    for (int r = 0; r < 4; ++r) f[r] += gelu_poly_f(acc[r]);
   };
   // (one wave per workgroup: its LDS accesses stay in program order)
   for (int pos = 0; pos < n; ++pos) {
@@ -392,7 +417,7 @@ __global__ void __launch_bounds__(64)
     cur = (cur >> 1) | ((cur & 1u) << (n - 1));
     bf16x4 o;
 #pragma unroll
-    for (int r = 0; r < 4; ++r) o[r] = static_cast<__bf16>(act_f(acc[r]));
+    for (int r = 0; r < 4; ++r) o[r] = static_cast<__bf16>(gelu_poly_f(acc[r]));
     *reinterpret_cast<bf16x4 *>(ring + ((pos & (kRing - 1)) * 16 + s) * CH + 4 * g) = o;
     if (pos < 4) *reinterpret_cast<bf16x4 *>(head + (pos * 16 + s) * CH + 4 * g) = o;
     if (pos >= 4) {
@@ -511,7 +536,7 @@ __global__ void __launch_bounds__(64)
   float *out = partial + static_cast<size_t>(blockIdx.x) * NPARAMS;
   const float gas = gsh[0][s], gts = gsh[1][s];
 
-  // heads and d3 = (ga u + gt v) act'(z3)  -> bufA; h2 = act(z2) -> bufB
+  // heads and d3 = (ga u + gt v) gelu'(z3)  -> bufA; h2 = gelu(z2) -> bufB
   float f[4] = {0.f, 0.f, 0.f, 0.f};
   float coef[4];
 #pragma unroll
@@ -520,13 +545,13 @@ __global__ void __launch_bounds__(64)
     floatx4 z = load4_f32(z3, n, pos), d;
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      f[r] += act_f(z[r]);
-      d[r] = valid ? coef[r] * act_grad_f(z[r]) : 0.0f;
+      f[r] += gelu_f(z[r]);
+      d[r] = valid ? coef[r] * gelu_grad_f(z[r]) : 0.0f;
     }
     store4(bufA, n, pos, d);
     floatx4 z2v = load4_f32(z2, n, pos), h;
 #pragma unroll
-    for (int r = 0; r < 4; ++r) h[r] = act_f(z2v[r]);
+    for (int r = 0; r < 4; ++r) h[r] = gelu_f(z2v[r]);
     store4(bufB, n, pos, h);
   }
   // du[co] = sum_s ga[s] f[s][co]: reduce over the 16 samples of a lane group
@@ -585,7 +610,7 @@ __global__ void __launch_bounds__(64)
     write_wgrad(OFF_W3, OFF_B3, cacc, db, false);
   }
   __syncthreads();
-  // d2 = conv^T(d3) * act'(z2) -> bufC ; h1 = act(z1) -> bufB
+  // d2 = conv^T(d3) * gelu'(z2) -> bufC ; h1 = gelu(z1) -> bufB
   {
     bf16x8 a[3];
     load_conv_weights(P + OFF_W3, true, a);
@@ -593,7 +618,7 @@ __global__ void __launch_bounds__(64)
       floatx4 acc = conv_at(a, bufA, n, pos);
       floatx4 z = load4_f32(z2, n, pos);
 #pragma unroll
-      for (int r = 0; r < 4; ++r) acc[r] *= act_grad_f(z[r]);
+      for (int r = 0; r < 4; ++r) acc[r] *= gelu_grad_f(z[r]);
       store4(bufC, n, pos, acc);
     }
   }
@@ -601,7 +626,7 @@ __global__ void __launch_bounds__(64)
   for (int pos = 0; pos < n; ++pos) {
     floatx4 z = load4_f32(z1, n, pos), h;
 #pragma unroll
-    for (int r = 0; r < 4; ++r) h[r] = act_f(z[r]);
+    for (int r = 0; r < 4; ++r) h[r] = gelu_f(z[r]);
     store4(bufB, n, pos, h);
   }
   __syncthreads();
@@ -615,7 +640,7 @@ __global__ void __launch_bounds__(64)
     write_wgrad(OFF_W2, OFF_B2, cacc, db, false);
   }
   __syncthreads();
-  // d1 = conv^T(d2) * act'(z1) -> bufA
+  // d1 = conv^T(d2) * gelu'(z1) -> bufA
   {
     bf16x8 a[3];
     load_conv_weights(P + OFF_W2, true, a);
@@ -623,7 +648,7 @@ __global__ void __launch_bounds__(64)
       floatx4 acc = conv_at(a, bufC, n, pos);
       floatx4 z = load4_f32(z1, n, pos);
 #pragma unroll
-      for (int r = 0; r < 4; ++r) acc[r] *= act_grad_f(z[r]);
+      for (int r = 0; r < 4; ++r) acc[r] *= gelu_grad_f(z[r]);
       store4(bufA, n, pos, acc);
     }
   }

@@ -40,8 +40,7 @@ def test_cnn_is_translation_invariant():
 def numpy_cnn_forward(p, n, states):
     """Independent restatement of the network, vectorised over states:
     h0 = +-1 spins, three circular 5-tap convolutions of 16 channels with
-    the gate x clip(1/2 + x/4, 0, 1) after each, sums over positions, two
-    linear heads.  Taps
+    tanh-form GELU after each, sums over positions, two linear heads.  Taps
     are np.roll views (tap d reads position i + d - 2), the parameters are
     sliced from the documented layout W1[16][1][5], b1, W2[16][16][5], b2,
     W3[16][16][5], b3, u[16], u0, v[16], v0."""
@@ -62,8 +61,8 @@ def numpy_cnn_forward(p, n, states):
     v0, off = take(off, (1,))
     assert off == len(p)
 
-    def act(x):  # hard-sigmoid gate
-        return x * np.clip(0.5 + x / 4.0, 0.0, 1.0)
+    def gelu(x):
+        return 0.5 * x * (1.0 + np.tanh(np.sqrt(2.0 / np.pi) * (x + 0.044715 * x ** 3)))
 
     def conv(h, W, b):  # h: (S, cin, n) -> (S, cout, n)
         taps = np.stack([np.roll(h, -(d - 2), axis=2) for d in range(K)], axis=3)  # (S, cin, n, K)
@@ -71,9 +70,9 @@ def numpy_cnn_forward(p, n, states):
 
     bits = (states[:, None] >> np.arange(n, dtype=np.uint64)[None, :]) & np.uint64(1)
     h = np.where(bits == 1, 1.0, -1.0)[:, None, :]
-    h = act(conv(h, W1, b1))
-    h = act(conv(h, W2, b2))
-    f = act(conv(h, W3, b3)).sum(axis=2)
+    h = gelu(conv(h, W1, b1))
+    h = gelu(conv(h, W2, b2))
+    f = gelu(conv(h, W3, b3)).sum(axis=2)
     return u0[0] + f @ u, v0[0] + f @ v
 
 

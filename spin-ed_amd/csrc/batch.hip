@@ -4,7 +4,7 @@
 // Work queue.  The instances, largest dimension first, are expanded into
 // slots (instance, member): one slot per workgroup of the instance's team,
 // whose size grows with the dimension (one workgroup for the small rings, up
-// to kMaxTeam for the largest).  Every workgroup of the launch takes the next
+// to kDefaultTeamCap for the largest).  Every workgroup of the launch takes the next
 // slot from a device counter, solves its share of that instance together
 // with its teammates, and takes the next slot when the instance is done; the
 // launch ends when the slots run out.  Nothing returns to the host between
@@ -15,8 +15,10 @@
 // basis vector, a second pass only when the first loses more than two digits
 // (Daniel, Gragg, Kaufman & Stewart), Jacobi on the projected matrix, restart
 // on the lowest Ritz vectors).  Member k owns row tiles k, k + t, k + 2t, ...
-// of every vector: matvec rows, dot partials, updates and restarts touch only
-// its own rows (the matvec gathers every row of its input).  Each member
+// of every vector: dot partials, updates and restarts touch only its own
+// rows.  The Lanczos matvec's rows are handed out in chunks from a counter
+// instead (a member that runs faster takes more; a team barrier follows),
+// and every matvec gathers every row of its input.  Each member
 // writes one partial per dot; after a team barrier every member sums the t
 // partials in member order and runs the same Jacobi sweep on the same
 // projected matrix in its own LDS, so all members take the same decisions
@@ -53,8 +55,11 @@ constexpr int kMaxM = 24;       // basis size limit of the batched solver
 constexpr int kPS = kMaxM + 4;  // partial slots per member and reduction
 constexpr int kChunk = 8;       // dots per pass
 constexpr int kMaxKeep = kMaxM / 2; // kept Ritz vectors (keep = m / 2)
-constexpr int kMaxTeam = 32;
+constexpr int kMaxTeam = 128;     // team capacity (partial slots)
+constexpr int kDefaultTeamCap = 32;
 constexpr uint64_t kRowsPerMember = 32768;
+constexpr uint32_t kChunkTiles = 4; // matvec rows handed out per grab: 1024
+constexpr int kBarWords = 3;        // per instance: arrivals, generation, chunk counter
 // s_memrealtime runs at 100 MHz: 60 s
 constexpr uint64_t kWaitLimitTicks = 6000000000ull;
 constexpr double kEta = 1e-2; // second Gram-Schmidt pass below this |w'| / |w|
@@ -71,7 +76,7 @@ struct PInst {
   double *V;      // (m + 2) vectors of D doubles: basis 0..m, slot m + 1 = H y
   double *w;      // D
   double *pm;     // 2 x kMaxTeam x kPS member partials (double buffered)
-  unsigned *bar;  // [0] arrivals, [1] generation
+  unsigned *bar;  // [0] arrivals, [1] generation, [2] matvec chunk counter
   double *out;    // [0] lambda [1] residual [2] cycles [3] matvecs [4] finish ticks [5] status
   uint32_t team;
   uint32_t ntiles;
@@ -271,6 +276,38 @@ __device__ void solve_instance(const PInst &I, uint32_t inst_id, uint32_t member
   double *V = I.V;
   auto vec = [&](int q) { return V + static_cast<uint64_t>(q) * D; };
   int64_t matvecs = 0;
+  // w = H x over all rows, in chunks of kChunkTiles tiles handed out by the
+  // instance's counter: members that run faster (next to lighter work on
+  // their CUs) take more chunks, where a fixed split would make the team
+  // wait for its slowest member.  Every member makes exactly one failing
+  // grab per matvec, so the counter advances by nch + t per matvec and a
+  // grab's chunk is its value minus that base; the next grab is issued
+  // before the current chunk runs.  A row's sum does not depend on who
+  // computes it.  Ends with a team barrier (the Gram-Schmidt passes read
+  // their own rows of w); false on abort.
+  __shared__ uint32_t s_chunk;
+  uint32_t mv_round = 0;
+  auto matvec_chunks = [&](const double *x) {
+    const uint32_t nch = (I.ntiles + kChunkTiles - 1) / kChunkTiles;
+    const uint32_t base = mv_round++ * (nch + tm.t);
+    unsigned *ctr = I.bar + 2;
+    if (threadIdx.x == 0) s_chunk = atomicAdd(ctr, 1u) - base;
+    __syncthreads();
+    uint32_t g = s_chunk;
+    __syncthreads();
+    while (g < nch) {
+      if (threadIdx.x == 0) s_chunk = atomicAdd(ctr, 1u) - base;
+      const uint32_t t_end = std::min((g + 1) * kChunkTiles, I.ntiles);
+      for (uint32_t tile = g * kChunkTiles; tile < t_end; ++tile) {
+        const uint64_t r = static_cast<uint64_t>(tile) * kTile + threadIdx.x;
+        if (r < D) I.w[r] = mv(r, x);
+      }
+      __syncthreads();
+      g = s_chunk;
+      __syncthreads();
+    }
+    return tm.sync();
+  };
 
   // start vector: w = hash(seed + instance, r), v_0 = w / |w|
   {
@@ -304,9 +341,13 @@ __device__ void solve_instance(const PInst &I, uint32_t inst_id, uint32_t member
     for (int j = j0; j < mm; ++j) {
       // w = H v_j (own rows); dots with V_0..V_j and w.w
       const double *x = vec(j);
-      for (uint32_t tile = member; tile < I.ntiles; tile += tm.t) {
-        const uint64_t r = static_cast<uint64_t>(tile) * kTile + threadIdx.x;
-        if (r < D) I.w[r] = mv(r, x);
+      if (tm.t > 1) {
+        if (!matvec_chunks(x)) return;
+      } else {
+        for (uint32_t tile = 0; tile < I.ntiles; ++tile) {
+          const uint64_t r = static_cast<uint64_t>(tile) * kTile + threadIdx.x;
+          if (r < D) I.w[r] = mv(r, x);
+        }
       }
       ++matvecs;
       dots_pass(tm, j + 1, true);
@@ -524,7 +565,7 @@ int batch_eigsh(const std::vector<Operator *> &ops, double tol, int m_req, uint6
   // workgroups (SPINED_BATCH_ROWS_PER_MEMBER / SPINED_BATCH_MAX_TEAM override
   // them, for tuning)
   uint64_t rows_per_member = kRowsPerMember;
-  int team_cap = kMaxTeam;
+  int team_cap = kDefaultTeamCap;
   if (const char *e = std::getenv("SPINED_BATCH_ROWS_PER_MEMBER"))
     rows_per_member = std::max<uint64_t>(kTile, std::strtoull(e, nullptr, 10));
   if (const char *e = std::getenv("SPINED_BATCH_MAX_TEAM"))
@@ -545,7 +586,7 @@ int batch_eigsh(const std::vector<Operator *> &ops, double tol, int m_req, uint6
   const uint64_t pm_per = 2ull * kMaxTeam * kPS;
   DeviceBuffer<double> pm(pm_per * n_inst);
   DeviceBuffer<double> outs(8ull * n_inst);
-  DeviceBuffer<unsigned> bars(2ull * n_inst);
+  DeviceBuffer<unsigned> bars(static_cast<uint64_t>(kBarWords) * n_inst);
   std::vector<PInst> hinst(n_inst);
   for (int i = 0; i < n_inst; ++i) {
     Operator &op = *ops[i];
@@ -564,7 +605,7 @@ int batch_eigsh(const std::vector<Operator *> &ops, double tol, int m_req, uint6
     I.V = vpool.ptr + voff[i];
     I.w = I.V + b.D * static_cast<uint64_t>(m + 2);
     I.pm = pm.ptr + pm_per * i;
-    I.bar = bars.ptr + 2ull * i;
+    I.bar = bars.ptr + static_cast<uint64_t>(kBarWords) * i;
     I.out = outs.ptr + 8ull * i;
     I.ntiles = static_cast<uint32_t>((b.D + kTile - 1) / kTile);
     const uint64_t want = (b.D + rows_per_member - 1) / rows_per_member;
@@ -586,7 +627,7 @@ int batch_eigsh(const std::vector<Operator *> &ops, double tol, int m_req, uint6
   DeviceBuffer<unsigned> ticket(1);
   DeviceBuffer<int> abort_flag(1);
   DeviceBuffer<uint64_t> t_start(1);
-  const uint64_t nbar = 2ull * n_inst;
+  const uint64_t nbar = static_cast<uint64_t>(kBarWords) * n_inst;
   batch_init_kernel<<<static_cast<unsigned>((nbar + 255) / 256), 256, 0, s>>>(
       ticket.ptr, abort_flag.ptr, t_start.ptr, bars.ptr, nbar);
   SPINED_CHECK_LAUNCH();

@@ -221,6 +221,28 @@ __device__ inline double row_pass_a(const OpArgs &o, const BondArgs &bd, uint64_
   return diag;
 }
 
+// Target index of bond j's move in a symmetry-free weight sector (rank
+// index), from the row's own index i.  When the bond joins sites a and a + 1
+// one up spin moves across it, and the colex rank sum_k C(p_k, k + 1)
+// changes only in that spin's term: + C(a, c) moving up, - C(a, c) moving
+// down (Pascal's rule), c = up spins below a.  False, k untouched, for
+// sites further apart and, with the spin flip, for a move into the top site
+// (the target's representative is then its complement); those take the
+// orbit and the full rank.  The target is a sector state (the row is, the
+// move keeps the weight and the top site down), so k is its index.
+__device__ inline bool rank_hop(const IndexArgs &ix, const GenericOrbitArgs &ga,
+                                const BondArgs &bd, uint64_t r, uint64_t i, int j, int64_t &k) {
+  const uint32_t st = bd.sites[j];
+  const int s0 = static_cast<int>(st & 0xffu), s1 = static_cast<int>(st >> 8);
+  const int a = s0 < s1 ? s0 : s1, b = s0 < s1 ? s1 : s0;
+  if (b != a + 1 || (ga.has_flip && b == ga.n_sites - 1)) return false;
+  const int c = __builtin_popcountll(r & ((1ull << a) - 1));
+  const int64_t cac = c == 0 ? 1 : static_cast<int64_t>(ix.rank_tab[(c - 1) * 64 + a]);
+  const int64_t t = static_cast<int64_t>(i) + (((r >> a) & 1) ? cac : -cac);
+  k = t >= 0 && static_cast<uint64_t>(t) < ix.rank_dim ? t : -1;
+  return true;
+}
+
 // One row of the real row gather (shared by the single and the batched
 // matvec): diagonal + active templates, two orbit scans / searches / gathers
 // in flight per iteration, accumulation in template order.
@@ -255,6 +277,30 @@ __device__ inline double gather_row_real(const OpArgs &o, const BondArgs &bd, co
   while (any_set<NW>(act)) {
     const int ja = pop_first<NW>(act);
     const int jb = pop_first<NW>(act);
+    if constexpr (METHOD == 0 && BONDS) {
+      if (ix.rank_mode == 1) {
+        // symmetry-free weight sector: nearest-neighbour moves from the
+        // row's index (rank_hop), the rest through the orbit and the rank
+        // (the identity, or the flip, reaches the target once: count 1)
+        int64_t ka = -1, kb = -1;
+        OrbitResult oa{0, 0, 1, true}, ob{0, 0, 1, true};
+        const bool fa = rank_hop(ix, ga, bd, r, i, ja, ka);
+        const bool fb = jb < 0 || rank_hop(ix, ga, bd, r, i, jb, kb);
+        if (!fa || !fb) {
+          if (!fa) oa = rob.orbit(o, bd, ga, ra, r, ja);
+          if (!fb) ob = rob.orbit(o, bd, ga, ra, r, jb);
+          int64_t k1 = -1, k2 = -1;
+          rank_of2(ix, oa.rep, !fa && oa.valid, ob.rep, !fb && ob.valid, k1, k2);
+          if (!fa) ka = k1;
+          if (!fb) kb = k2;
+        }
+        const double xa = ka >= 0 ? ld_re(x, static_cast<uint64_t>(ka)) : 0.0;
+        const double xb = kb >= 0 ? ld_re(x, static_cast<uint64_t>(kb)) : 0.0;
+        if (ka >= 0) acc = fma(coef_of(ja, oa), xa, acc);
+        if (kb >= 0) acc = fma(coef_of(jb, ob), xb, acc);
+        continue;
+      }
+    }
     const OrbitResult oa = rob.orbit(o, bd, ga, ra, r, ja);
     OrbitResult ob{0, 0, 0, false};
     if (jb >= 0) ob = rob.orbit(o, bd, ga, ra, r, jb);

@@ -21,7 +21,7 @@ __device__ inline void jacobi_eigh_device(const double *__restrict__ T, int ldt,
   __shared__ double V[MAXM][MAXM + 1];
   __shared__ double cs[MAXM / 2], sn[MAXM / 2];
   __shared__ int pp[MAXM / 2], qq[MAXM / 2];
-  __shared__ double red[256];
+  __shared__ double red[2][4]; // per-wave sums of (off, dia)
   __shared__ int done;
   const int tid = threadIdx.x;
   const int mp = m + (m & 1);
@@ -44,21 +44,18 @@ __device__ inline void jacobi_eigh_device(const double *__restrict__ T, int ldt,
       else
         off += a * a;
     }
-    red[tid] = off;
-    __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-      if (tid < s) red[tid] += red[tid + s];
-      __syncthreads();
+    // wave sums, then the four waves in order (256 threads)
+    for (int off2 = 32; off2 > 0; off2 >>= 1) {
+      off += __shfl_down(off, off2, 64);
+      dia += __shfl_down(dia, off2, 64);
     }
-    const double off_tot = red[0];
-    __syncthreads();
-    red[tid] = dia;
-    __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-      if (tid < s) red[tid] += red[tid + s];
-      __syncthreads();
+    if ((tid & 63) == 0) {
+      red[0][tid >> 6] = off;
+      red[1][tid >> 6] = dia;
     }
-    const double dia_tot = red[0];
+    __syncthreads();
+    const double off_tot = (red[0][0] + red[0][1]) + (red[0][2] + red[0][3]);
+    const double dia_tot = (red[1][0] + red[1][1]) + (red[1][2] + red[1][3]);
     if (tid == 0) done = off_tot <= 1e-30 * (dia_tot + off_tot) || off_tot == 0.0;
     __syncthreads();
     if (done) break;

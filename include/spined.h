@@ -36,6 +36,29 @@ typedef enum {
   SPINED_START_LEARNED = 2, /* amplitudes from the periodic CNN in `model` */
 } spined_start;
 
+typedef struct spined_monitor_info {
+  int64_t matvecs;
+  int32_t restarts;
+  int32_t phase;            /* 0 = FP64, 1 = FP32-storage phase */
+  int32_t nconverged;
+  int32_t k;
+  const double *ritz_values;   /* k values */
+  const double *residuals;     /* k residual estimates */
+} spined_monitor_info;
+
+/* Called once per restart cycle (mirrors PrimmeMonitor, SpinED.hs:379-382).
+ * Return non-zero to stop. */
+typedef int (*spined_monitor_fn)(const spined_monitor_info *info, void *ctx);
+
+/* Checkpoint (no counterpart in the reference, which creates /_workspace
+ * and leaves it empty): after every checkpoint_every-th thick restart the
+ * solver passes the lowest Ritz vector of its current basis, on the device
+ * and of unit norm (valid during the call only): dtype LS_FLOAT32 in the
+ * FP32-storage phase, else LS_FLOAT64 (LS_COMPLEX128 for complex sectors),
+ * n elements.  Handing it back as start_vector resumes the solve. */
+typedef void (*spined_checkpoint_fn)(const void *ritz_device, int dtype, uint64_t n,
+                                     const spined_monitor_info *info, void *ctx);
+
 typedef struct spined_eigsh_options {
   int32_t precision;        /* spined_precision */
   int32_t start;            /* spined_start */
@@ -53,21 +76,10 @@ typedef struct spined_eigsh_options {
   const double *start_vector;
   const void *model;        /* spined_cnn* for SPINED_START_LEARNED */
   uint64_t start_sign_mask; /* SPINED_START_LEARNED: Marshall sublattice mask */
+  spined_checkpoint_fn checkpoint; /* NULL = none */
+  void *checkpoint_ctx;
+  int32_t checkpoint_every; /* restarts between checkpoints; 0 = none */
 } spined_eigsh_options;
-
-typedef struct spined_monitor_info {
-  int64_t matvecs;
-  int32_t restarts;
-  int32_t phase;            /* 0 = FP64, 1 = FP32-storage phase */
-  int32_t nconverged;
-  int32_t k;
-  const double *ritz_values;   /* k values */
-  const double *residuals;     /* k residual estimates */
-} spined_monitor_info;
-
-/* Called once per restart cycle (mirrors PrimmeMonitor, SpinED.hs:379-382).
- * Return non-zero to stop. */
-typedef int (*spined_monitor_fn)(const spined_monitor_info *info, void *ctx);
 
 typedef struct spined_stats {
   int64_t matvecs;          /* total H*v applications */

@@ -508,6 +508,15 @@ template <class VT, bool C = false> struct Phase {
   unsigned G;
   bool small;
   int mlast = 0; // size of the last projected problem (Ritz assembly)
+  // checkpoint after every cp_every-th restart (spined_eigsh_options)
+  spined_checkpoint_fn cp_fn = nullptr;
+  void *cp_ctx = nullptr;
+  int cp_every = 0;
+  void set_checkpoint(const spined_eigsh_options &o) {
+    cp_fn = o.checkpoint;
+    cp_ctx = o.checkpoint_ctx;
+    cp_every = o.checkpoint_every;
+  }
 
   Phase(Operator &o, uint64_t n_, int m_, int k_, int keep_, hipStream_t s_)
       : op(o), n(n_), ne(C ? 2 * n_ : n_), m(m_), k(k_), keep(keep_), s(s_) {
@@ -789,6 +798,15 @@ template <class VT, bool C = false> struct Phase {
       reset_projected_kernel<<<1, 256, 0, s>>>(T.ptr, kMaxBasis, theta.ptr, keep);
       SPINED_CHECK_LAUNCH();
       j0 = keep;
+      // V_0 now holds the lowest Ritz vector
+      if (cp_fn && cp_every > 0 && res.cycles % cp_every == 0) {
+        SPINED_HIP(hipStreamSynchronize(s));
+        spined_monitor_info info{matvecs, res.cycles, phase_id, nconv, k, res.theta.data(),
+                                 res.est.data()};
+        cp_fn(vecs[0].ptr,
+              C ? LS_COMPLEX128 : (std::is_same<VT, float>::value ? LS_FLOAT32 : LS_FLOAT64), n,
+              &info, cp_ctx);
+      }
     }
   }
 
@@ -973,6 +991,7 @@ bool final_phase(Operator &op, uint64_t n, int m, int keep, uint64_t k, double t
   const uint64_t ne = C ? 2 * n : n;
   const VecType vt = C ? VecType::C128 : VecType::F64;
   Phase<double, C> p(op, n, m, static_cast<int>(k), keep, s);
+  p.set_checkpoint(opts);
   SPINED_HIP(hipMemcpyAsync(p.w.ptr, start.ptr, ne * 8, hipMemcpyDeviceToDevice, s));
   p.set_start_from_w();
   start.release();
@@ -1208,6 +1227,7 @@ int eigsh(Operator &op, int dtype, uint64_t k, double tol, const spined_eigsh_op
     // (sum of the k lowest) to the FP64 phase.
     const auto ta = std::chrono::steady_clock::now();
     Phase<float> p(op, n, m, static_cast<int>(k), keep, s);
+    p.set_checkpoint(opts);
     if (opts.verbose)
       log_line("FP32 phase: %d vectors allocated in %.3f s (since the start vector %.3f s)", m + 1,
                seconds_since(ta), seconds_since(ts) - st.seconds_start);

@@ -63,6 +63,23 @@ class SpinedAutotuneReport(C.Structure):
     ]
 
 
+class SpinedMonitorInfo(C.Structure):
+    _fields_ = [
+        ("matvecs", C.c_int64),
+        ("restarts", C.c_int32),
+        ("phase", C.c_int32),
+        ("nconverged", C.c_int32),
+        ("k", C.c_int32),
+        ("ritz_values", C.POINTER(C.c_double)),
+        ("residuals", C.POINTER(C.c_double)),
+    ]
+
+
+# spined_checkpoint_fn(ritz_device, dtype, n, info, ctx)
+CHECKPOINT_FN = C.CFUNCTYPE(None, C.c_void_p, C.c_int, C.c_uint64, C.POINTER(SpinedMonitorInfo),
+                            C.c_void_p)
+
+
 class SpinedEigshOptions(C.Structure):
     _fields_ = [
         ("precision", C.c_int32),
@@ -76,18 +93,9 @@ class SpinedEigshOptions(C.Structure):
         ("start_vector", C.c_void_p),
         ("model", C.c_void_p),
         ("start_sign_mask", C.c_uint64),
-    ]
-
-
-class SpinedMonitorInfo(C.Structure):
-    _fields_ = [
-        ("matvecs", C.c_int64),
-        ("restarts", C.c_int32),
-        ("phase", C.c_int32),
-        ("nconverged", C.c_int32),
-        ("k", C.c_int32),
-        ("ritz_values", C.POINTER(C.c_double)),
-        ("residuals", C.POINTER(C.c_double)),
+        ("checkpoint", CHECKPOINT_FN),
+        ("checkpoint_ctx", C.c_void_p),
+        ("checkpoint_every", C.c_int32),
     ]
 
 

@@ -29,6 +29,13 @@ BASIS_PATH = "/basis"
 HAMILTONIAN_PATH = "/hamiltonian"
 OBSERVABLES_PATH = "/observables"
 WORKSPACE_PATH = "/_workspace"
+# Lanczos checkpoint (no counterpart in the reference, which creates
+# /_workspace and leaves it empty): the lowest Ritz vector of the current
+# basis, its Ritz values and the matvecs so far, rewritten every
+# checkpoint_every restarts; a later run on the same output file starts
+# from it (ground-state runs, real sectors)
+CHECKPOINT_PATH = WORKSPACE_PATH + "/lanczos"
+DEFAULT_CHECKPOINT_EVERY = 8
 
 
 @dataclass
@@ -121,9 +128,22 @@ def with_datatype(is_real: bool, datatype: str):
     return np.complex64 if datatype == "float32" else np.complex128
 
 
-def diagonalize(cfg: UserConfig, dtype, model=None):
+def read_checkpoint(cfg: UserConfig, dim: int):
+    """The start vector of an earlier run's checkpoint, or None."""
+    with hdf5.File(cfg.output, "a") as f:
+        if not f.exists(CHECKPOINT_PATH + "/start_vector"):
+            return None
+        v = f.read(CHECKPOINT_PATH + "/start_vector", np.float64)
+    return v if v.shape == (dim,) and np.all(np.isfinite(v)) and np.any(v) else None
+
+
+def diagonalize(cfg: UserConfig, dtype, model=None,
+                checkpoint_every: int = DEFAULT_CHECKPOINT_EVERY):
     """diagonalize (SpinED.hs:370-411): lowest number_vectors eigenpairs,
-    written to /hamiltonian/{eigenvalues, eigenvectors, residuals}."""
+    written to /hamiltonian/{eigenvalues, eigenvectors, residuals}; with
+    checkpoint_every > 0 the solver's lowest Ritz vector goes to
+    /_workspace/lanczos every checkpoint_every restarts, and a checkpoint
+    found there (same dimension, one eigenpair, real sector) is the start."""
     dtype = np.dtype(dtype)
     log.debug("Using max_primme_basis_size=%d, max_primme_block_size=%d, "
               "min_primme_restart_size=%d", cfg.max_basis_size, cfg.max_block_size,
@@ -135,15 +155,31 @@ def diagonalize(cfg: UserConfig, dtype, model=None):
                  info["ritz_values"], info["residuals"])
         return False
 
+    real = np.dtype(dtype).kind != "c"
+    dim = I.get_number_states(cfg.basis)
     start_vector = None
-    if cfg.spec.start_vector == "learned":
+    resumed = read_checkpoint(cfg, dim) if real and cfg.num_evals == 1 else None
+    if resumed is not None:
+        log.info("Resuming from the checkpoint in %s...", CHECKPOINT_PATH)
+        start_vector = resumed
+    elif cfg.spec.start_vector == "learned":
         from . import learned
         start_vector = learned.start_vector_for(cfg, model)
+
+    def write_checkpoint(v, info):
+        with hdf5.File(cfg.output, "a") as f:
+            f.write(CHECKPOINT_PATH + "/start_vector", v.astype(np.float64), warn=False)
+            f.write(CHECKPOINT_PATH + "/ritz_values", np.asarray(info["ritz_values"]), warn=False)
+            f.write(CHECKPOINT_PATH + "/matvecs", np.array([info["matvecs"]], np.int64),
+                    warn=False)
+
     res = eigsh(cfg.hamiltonian.obj, k=cfg.num_evals, tol=cfg.eps if cfg.eps > 0 else 1e-10,
                 precision=cfg.spec.solver_precision, seed=cfg.spec.seed,
                 start_vector=start_vector, max_basis_size=cfg.max_basis_size,
                 min_restart_size=cfg.min_restart_size, max_block_size=cfg.max_block_size,
-                dtype=dtype, monitor=monitor)
+                dtype=dtype, monitor=monitor,
+                checkpoint=write_checkpoint if real and checkpoint_every > 0 else None,
+                checkpoint_every=checkpoint_every)
     if res.status != 0:
         log.warning("eigensolver stopped before reaching the tolerance: residuals %s", res.rnorms)
     log.info("Obtained eigenvalues %s", res.evals)
@@ -155,6 +191,14 @@ def diagonalize(cfg: UserConfig, dtype, model=None):
         # KnownDataset (Block a): the column-major D x k block as a row-major (k, D) matrix
         f.write(HAMILTONIAN_PATH + "/eigenvectors", np.ascontiguousarray(res.evecs.T))
         f.write(HAMILTONIAN_PATH + "/residuals", rnorms)
+        if real and checkpoint_every > 0 and res.status == 0:
+            # the converged vector: a later run on this file starts from it
+            f.write(CHECKPOINT_PATH + "/start_vector", res.evecs[:, 0].astype(np.float64),
+                    warn=False)
+            f.write(CHECKPOINT_PATH + "/ritz_values", np.asarray(res.evals), warn=False)
+            f.write(CHECKPOINT_PATH + "/matvecs", np.array([res.stats["matvecs"]], np.int64),
+                    warn=False)
+    diagonalize.last = dict(matvecs=int(res.stats["matvecs"]), resumed=resumed is not None)
     return evals, res.evecs, rnorms
 
 
@@ -172,7 +216,8 @@ def compute_expectations(cfg: UserConfig, evecs: np.ndarray) -> None:
             f.write(OBSERVABLES_PATH + "/" + o.name, m.astype(np.complex128))
 
 
-def run_app(path: str, debug: bool = False) -> dict:
+def run_app(path: str, debug: bool = False,
+            checkpoint_every: int = DEFAULT_CHECKPOINT_EVERY) -> dict:
     """runApp (app/Main.hs:14-27)."""
     if debug:
         I._lib.load().ls_enable_logging()
@@ -182,10 +227,11 @@ def run_app(path: str, debug: bool = False) -> dict:
     build_representatives(cfg)
     is_real = I.is_operator_real(cfg.hamiltonian.obj)
     dtype = with_datatype(is_real, cfg.datatype)
-    evals, evecs, rnorms = diagonalize(cfg, dtype)
+    evals, evecs, rnorms = diagonalize(cfg, dtype, checkpoint_every=checkpoint_every)
     compute_expectations(cfg, evecs)
     return dict(evals=evals, rnorms=rnorms, seconds=time.perf_counter() - t0,
-                dimension=I.get_number_states(cfg.basis), output=cfg.output)
+                dimension=I.get_number_states(cfg.basis), output=cfg.output,
+                **diagonalize.last)
 
 
 def main(argv=None) -> int:
@@ -195,12 +241,15 @@ def main(argv=None) -> int:
         description="Exact diagonalization of spin systems on an AMD Instinct GPU. Takes a YAML "
                     "file describing the basis, Hamiltonian and observables.")
     p.add_argument("--debug", action="store_true", help="Enable debug output from libspined")
+    p.add_argument("--checkpoint-every", type=int, default=DEFAULT_CHECKPOINT_EVERY,
+                   help="Write the Lanczos checkpoint to /_workspace/lanczos every N restarts "
+                        "(0: never); a later run on the same output file resumes from it")
     p.add_argument("-V", "--version", action="version", version="spined 0.1")
     p.add_argument("input_file", help="Input yaml file")
     a = p.parse_args(argv)
     logging.basicConfig(level=logging.DEBUG if a.debug else logging.INFO,
                         format="[%(asctime)s] %(levelname)s %(message)s")
-    out = run_app(a.input_file, a.debug)
+    out = run_app(a.input_file, a.debug, a.checkpoint_every)
     log.info("done in %.3f s", out["seconds"])
     return 0
 

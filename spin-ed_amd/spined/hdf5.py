@@ -110,6 +110,8 @@ def _mem_and_file_type(dtype):
     dtype = np.dtype(dtype)
     if dtype == np.uint64:
         return _type("H5T_NATIVE_ULLONG_g"), _type("H5T_STD_U64LE_g"), None
+    if dtype == np.int64:
+        return _type("H5T_NATIVE_LLONG_g"), _type("H5T_STD_I64LE_g"), None
     if dtype == np.float64:
         return _type("H5T_NATIVE_DOUBLE_g"), _type("H5T_IEEE_F64LE_g"), None
     if dtype == np.float32:
@@ -166,21 +168,26 @@ class File:
         return True
 
     def create_group(self, path: str) -> None:
-        """createGroupWhenNeeded (SpinED.hs:279-282)."""
-        if not self.exists(path):
-            g = _check(lib().H5Gcreate2(self.fid, path.encode(), H5P_DEFAULT, H5P_DEFAULT,
-                                        H5P_DEFAULT), "create group")
-            lib().H5Gclose(g)
+        """createGroupWhenNeeded (SpinED.hs:279-282), each missing level."""
+        cur = ""
+        for q in [q for q in path.strip("/").split("/") if q]:
+            cur += "/" + q
+            if not self.exists(cur):
+                g = _check(lib().H5Gcreate2(self.fid, cur.encode(), H5P_DEFAULT, H5P_DEFAULT,
+                                            H5P_DEFAULT), "create group")
+                lib().H5Gclose(g)
 
-    def write(self, path: str, data: np.ndarray) -> None:
-        """writeDataset (SpinED.hs:354-360): overwrite with a warning."""
+    def write(self, path: str, data: np.ndarray, warn: bool = True) -> None:
+        """writeDataset (SpinED.hs:354-360): overwrite with a warning (warn=False:
+        silently, for the solver's periodic checkpoints)."""
         L = lib()
         data = np.ascontiguousarray(data)
         parent = path.rsplit("/", 1)[0]
         if parent:
             self.create_group(parent)
         if self.exists(path):
-            log.warning("Overwriting %s...", path)
+            if warn:
+                log.warning("Overwriting %s...", path)
             L.H5Ldelete(self.fid, path.encode(), H5P_DEFAULT)
         mt, ft, owned = _mem_and_file_type(data.dtype)
         dims = (C.c_uint64 * max(data.ndim, 1))(*(data.shape or (1,)))

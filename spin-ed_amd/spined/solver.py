@@ -29,7 +29,13 @@ def eigsh(op: Operator_, k: int = 1, tol: float = 1e-10, *, precision: str = "fp
           max_basis_size: int = 0, min_restart_size: int = 0, max_block_size: int = 0,
           max_matvecs: int = 0, want_vectors: bool = True, dtype=np.float64,
           verbose: bool = False, monitor: Optional[Callable[[dict], bool]] = None,
-          model=None, sign_mask: int = 0) -> EigshResult:
+          model=None, sign_mask: int = 0,
+          checkpoint: Optional[Callable[[np.ndarray, dict], None]] = None,
+          checkpoint_every: int = 0) -> EigshResult:
+    """... checkpoint(v, info) receives, after every checkpoint_every-th
+    restart, the lowest Ritz vector of the current basis as a host array
+    (float32 in the FP32-storage phase, else float64, complex128 for complex
+    sectors) and the monitor's info; start_vector=v resumes from it."""
     L = _lib.load()
     D = get_number_states(op.basis)
     opts = _lib.SpinedEigshOptions()
@@ -74,6 +80,21 @@ def eigsh(op: Operator_, k: int = 1, tol: float = 1e-10, *, precision: str = "fp
         return 1 if monitor(d) else 0
 
     cb = _lib.MONITOR_FN(_mon)
+
+    def _cp(ptr, code, n, info_p, _ctx):
+        dt = {0: np.float32, 1: np.float64, 3: np.complex128}[int(code)]
+        v = np.empty(int(n), dtype=dt)
+        check_status(L.spined_memcpy(v.ctypes.data, ptr, v.nbytes, 1))
+        info = info_p.contents
+        checkpoint(v, dict(matvecs=info.matvecs, restarts=info.restarts, phase=info.phase,
+                           nconverged=info.nconverged,
+                           ritz_values=[info.ritz_values[i] for i in range(info.k)],
+                           residuals=[info.residuals[i] for i in range(info.k)]))
+
+    cpf = _lib.CHECKPOINT_FN(_cp) if checkpoint is not None else _lib.CHECKPOINT_FN()
+    if checkpoint is not None and checkpoint_every > 0:
+        opts.checkpoint = cpf
+        opts.checkpoint_every = checkpoint_every
     status = L.spined_eigsh(op.ptr, code, k, tol, C.byref(opts), evals.ctypes.data,
                             evecs.ctypes.data if evecs is not None else None, D,
                             rn.ctypes.data, cb, None)

@@ -122,12 +122,17 @@ def test_hdf5_round_trip(tmp_path):
         f.write("/observables/c", c128)
         f.write("/observables/c64", c64)
         f.write("/hamiltonian/eigenvalues", ev64 * 2)  # overwrite with a warning
+        # nested groups made level by level; int64; a silent overwrite
+        f.write("/_workspace/lanczos/matvecs", np.array([7], np.int64))
+        f.write("/_workspace/lanczos/matvecs", np.array([-3, 2 ** 40], np.int64), warn=False)
     with hdf5.File(p, "r") as f:
         assert f.exists("/basis/representatives") and not f.exists("/basis/missing")
         np.testing.assert_array_equal(f.read("/basis/representatives", np.uint64), reps)
         np.testing.assert_array_equal(f.read("/hamiltonian/eigenvalues", np.float64), ev64 * 2)
         np.testing.assert_array_equal(f.read("/hamiltonian/residuals", np.float32), ev32)
         np.testing.assert_array_equal(f.read("/observables/c", np.complex128), c128)
+        np.testing.assert_array_equal(f.read("/_workspace/lanczos/matvecs", np.int64),
+                                      np.array([-3, 2 ** 40], np.int64))
         np.testing.assert_array_equal(f.read("/observables/c64", np.complex64), c64)
         # (k, D) on disk, as SpinED writes the column-major block
         np.testing.assert_array_equal(hdf5.read_eigenvectors(f, 7), vec.T)

@@ -134,3 +134,83 @@ output: {out}
     assert abs(ev[0] - ref) <= 1e-10 * abs(ref)
     v = v.reshape(-1)
     assert np.linalg.norm(o.matvec(v) - ev[0] * v) <= 1e-9
+
+
+def test_solver_checkpoints_and_resume():
+    """spined_eigsh's checkpoint: after every restart the lowest Ritz
+    vector of the current basis, unit norm, whose Rayleigh quotient is the
+    reported Ritz value; a solve started from a mid-run checkpoint reaches
+    the same E0 with fewer matvecs than the full run.  In the mixed solver
+    the FP32-storage phase hands over float32 vectors, the FP64 phase
+    float64 ones."""
+    from helpers import engine, ring
+    spec = ring(24)
+    _, op = engine(spec)
+    seen = []
+    full = spined.eigsh(op, k=1, tol=1e-10, checkpoint=lambda v, info: seen.append((v, info)),
+                        checkpoint_every=1)
+    assert full.status == 0 and len(seen) >= 2
+    assert [info["restarts"] for _, info in seen] == list(range(1, len(seen) + 1))
+    for v, info in seen:
+        assert v.dtype == np.float64
+        assert abs(np.linalg.norm(v) - 1.0) <= 1e-8
+        rq = float(np.dot(v, spined.apply(op, v)))
+        assert abs(rq - info["ritz_values"][0]) <= 1e-8 * abs(rq)
+    v, info = seen[len(seen) // 2]
+    res = spined.eigsh(op, k=1, tol=1e-10, start_vector=v)
+    assert res.status == 0
+    assert abs(res.evals[0] - full.evals[0]) <= 1e-10 * abs(full.evals[0])
+    assert res.stats["matvecs"] < full.stats["matvecs"]
+    mixed = []
+    r = spined.eigsh(op, k=1, tol=1e-10, precision="mixed",
+                     checkpoint=lambda v, info: mixed.append((v.dtype, info["phase"])),
+                     checkpoint_every=1)
+    assert r.status == 0 and any(ph == 1 for _, ph in mixed)
+    # FP32-storage phase: float32; FP64 phase (if it restarts at all): float64
+    assert all((dt == np.dtype(np.float32)) == (ph == 1) for dt, ph in mixed)
+
+
+@pytest.mark.skipif(not hdf5.available(), reason="libhdf5 not found")
+def test_app_workspace_checkpoint_resume(tmp_path):
+    """The application writes its Lanczos checkpoint to /_workspace/lanczos
+    (the converged vector at the end); a second run on the same file starts
+    from it and needs only a few matvecs for the same E0."""
+    out = tmp_path / "ring24.h5"
+    deck = tmp_path / "ring24.yaml"
+    perm = [(i + 1) % 24 for i in range(24)]
+    refl = [23 - i for i in range(24)]
+    deck.write_text(f"""
+basis:
+  number_spins: 24
+  hamming_weight: 12
+  spin_inversion: 1
+  symmetries:
+    - permutation: {perm}
+      sector: 0
+    - permutation: {refl}
+      sector: 0
+hamiltonian:
+  name: H
+  terms:
+    - matrix: [[1, 0, 0, 0], [0, -1, 2, 0], [0, 2, -1, 0], [0, 0, 0, 1]]
+      sites: {[[i, (i + 1) % 24] for i in range(24)]}
+observables: []
+number_vectors: 1
+precision: 1e-10
+output: {out}
+""")
+    r = app.run_app(str(deck), checkpoint_every=2)
+    assert not r["resumed"]
+    D = r["dimension"]
+    with hdf5.File(out, "r") as f:
+        v = f.read("/_workspace/lanczos/start_vector", np.float64)
+        rv = f.read("/_workspace/lanczos/ritz_values", np.float64)
+        mv = f.read("/_workspace/lanczos/matvecs", np.int64)
+    assert v.shape == (D,) and abs(np.linalg.norm(v) - 1.0) <= 1e-8
+    np.testing.assert_allclose(rv[0], r["evals"][0], rtol=1e-12)
+    assert mv[0] == r["matvecs"]
+    r2 = app.run_app(str(deck), checkpoint_every=2)
+    assert r2["resumed"]
+    assert r2["matvecs"] < r["matvecs"] // 2
+    np.testing.assert_allclose(r2["evals"], r["evals"], rtol=1e-10)
+    assert abs(r["evals"][0] - (-42.680058066148944)) <= 1e-10 * 42.68   # Bethe ansatz, N = 24

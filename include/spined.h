@@ -72,7 +72,7 @@ typedef struct spined_eigsh_options {
                                blocks of that width (block matvec), then the
                                one-vector FP64 phase finishes */
   int32_t verbose;          /* 1 = one log line per restart on stderr */
-  int64_t max_matvecs;      /* 0 = auto */
+  int64_t max_matvecs;      /* 0 = auto: 20,000 (LS_NOT_CONVERGED beyond) */
   const double *start_vector;
   const void *model;        /* spined_cnn* for SPINED_START_LEARNED */
   uint64_t start_sign_mask; /* SPINED_START_LEARNED: Marshall sublattice mask */

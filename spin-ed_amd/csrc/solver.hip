@@ -192,6 +192,7 @@ constexpr int kFuseMax = 32; // vectors held in registers by the fused passes
 // The second Gram-Schmidt pass runs only when the first update keeps less
 // than this fraction of |w| (second_pass_test_kernel).
 constexpr double kSecondPassEta = 1e-2;
+constexpr int64_t kAutoMaxMatvecs = 20000; // spined_eigsh_options.max_matvecs = 0
 
 // Block reduction of acc[q] (q < nq) into partial[q * gridDim.x + blockIdx.x]
 // with the multidot kernel's order (wave shuffles, then the 4 waves in
@@ -1114,10 +1115,14 @@ void dense_small_eigsh(Operator &op, uint64_t n, uint64_t k, int dtype, double *
   }
 }
 
-int eigsh(Operator &op, int dtype, uint64_t k, double tol, const spined_eigsh_options &opts,
+int eigsh(Operator &op, int dtype, uint64_t k, double tol, const spined_eigsh_options &opts_in,
           double *evals, void *evecs, uint64_t ld, double *rnorms, spined_monitor_fn monitor,
           void *ctx) {
   const auto t0 = std::chrono::steady_clock::now();
+  // max_matvecs 0 = automatic: a cap far beyond any solve that converges,
+  // so that one which stops converging returns LS_NOT_CONVERGED
+  spined_eigsh_options opts = opts_in;
+  if (opts.max_matvecs <= 0) opts.max_matvecs = kAutoMaxMatvecs;
   Basis &b = *op.basis;
   const uint64_t n = b.D;
   if (k == 0 || k > n) throw Error(LS_INVALID_ARGUMENT, "number of eigenpairs must be in [1, dim]");

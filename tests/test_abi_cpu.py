@@ -219,3 +219,35 @@ def test_hop_orbit_matches_full_scan(n, w, z, gens):
     v = f[3]
     np.testing.assert_array_equal(h[1][v], f[1][v])
     assert len(states) > 0
+
+
+def test_eigsh_options_layout_matches_header():
+    """The ctypes mirror of spined_eigsh_options has the C layout (size and
+    the offsets of the fields a host sets through pokeByteOff), compiled
+    from include/spined.h with the system C compiler."""
+    import ctypes as C
+    import shutil
+    import subprocess
+    import tempfile
+    from pathlib import Path
+    from spined import _lib
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        pytest.skip("no C compiler")
+    inc = Path(__file__).resolve().parent.parent / "include"
+    fields = [f for f, _ in _lib.SpinedEigshOptions._fields_]
+    src = "#include <stddef.h>\n#include <stdio.h>\n#include \"spined.h\"\nint main(void) {\n"
+    src += '  printf("%zu\\n", sizeof(spined_eigsh_options));\n'
+    for f in fields:
+        src += f'  printf("%zu\\n", offsetof(spined_eigsh_options, {f}));\n'
+    src += "  return 0;\n}\n"
+    with tempfile.TemporaryDirectory() as d:
+        c = Path(d) / "layout.c"
+        c.write_text(src)
+        exe = Path(d) / "layout"
+        subprocess.run([cc, f"-I{inc}", str(c), "-o", str(exe)], check=True)
+        out = [int(v) for v in subprocess.run([str(exe)], check=True, capture_output=True,
+                                               text=True).stdout.split()]
+    assert out[0] == C.sizeof(_lib.SpinedEigshOptions)
+    for f, off in zip(fields, out[1:]):
+        assert getattr(_lib.SpinedEigshOptions, f).offset == off, f

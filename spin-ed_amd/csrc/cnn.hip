@@ -67,23 +67,38 @@ __device__ inline float gelu_grad_f(float x) {
 }
 
 // The same tanh-form GELU without a transcendental, for the inference
-// kernel: x G(x) with G(x) = clamp(1/2 + x P(min(x^2, c^2)), 0, 1), P of
-// degree 5, c = 3.75 (a minimax fit of (GELU(x) - x/2) / x^2 on (0, c]);
-// beyond c the clamp gives 0 or x.  Its largest error over all x is
-// 5.1e-4 absolute (evaluated in FP32), below the BF16 resolution of the
-// activations it produces (2^-8 relative) wherever they exceed 0.13.
-// Nine VALU operations (the last fma clamps with its output modifier)
-// instead of the exponential and reciprocal, which run at a quarter of the
+// kernel: x G(x) with the gate G(x) = clamp(1/2 + x P(x^2), 0, 1), where P
+// is a polynomial with a positive leading coefficient, fitted (linear
+// program, minimax) to (GELU(x) - x/2) / x^2 on [0, c] under the
+// constraint x P(x^2) >= 1/2 on [c, 40]: beyond c the clamp gives x (and 0
+// for x <= -c), so y = x^2 needs no clamp of its own.  The last fma clamps
+// with its output modifier.  Largest error over all x, evaluated in FP32:
+//   degree 4, c = 3.4:  1.06e-3 absolute (7 VALU operations with the
+//     product), for layer 2, whose outputs are rounded to BF16 next
+//     (2^-8 relative, larger wherever they exceed 0.27);
+//   degree 6, c = 3.85: 1.5e-4 absolute (8 operations and the summing fma),
+//     for layer 3, whose outputs are summed over positions in FP32.
+// The exponential and reciprocal of the exact form run at a quarter of the
 // VALU rate and made the inference kernel VALU-bound.
-__device__ inline float gelu_poly_f(float x) {
-  const float y = fminf(x * x, 3.75f * 3.75f);
-  float p = -5.1259361e-07f;
-  p = fmaf(p, y, 2.775556728e-05f);
-  p = fmaf(p, y, -6.292571424e-04f);
-  p = fmaf(p, y, 7.911820231e-03f);
-  p = fmaf(p, y, -6.266994169e-02f);
-  p = fmaf(p, y, 3.963627676e-01f);
-  return x * __builtin_amdgcn_fmed3f(fmaf(x, p, 0.5f), 0.0f, 1.0f);
+__device__ inline float gelu_gate4(float x) {
+  const float y = x * x;
+  float p = 1.063038355e-05f;
+  p = fmaf(p, y, -4.166025794e-04f);
+  p = fmaf(p, y, 6.714980851e-03f);
+  p = fmaf(p, y, -5.973969748e-02f);
+  p = fmaf(p, y, 3.940491249e-01f);
+  return __builtin_amdgcn_fmed3f(fmaf(x, p, 0.5f), 0.0f, 1.0f);
+}
+__device__ inline float gelu_gate6(float x) {
+  const float y = x * x;
+  float p = 2.995527462e-08f;
+  p = fmaf(p, y, -1.956490401e-06f);
+  p = fmaf(p, y, 5.489364144e-05f);
+  p = fmaf(p, y, -8.792124575e-04f);
+  p = fmaf(p, y, 9.069545159e-03f);
+  p = fmaf(p, y, -6.511615272e-02f);
+  p = fmaf(p, y, 3.980636342e-01f);
+  return __builtin_amdgcn_fmed3f(fmaf(x, p, 0.5f), 0.0f, 1.0f);
 }
 
 __device__ inline floatx4 mfma(bf16x8 a, bf16x8 b, floatx4 c) {
@@ -330,22 +345,46 @@ __global__ void __launch_bounds__(64)
 }
 
 // Streaming inference (no stored pre-activations, n >= kInferMinN): the
-// forward kernel's outputs (same operands; MFMA sums that start from the
-// bias instead of adding it last) from 7 KB of LDS per wave instead of
-// 21 KB, so about three times the waves stay resident.
+// forward kernel's network (same BF16 operands; the polynomial GELU gates
+// in layers 2 and 3; MFMA sums that start from the bias instead of adding
+// it last) from 7.5 KB of LDS per wave instead of 21 KB, so about three
+// times the waves stay resident.
 //  * layer 1 is the table of its 32 input patterns; layer 2 reads its B
 //    fragments from the table directly, so layer 1 is never stored;
-//  * layer 2's outputs go to a ring of the last 8 positions (plus a copy of
-//    positions 0..3 for the wrap-around), and layer 3 at position p - 2
-//    runs as soon as layer 2 at p is done.
+//  * layer 2's outputs go to a ring of the last 8 positions (slot pos mod 8,
+//    slot 0 mirrored in slot 8, plus a copy of positions 0..3 for the
+//    wrap-around), and layer 3 at position p - 2 runs as soon as layer 2 at
+//    p is done;
+//  * the position loop is unrolled over the ring, so every ring address is
+//    a per-lane base plus a constant: no per-position address arithmetic
+//    beyond the two table lookups' pattern extraction.
 constexpr int kRing = 8;
 constexpr int kInferMinN = 9;
+constexpr int kSlotBytes = 16 * CH * 2; // one position of 16 samples x 16 channels
+
+// The layer-1 table of the inference kernel, tab[h][pattern][8 channels]:
+// channels 8h .. 8h + 7 of GELU(b1 + W1 h0) for each 5-spin pattern (bit k
+// of the pattern = spin at offset k - 2), 16 bytes per pattern, 1 KB in all.
+// Computed once per evaluation instead of once per wave.
+__global__ void __launch_bounds__(64) cnn_table_kernel(const float *__restrict__ P,
+                                                       __bf16 *__restrict__ tab) {
+  for (int e = threadIdx.x; e < 32 * CH; e += 64) {
+    const int pat = e / CH, ch = e % CH;
+    float v = 0.0f;
+    for (int k = 0; k < KW; ++k) {
+      const float wk = static_cast<float>(static_cast<__bf16>(P[OFF_W1 + ch * KW + k]));
+      v += ((pat >> k) & 1) ? wk : -wk;
+    }
+    tab[(ch >> 3) * 256 + pat * 8 + (ch & 7)] = static_cast<__bf16>(gelu_f(v + P[OFF_B1 + ch]));
+  }
+}
 
 __global__ void __launch_bounds__(64)
-    cnn_infer_kernel(CnnArgs c, const uint64_t *__restrict__ states, float *__restrict__ out_a,
-                     float *__restrict__ out_t) {
-  __shared__ __attribute__((aligned(16))) __bf16 tab[32 * CH];
-  __shared__ __attribute__((aligned(16))) __bf16 ring[kRing * 16 * CH];
+    cnn_infer_kernel(CnnArgs c, const uint4 *__restrict__ table, const uint64_t *__restrict__ states,
+                     float *__restrict__ out_a, float *__restrict__ out_t) {
+  // the layer-1 table (cnn_table_kernel's layout), one 16-byte load per lane
+  __shared__ __attribute__((aligned(16))) __bf16 tab[2 * 32 * 8];
+  __shared__ __attribute__((aligned(16))) __bf16 ring[(kRing + 1) * 16 * CH];
   __shared__ __attribute__((aligned(16))) __bf16 head[4 * 16 * CH];
   const int n = c.n;
   const int lane = threadIdx.x, s = lane & 15, g = lane >> 4;
@@ -353,15 +392,7 @@ __global__ void __launch_bounds__(64)
   const uint64_t sample = base + s;
   const uint64_t state = sample < c.count ? states[sample] : 0;
   const float *P = c.params;
-  for (int e = lane; e < 32 * CH; e += 64) {
-    const int pat = e / CH, ch = e % CH;
-    float v = 0.0f;
-    for (int k = 0; k < KW; ++k) {
-      const float wk = static_cast<float>(static_cast<__bf16>(P[OFF_W1 + ch * KW + k]));
-      v += ((pat >> k) & 1) ? wk : -wk;
-    }
-    tab[e] = static_cast<__bf16>(gelu_f(v + P[OFF_B1 + ch]));
-  }
+  reinterpret_cast<uint4 *>(tab)[lane] = table[lane];
   __syncthreads();
   bf16x8 a2[3], a3[3];
   load_conv_weights(P + OFF_W2, false, a2);
@@ -376,25 +407,28 @@ __global__ void __launch_bounds__(64)
   // channels c0 = k & 15 (0 or 8).  Rows k >= 80 (offset 5) meet zero
   // weights (load_conv_weights), so their lanes read the offset-4 data
   // instead of building zeros, and no fragment load needs a branch.
+  // Fragments 0 and 1 have d = 2q + hi, fragment 2 has d = 4 in every lane.
+  const int hi = g >> 1, h = g & 1;
   int dq[3];
 #pragma unroll
   for (int q = 0; q < 3; ++q) dq[q] = std::min((32 * q + 8 * g) >> 4, KW - 1);
-  const int c0 = (8 * g) & 15;
-  // cur: the state rotated right by pos - 4 (mod n, exactly n bits).  Layer
-  // 2's tap at offset d reads the layer-1 table at the 5-spin pattern around
-  // pos + d - 2, i.e. bits d .. d + 4 of cur (below n for n >= kInferMinN).
+  const int c0 = 8 * h;
+  const char *tabc = reinterpret_cast<const char *>(tab);
+  char *ringc = reinterpret_cast<char *>(ring);
+  const uint32_t hoff = 512u * h;                          // table half, bytes
+  const uint32_t rl = static_cast<uint32_t>(s * CH + c0) * 2; // ring read base
+  const uint32_t rlh = rl + hi * kSlotBytes;               // ... one slot up for d odd
+  const uint32_t wl = static_cast<uint32_t>(s * CH + 4 * g) * 2; // ring write base
+  // ext bit k = state bit (k - 4) mod n for k < 2n (n >= kInferMinN makes
+  // 2n >= n + 9): layer 2 at pos reads the layer-1 table at the 5-spin
+  // pattern around pos + d - 2, i.e. bits d .. d + 4 of win = ext >> pos.
   const uint64_t nmask = n == 64 ? ~0ull : ((1ull << n) - 1);
-  uint64_t cur = ((state >> (n - 4)) | (state << 4)) & nmask;
-  auto frag2 = [&](int q) {
-    const int pat = static_cast<int>((cur >> dq[q]) & 31u);
-    return *reinterpret_cast<const bf16x8 *>(tab + pat * CH + c0);
-  };
-  // layer 3 at position pos - 2 inside the loop (pos >= 4) reads layer 2 at
-  // pos + d - 4, which lies in [0, n): no wrap-around, always the ring
-  auto frag3_ring = [&](int pos, int q) {
-    const int p = (pos + dq[q] - 4) & (kRing - 1);
-    return *reinterpret_cast<const bf16x8 *>(ring + (p * 16 + s) * CH + c0);
-  };
+  const uint64_t rot = ((state >> (n - 4)) | (state << 4)) & nmask;
+  const uint64_t ext_lo = n == 64 ? rot : (rot | (rot << n));
+  const uint64_t ext_hi = n == 64 ? rot : (rot >> (64 - n));
+  const uint32_t e0 = static_cast<uint32_t>(ext_lo), e1 = static_cast<uint32_t>(ext_lo >> 32),
+                 e2 = static_cast<uint32_t>(ext_hi);
+  auto ld = [](const char *p) { return *reinterpret_cast<const bf16x8 *>(p); };
   // the last four layer-3 positions wrap around to the head copy of 0..3
   auto frag3_tail = [&](int pos, int q) {
     int p = pos + dq[q] - 2;
@@ -407,24 +441,36 @@ __global__ void __launch_bounds__(64)
   float f[4] = {0.f, 0.f, 0.f, 0.f};
   auto layer3_out = [&](floatx4 acc) {
 #pragma unroll
-    for (int r = 0; r < 4; ++r) f[r] += gelu_poly_f(acc[r]);
+    for (int r = 0; r < 4; ++r) f[r] = fmaf(acc[r], gelu_gate6(acc[r]), f[r]);
   };
   // (one wave per workgroup: its LDS accesses stay in program order)
-  for (int pos = 0; pos < n; ++pos) {
-    floatx4 acc = {b2[0], b2[1], b2[2], b2[3]};
+  for (int pos0 = 0; pos0 < n; pos0 += kRing) {
+    const uint32_t wlo = pos0 < 32 ? e0 : e1, whi = pos0 < 32 ? e1 : e2;
+    const int sh0 = pos0 & 31;
 #pragma unroll
-    for (int k = 0; k < 3; ++k) acc = mfma(a2[k], frag2(k), acc);
-    cur = (cur >> 1) | ((cur & 1u) << (n - 1));
-    bf16x4 o;
+    for (int u = 0; u < kRing; ++u) {
+      const int pos = pos0 + u;
+      if (pos >= n) break;
+      const uint32_t win = __builtin_amdgcn_alignbit(whi, wlo, sh0 + u);
+      floatx4 acc = {b2[0], b2[1], b2[2], b2[3]};
+      acc = mfma(a2[0], ld(tabc + ((__builtin_amdgcn_ubfe(win, dq[0], 5) << 4) | hoff)), acc);
+      acc = mfma(a2[1], ld(tabc + ((__builtin_amdgcn_ubfe(win, dq[1], 5) << 4) | hoff)), acc);
+      acc = mfma(a2[2], ld(tabc + ((win & 0x1f0u) | hoff)), acc);
+      bf16x4 o;
 #pragma unroll
-    for (int r = 0; r < 4; ++r) o[r] = static_cast<__bf16>(gelu_poly_f(acc[r]));
-    *reinterpret_cast<bf16x4 *>(ring + ((pos & (kRing - 1)) * 16 + s) * CH + 4 * g) = o;
-    if (pos < 4) *reinterpret_cast<bf16x4 *>(head + (pos * 16 + s) * CH + 4 * g) = o;
-    if (pos >= 4) {
-      floatx4 acc3 = {b3[0], b3[1], b3[2], b3[3]};
-#pragma unroll
-      for (int k = 0; k < 3; ++k) acc3 = mfma(a3[k], frag3_ring(pos, k), acc3);
-      layer3_out(acc3);
+      for (int r = 0; r < 4; ++r) o[r] = static_cast<__bf16>(acc[r] * gelu_gate4(acc[r]));
+      *reinterpret_cast<bf16x4 *>(ringc + wl + u * kSlotBytes) = o;
+      if (u == 0) *reinterpret_cast<bf16x4 *>(ringc + wl + kRing * kSlotBytes) = o;
+      if (pos < 4) *reinterpret_cast<bf16x4 *>(reinterpret_cast<char *>(head) + wl + pos * kSlotBytes) = o;
+      if (pos >= 4) {
+        // layer 3 at pos - 2 reads layer 2 at pos - 4 + d, slot (u - 4 + d) mod 8;
+        // d = 2q + hi for q < 2 (slot 8 stands for slot 0), d = 4 for q = 2
+        floatx4 acc3 = {b3[0], b3[1], b3[2], b3[3]};
+        acc3 = mfma(a3[0], ld(ringc + rlh + ((u - 4) & (kRing - 1)) * kSlotBytes), acc3);
+        acc3 = mfma(a3[1], ld(ringc + rlh + ((u - 2) & (kRing - 1)) * kSlotBytes), acc3);
+        acc3 = mfma(a3[2], ld(ringc + rl + u * kSlotBytes), acc3);
+        layer3_out(acc3);
+      }
     }
   }
   const int tail_pos[4] = {n - 2, n - 1, 0, 1};
@@ -743,8 +789,16 @@ size_t lds_bytes_backward(int n) { return static_cast<size_t>(3) * 16 * n * CH *
 
 struct CnnModel {
   DeviceBuffer<float> params, m, v, grad;
+  DeviceBuffer<uint4> table; // the inference kernel's layer-1 table (64 x 16 bytes)
   int64_t step = 0;
 };
+
+// the table from the current parameters, then the inference kernel
+inline void launch_infer(const CnnModel &m, const CnnArgs &a, unsigned blocks,
+                         const uint64_t *states, float *out_a, float *out_t, hipStream_t s) {
+  cnn_table_kernel<<<1, 64, 0, s>>>(a.params, reinterpret_cast<__bf16 *>(m.table.ptr));
+  cnn_infer_kernel<<<blocks, 64, 0, s>>>(a, m.table.ptr, states, out_a, out_t);
+}
 
 } // namespace spined
 
@@ -768,6 +822,7 @@ int spined_cnn_create(spined_cnn **out, const float *init_params) {
     c->m.m.allocate(spined::NPARAMS);
     c->m.v.allocate(spined::NPARAMS);
     c->m.grad.allocate(spined::NPARAMS);
+    c->m.table.allocate(64);
     SPINED_HIP(hipMemcpy(c->m.params.ptr, init_params, 4 * spined::NPARAMS, hipMemcpyHostToDevice));
     SPINED_HIP(hipMemset(c->m.m.ptr, 0, 4 * spined::NPARAMS));
     SPINED_HIP(hipMemset(c->m.v.ptr, 0, 4 * spined::NPARAMS));
@@ -801,7 +856,7 @@ int spined_cnn_forward_device(const spined_cnn *c, int n, uint64_t count, const 
     spined::CnnArgs a{c->m.params.ptr, n, count};
     const unsigned blocks = static_cast<unsigned>((count + 15) / 16);
     if (!z_store && n >= spined::kInferMinN)
-      spined::cnn_infer_kernel<<<blocks, 64, 0, spined::stream()>>>(a, states, out_a, out_t);
+      spined::launch_infer(c->m, a, blocks, states, out_a, out_t, spined::stream());
     else
       hipLaunchKernelGGL(spined::cnn_forward_kernel, dim3(blocks), dim3(64),
                          spined::lds_bytes_forward(n), spined::stream(), a, states, out_a, out_t,
@@ -890,7 +945,7 @@ void cnn_start_vector(const spined_cnn *c, const Basis &b, uint64_t marshall_mas
   CnnArgs args{c->m.params.ptr, b.n, D};
   const unsigned blocks = static_cast<unsigned>((D + 15) / 16);
   if (b.n >= kInferMinN)
-    cnn_infer_kernel<<<blocks, 64, 0, s>>>(args, b.reps.ptr, a.ptr, t.ptr);
+    launch_infer(c->m, args, blocks, b.reps.ptr, a.ptr, t.ptr, s);
   else
     hipLaunchKernelGGL(cnn_forward_kernel, dim3(blocks), dim3(64), lds_bytes_forward(b.n), s,
                        args, b.reps.ptr, a.ptr, t.ptr, static_cast<float *>(nullptr));

@@ -68,18 +68,16 @@ __device__ inline float gelu_grad_f(float x) {
 
 // The same tanh-form GELU without a transcendental, for the inference
 // kernel: x G(x) with the gate G(x) = clamp(1/2 + x P(x^2), 0, 1), where P
-// is a polynomial with a positive leading coefficient, fitted (linear
-// program, minimax) to (GELU(x) - x/2) / x^2 on [0, c] under the
-// constraint x P(x^2) >= 1/2 on [c, 40]: beyond c the clamp gives x (and 0
-// for x <= -c), so y = x^2 needs no clamp of its own.  The last fma clamps
-// with its output modifier.  Largest error over all x, evaluated in FP32:
-//   degree 4, c = 3.4:  1.06e-3 absolute (7 VALU operations with the
-//     product), for layer 2, whose outputs are rounded to BF16 next
-//     (2^-8 relative, larger wherever they exceed 0.27);
-//   degree 6, c = 3.85: 1.5e-4 absolute (8 operations and the summing fma),
-//     for layer 3, whose outputs are summed over positions in FP32.
-// The exponential and reciprocal of the exact form run at a quarter of the
-// VALU rate and made the inference kernel VALU-bound.
+// has degree 4 and a positive leading coefficient, fitted (linear program,
+// minimax) to (GELU(x) - x/2) / x^2 on [0, 3.4] under the constraint
+// x P(x^2) >= 1/2 on [3.4, 40]: beyond 3.4 the clamp gives x (and 0 for
+// x <= -3.4), so y = x^2 needs no clamp of its own.  Largest error over all
+// x, evaluated in FP32: 1.06e-3 absolute, below the BF16 resolution of the
+// layer-2 activations it produces (2^-8 relative) wherever they exceed
+// 0.27; the network's outputs move by less than their BF16 rounding does
+// (both measured against the FP64 network).  Seven VALU operations with
+// the product (the last fma clamps with its output modifier) instead of an
+// exponential and a reciprocal, which run at a quarter of the VALU rate.
 __device__ inline float gelu_gate4(float x) {
   const float y = x * x;
   float p = 1.063038355e-05f;
@@ -87,17 +85,6 @@ __device__ inline float gelu_gate4(float x) {
   p = fmaf(p, y, 6.714980851e-03f);
   p = fmaf(p, y, -5.973969748e-02f);
   p = fmaf(p, y, 3.940491249e-01f);
-  return __builtin_amdgcn_fmed3f(fmaf(x, p, 0.5f), 0.0f, 1.0f);
-}
-__device__ inline float gelu_gate6(float x) {
-  const float y = x * x;
-  float p = 2.995527462e-08f;
-  p = fmaf(p, y, -1.956490401e-06f);
-  p = fmaf(p, y, 5.489364144e-05f);
-  p = fmaf(p, y, -8.792124575e-04f);
-  p = fmaf(p, y, 9.069545159e-03f);
-  p = fmaf(p, y, -6.511615272e-02f);
-  p = fmaf(p, y, 3.980636342e-01f);
   return __builtin_amdgcn_fmed3f(fmaf(x, p, 0.5f), 0.0f, 1.0f);
 }
 
@@ -362,12 +349,24 @@ constexpr int kRing = 8;
 constexpr int kInferMinN = 9;
 constexpr int kSlotBytes = 16 * CH * 2; // one position of 16 samples x 16 channels
 
-// The layer-1 table of the inference kernel, tab[h][pattern][8 channels]:
-// channels 8h .. 8h + 7 of GELU(b1 + W1 h0) for each 5-spin pattern (bit k
-// of the pattern = spin at offset k - 2), 16 bytes per pattern, 1 KB in all.
-// Computed once per evaluation instead of once per wave.
+// What every wave of the inference kernel needs, once per evaluation
+// instead of once per wave (64 x 7 16-byte words):
+//  * words 0..63, the layer-1 table tab[h][pattern][8 channels]: channels
+//    8h .. 8h + 7 of GELU(b1 + W1 h0) for each 5-spin pattern (bit k of the
+//    pattern = spin at offset k - 2), 16 bytes per pattern;
+//  * words 64 + 6 lane + q, lane's BF16 A fragments of layers 2 (q < 3) and
+//    3 (q >= 3), as load_conv_weights builds them.
+constexpr int kInferPrepWords = 64 * 7;
 __global__ void __launch_bounds__(64) cnn_table_kernel(const float *__restrict__ P,
-                                                       __bf16 *__restrict__ tab) {
+                                                       uint4 *__restrict__ prep) {
+  __bf16 *tab = reinterpret_cast<__bf16 *>(prep);
+  bf16x8 a[3];
+  load_conv_weights(P + OFF_W2, false, a);
+#pragma unroll
+  for (int q = 0; q < 3; ++q) reinterpret_cast<bf16x8 *>(prep + 64 + 6 * threadIdx.x)[q] = a[q];
+  load_conv_weights(P + OFF_W3, false, a);
+#pragma unroll
+  for (int q = 0; q < 3; ++q) reinterpret_cast<bf16x8 *>(prep + 64 + 6 * threadIdx.x)[3 + q] = a[q];
   for (int e = threadIdx.x; e < 32 * CH; e += 64) {
     const int pat = e / CH, ch = e % CH;
     float v = 0.0f;
@@ -395,8 +394,12 @@ __global__ void __launch_bounds__(64)
   reinterpret_cast<uint4 *>(tab)[lane] = table[lane];
   __syncthreads();
   bf16x8 a2[3], a3[3];
-  load_conv_weights(P + OFF_W2, false, a2);
-  load_conv_weights(P + OFF_W3, false, a3);
+  const bf16x8 *fr = reinterpret_cast<const bf16x8 *>(table + 64 + 6 * lane);
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    a2[q] = fr[q];
+    a3[q] = fr[3 + q];
+  }
   float b2[4], b3[4];
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
@@ -404,21 +407,28 @@ __global__ void __launch_bounds__(64)
     b3[r] = P[OFF_B3 + 4 * g + r];
   }
   // B fragment q holds rows k = 32 q + 8 g .. + 7: tap offset d = k >> 4,
-  // channels c0 = k & 15 (0 or 8).  Rows k >= 80 (offset 5) meet zero
-  // weights (load_conv_weights), so their lanes read the offset-4 data
-  // instead of building zeros, and no fragment load needs a branch.
-  // Fragments 0 and 1 have d = 2q + hi, fragment 2 has d = 4 in every lane.
+  // channels 8h .. 8h + 7.  Rows k >= 80 (offset 5) meet zero weights
+  // (load_conv_weights), so their lanes read the offset-4 data instead of
+  // building zeros, and no fragment load needs a branch: d = 2q + hi for
+  // q < 2, d = 4 for q = 2.  (A 16x16x16 step for tap 4 alone would need
+  // its accumulator from a 16x16x32 step, and such a mixed-shape dependent
+  // pair gives wrong sums as the compiler issues it, with no wait states.)
   const int hi = g >> 1, h = g & 1;
-  int dq[3];
-#pragma unroll
-  for (int q = 0; q < 3; ++q) dq[q] = std::min((32 * q + 8 * g) >> 4, KW - 1);
-  const int c0 = 8 * h;
+  const int dq[3] = {hi, 2 + hi, 4};
+  // Ring and head rows (32 bytes per sample) hold channel c of sample s at
+  // c ^ 8 fs(s): the 16-byte halves swap for fs = 1, so the lanes of one
+  // LDS phase (8 or 16 consecutive samples reading the same half) spread
+  // over all banks instead of meeting every 4th / 8th sample.
+  const int fs = ((s >> 2) ^ (s >> 3)) & 1;
+  const int c0 = 8 * (h ^ fs);                             // this lane's read half
   const char *tabc = reinterpret_cast<const char *>(tab);
   char *ringc = reinterpret_cast<char *>(ring);
+  const char *headc = reinterpret_cast<const char *>(head);
   const uint32_t hoff = 512u * h;                          // table half, bytes
   const uint32_t rl = static_cast<uint32_t>(s * CH + c0) * 2; // ring read base
   const uint32_t rlh = rl + hi * kSlotBytes;               // ... one slot up for d odd
-  const uint32_t wl = static_cast<uint32_t>(s * CH + 4 * g) * 2; // ring write base
+  // ring write base: channels 4g .. 4g + 3 of sample s
+  const uint32_t wl = static_cast<uint32_t>(s * CH + 4 * (g ^ (2 * fs))) * 2;
   // ext bit k = state bit (k - 4) mod n for k < 2n (n >= kInferMinN makes
   // 2n >= n + 9): layer 2 at pos reads the layer-1 table at the 5-spin
   // pattern around pos + d - 2, i.e. bits d .. d + 4 of win = ext >> pos.
@@ -430,18 +440,15 @@ __global__ void __launch_bounds__(64)
                  e2 = static_cast<uint32_t>(ext_hi);
   auto ld = [](const char *p) { return *reinterpret_cast<const bf16x8 *>(p); };
   // the last four layer-3 positions wrap around to the head copy of 0..3
-  auto frag3_tail = [&](int pos, int q) {
-    int p = pos + dq[q] - 2;
+  auto tail_src = [&](int p) {
     p = p < 0 ? p + n : (p >= n ? p - n : p);
-    const __bf16 *src =
-        p < 4 ? head + (p * 16 + s) * CH : ring + ((p & (kRing - 1)) * 16 + s) * CH;
-    return *reinterpret_cast<const bf16x8 *>(src + c0);
+    return p < 4 ? headc + p * kSlotBytes : ringc + (p & (kRing - 1)) * kSlotBytes;
   };
   // the biases start the MFMA accumulators
   float f[4] = {0.f, 0.f, 0.f, 0.f};
   auto layer3_out = [&](floatx4 acc) {
 #pragma unroll
-    for (int r = 0; r < 4; ++r) f[r] = fmaf(acc[r], gelu_gate6(acc[r]), f[r]);
+    for (int r = 0; r < 4; ++r) f[r] = fmaf(acc[r], gelu_gate4(acc[r]), f[r]);
   };
   // (one wave per workgroup: its LDS accesses stay in program order)
   for (int pos0 = 0; pos0 < n; pos0 += kRing) {
@@ -461,10 +468,11 @@ __global__ void __launch_bounds__(64)
       for (int r = 0; r < 4; ++r) o[r] = static_cast<__bf16>(acc[r] * gelu_gate4(acc[r]));
       *reinterpret_cast<bf16x4 *>(ringc + wl + u * kSlotBytes) = o;
       if (u == 0) *reinterpret_cast<bf16x4 *>(ringc + wl + kRing * kSlotBytes) = o;
-      if (pos < 4) *reinterpret_cast<bf16x4 *>(reinterpret_cast<char *>(head) + wl + pos * kSlotBytes) = o;
+      if (pos < 4) *reinterpret_cast<bf16x4 *>(const_cast<char *>(headc) + wl + pos * kSlotBytes) = o;
       if (pos >= 4) {
-        // layer 3 at pos - 2 reads layer 2 at pos - 4 + d, slot (u - 4 + d) mod 8;
-        // d = 2q + hi for q < 2 (slot 8 stands for slot 0), d = 4 for q = 2
+        // layer 3 at pos - 2 reads layer 2 at pos - 4 + d, slot (u - 4 + d)
+        // mod 8: d = 2q + hi for q < 2 (slot 8 stands for slot 0), d = 4
+        // (slot u) for q = 2
         floatx4 acc3 = {b3[0], b3[1], b3[2], b3[3]};
         acc3 = mfma(a3[0], ld(ringc + rlh + ((u - 4) & (kRing - 1)) * kSlotBytes), acc3);
         acc3 = mfma(a3[1], ld(ringc + rlh + ((u - 2) & (kRing - 1)) * kSlotBytes), acc3);
@@ -476,9 +484,10 @@ __global__ void __launch_bounds__(64)
   const int tail_pos[4] = {n - 2, n - 1, 0, 1};
 #pragma unroll
   for (int t = 0; t < 4; ++t) {
+    const int tp = tail_pos[t];
     floatx4 acc3 = {b3[0], b3[1], b3[2], b3[3]};
 #pragma unroll
-    for (int k = 0; k < 3; ++k) acc3 = mfma(a3[k], frag3_tail(tail_pos[t], k), acc3);
+    for (int q = 0; q < 3; ++q) acc3 = mfma(a3[q], ld(tail_src(tp + dq[q] - 2) + rl), acc3);
     layer3_out(acc3);
   }
   float pa = 0.f, pt = 0.f;
@@ -789,14 +798,14 @@ size_t lds_bytes_backward(int n) { return static_cast<size_t>(3) * 16 * n * CH *
 
 struct CnnModel {
   DeviceBuffer<float> params, m, v, grad;
-  DeviceBuffer<uint4> table; // the inference kernel's layer-1 table (64 x 16 bytes)
+  DeviceBuffer<uint4> table; // the inference kernel's table and weight fragments
   int64_t step = 0;
 };
 
 // the table from the current parameters, then the inference kernel
 inline void launch_infer(const CnnModel &m, const CnnArgs &a, unsigned blocks,
                          const uint64_t *states, float *out_a, float *out_t, hipStream_t s) {
-  cnn_table_kernel<<<1, 64, 0, s>>>(a.params, reinterpret_cast<__bf16 *>(m.table.ptr));
+  cnn_table_kernel<<<1, 64, 0, s>>>(a.params, m.table.ptr);
   cnn_infer_kernel<<<blocks, 64, 0, s>>>(a, m.table.ptr, states, out_a, out_t);
 }
 
@@ -822,7 +831,7 @@ int spined_cnn_create(spined_cnn **out, const float *init_params) {
     c->m.m.allocate(spined::NPARAMS);
     c->m.v.allocate(spined::NPARAMS);
     c->m.grad.allocate(spined::NPARAMS);
-    c->m.table.allocate(64);
+    c->m.table.allocate(spined::kInferPrepWords);
     SPINED_HIP(hipMemcpy(c->m.params.ptr, init_params, 4 * spined::NPARAMS, hipMemcpyHostToDevice));
     SPINED_HIP(hipMemset(c->m.m.ptr, 0, 4 * spined::NPARAMS));
     SPINED_HIP(hipMemset(c->m.v.ptr, 0, 4 * spined::NPARAMS));

@@ -47,7 +47,10 @@ typedef struct spined_monitor_info {
 } spined_monitor_info;
 
 /* Called once per restart cycle (mirrors PrimmeMonitor, SpinED.hs:379-382).
- * Return non-zero to stop. */
+ * Return non-zero to stop the current phase at this cycle: the FP32-storage
+ * (or block) phase hands over to the FP64 phase; the FP64 phase returns its
+ * current Ritz values with their explicit residuals (LS_NOT_CONVERGED when
+ * they miss tol) and makes no further attempt. */
 typedef int (*spined_monitor_fn)(const spined_monitor_info *info, void *ctx);
 
 /* Checkpoint (no counterpart in the reference, which creates /_workspace

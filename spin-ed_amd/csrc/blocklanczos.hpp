@@ -503,6 +503,7 @@ template <class VT, int K> struct BlockPhase {
         res.converged = true;
         return res;
       }
+      res.stopped = stop != 0;
       if (stop || (max_matvecs > 0 && matvecs >= max_matvecs) || res.cycles >= kMaxBlockCycles)
         return res;
       if (stall_detect) {

@@ -486,6 +486,7 @@ struct PhaseResult {
   std::vector<double> theta;  // lowest k Ritz values
   std::vector<double> est;    // residual estimates
   bool converged = false;
+  bool stopped = false;       // the monitor asked to stop
   int cycles = 0;
 };
 
@@ -782,6 +783,7 @@ template <class VT, bool C = false> struct Phase {
         res.converged = true;
         return res;
       }
+      res.stopped = stop != 0;
       if (stop || (max_matvecs > 0 && matvecs >= max_matvecs) || m >= static_cast<int>(n))
         return res;
       if (stall_detect) {
@@ -1004,10 +1006,14 @@ bool final_phase(Operator &op, uint64_t n, int m, int keep, uint64_t k, double t
   for (int attempt = 0; attempt < 8; ++attempt) {
     PhaseResult r = p.run(tol_run, budget, false, 0, monitor, ctx, opts.verbose, opts.seed + 1);
     st.restarts += r.cycles;
-    // explicit residuals of the Ritz vectors (FP64 matvec)
+    // explicit residuals of the Ritz vectors (FP64 matvec); their sum goes
+    // to w, the start of a further attempt
     bool ok = true;
+    SPINED_HIP(hipMemsetAsync(p.w.ptr, 0, ne * 8, s));
     for (uint64_t i = 0; i < k; ++i) {
       p.ritz(static_cast<int>(i), y.ptr);
+      residual_kernel<<<grid_for(ne), kBlock, 0, s>>>(p.w.ptr, y.ptr, -1.0, p.w.ptr, ne);
+      SPINED_CHECK_LAUNCH();
       p.matvec(y.ptr, vt, hy.ptr);
       // Rayleigh quotient and residual norm
       const double *yp = y.ptr;
@@ -1050,12 +1056,15 @@ bool final_phase(Operator &op, uint64_t n, int m, int keep, uint64_t k, double t
       }
     }
     p.harvest_events();
-    if (ok || (budget > 0 && p.matvecs >= budget)) {
+    // a stop requested by the monitor ends the solve here, as does the budget
+    if (ok || r.stopped || (budget > 0 && p.matvecs >= budget)) {
       converged = ok;
       break;
     }
-    // estimates said converged but the explicit residual disagrees: keep
-    // iterating from the current basis with a tighter target
+    // estimates said converged but the explicit residual disagrees (the
+    // basis has drifted from the recurrence): a fresh recurrence from the
+    // current Ritz vectors, with a tighter target for its estimates
+    p.set_start_from_w();
     tol_run *= 0.5;
   }
   st.matvecs += p.matvecs;

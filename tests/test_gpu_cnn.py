@@ -31,7 +31,8 @@ def _bf16(x):
     return r.astype(np.uint32).view(np.float32)
 
 
-@pytest.mark.parametrize("n", [16, 22, 40])
+# 9: the inference kernel's smallest n; 33 and 64: windows past bit 32, all 64 bits
+@pytest.mark.parametrize("n", [9, 16, 22, 33, 40, 64])
 def test_forward_matches_oracle(n):
     rng = np.random.default_rng(n)
     p = learned.init_params(3)

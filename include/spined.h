@@ -187,6 +187,13 @@ int spined_operator_set_orbit_method(ls_operator *op, int method);
  * 1 on; built at the next matvec.  get: 1 if a schedule is in use. */
 int spined_operator_set_schedule(ls_operator *op, int mode);
 int spined_operator_get_schedule(const ls_operator *op);
+/* Rows per wave of the matrix-free real matvec of exchange-bond operators:
+ * 64 (one lane per row), 32 or 16 (the row's templates spread over 2 or 4
+ * lanes, its terms summed in the same order: the sums are unchanged); -1 =
+ * automatic (64).  The autotuner times all three.  Other operators and the
+ * gather-cache matvec keep one lane per row. */
+int spined_operator_set_rows_per_wave(ls_operator *op, int rows);
+int spined_operator_get_rows_per_wave(const ls_operator *op);
 /* Gather cache of a real operator: the target of every term of every row
  * (4 bytes per term, padded per wave: ~74 GB at D = 8.6e8), written once by the orbit and
  * index kernels so that later real single-vector matvecs read it instead of
@@ -227,8 +234,9 @@ int spined_basis_is_complete(const ls_spin_basis *basis);
  * (bits ceil(log2 D) + 1 .. + 4) with the row schedule, the default size
  * without it, the hash table when the bucket search must confirm its
  * hits (spined_basis_is_complete), and the rank index where it applies;
- * then the per-vector time of the block
- * matvec for K = 1, 2, 4, 8.  Index tables larger than a quarter of the
+ * then, on the chosen index, 64, 32 and 16 rows per wave for exchange-bond
+ * operators (spined_operator_set_rows_per_wave); then the per-vector time
+ * of the block matvec for K = 1, 2, 4, 8.  Index tables larger than a quarter of the
  * free device memory are skipped, leaving room for the eigensolver's
  * vectors.  Each timing is the mean of `repeats` launches after one warm-up.
  * The chosen setting stays on the basis / operator. */
@@ -241,6 +249,8 @@ typedef struct spined_autotune_report {
   double ms_per_matvec_first; /* the setting in force before tuning */
   double ms_per_vector_block[4]; /* block matvec, K = 1, 2, 4, 8 (0: not timed) */
   int32_t best_block;         /* K with the lowest per-vector time */
+  int32_t rows_per_wave;      /* chosen: 64, 32 or 16 (exchange-bond operators) */
+  double ms_per_rows_per_wave[3]; /* FP64 matvec at 64, 32, 16 rows per wave (0: not timed) */
 } spined_autotune_report;
 int spined_operator_autotune(ls_operator *op, int32_t repeats, spined_autotune_report *report);
 /* Exact sector dimension by character counting (before ls_build). */

@@ -810,7 +810,27 @@ int spined_operator_autotune(ls_operator *op, int32_t repeats, spined_autotune_r
     report->index_kind = b.index_kind;
     report->bucket_bits = b.bucket_bits;
     report->schedule = o.schedule_built ? 1 : 0;
+    // rows per wave of the exchange-bond kernel (lanes per row 1, 2, 4) on
+    // the chosen index; the sums are the same for each
+    o.lanes_per_row = 1;
     report->ms_per_matvec = time_matvec(o, 1, repeats, x.ptr, y.ptr);
+    report->rows_per_wave = kRowsPerWaveMax;
+    report->ms_per_rows_per_wave[0] = report->ms_per_matvec;
+    if (o.tables.bonds && o.tables.h_bond_sites.size() <= 128) {
+      int best_l = 1;
+      for (int q = 1; q < 3; ++q) {
+        const int l = 1 << q;
+        o.lanes_per_row = l;
+        const double ms = time_matvec(o, 1, repeats, x.ptr, y.ptr);
+        report->ms_per_rows_per_wave[q] = ms;
+        if (ms < report->ms_per_matvec) {
+          report->ms_per_matvec = ms;
+          best_l = l;
+        }
+      }
+      o.lanes_per_row = best_l;
+      report->rows_per_wave = kRowsPerWaveMax / best_l;
+    }
     double best_pv = 0.0;
     const int Ks[4] = {1, 2, 4, 8};
     for (int q = 0; q < 4; ++q) {
@@ -861,6 +881,20 @@ int spined_operator_set_schedule(ls_operator *op, int mode) {
 
 int spined_operator_get_schedule(const ls_operator *op) {
   return op ? (op->p->schedule_built ? 1 : 0) : -1;
+}
+
+int spined_operator_set_rows_per_wave(ls_operator *op, int rows) {
+  return guarded([&] {
+    if (!op || !(rows == -1 || rows == 64 || rows == 32 || rows == 16))
+      throw Error(LS_INVALID_ARGUMENT, "rows per wave must be 64, 32, 16 or -1");
+    Operator &o = *op->p;
+    std::lock_guard<std::mutex> lk(o.mu);
+    o.lanes_per_row = rows == -1 ? 1 : kRowsPerWaveMax / rows;
+  });
+}
+
+int spined_operator_get_rows_per_wave(const ls_operator *op) {
+  return op ? kRowsPerWaveMax / op->p->lanes_per_row : -1;
 }
 
 int spined_operator_set_gather_cache(ls_operator *op, int mode) {

@@ -233,6 +233,10 @@ struct Operator {
   // active off-diagonal templates, so a wave's lanes carry equal work.
   // sched[i] = offset within the tile of the row processed at slot i.
   int schedule_request = -1; // -1 automatic, 0 off, 1 on
+  // Lanes per row of the matrix-free real matvec of exchange-bond
+  // operators: 1 (kRowsPerWaveMax rows per wave), 2 or 4; the sums do not
+  // depend on it
+  int lanes_per_row = 1;
   bool schedule_built = false;
   DeviceBuffer<uint16_t> sched;
   std::unique_ptr<GatherCache> cache = std::make_unique<GatherCache>();
@@ -241,6 +245,7 @@ struct Operator {
 };
 
 constexpr int kScheduleTile = 4096;
+constexpr int kRowsPerWaveMax = 64; // one lane per row
 void ensure_schedule(Operator &op); // matvec.hip
 // Drops the operator's gather cache (the schedule or the basis changed).
 void drop_gather_cache(Operator &op);  // matvec.hip

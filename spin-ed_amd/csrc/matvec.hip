@@ -62,6 +62,28 @@ __global__ void __launch_bounds__(kRowBlock)
         gather_row_real<METHOD, NW, BONDS>(o, bd, rt, ga, ra, ix, stab, i, x, BONDS ? so : nullptr));
 }
 
+// matvec_real_kernel with L lanes per row (64 / L rows per wave): the same
+// sums, the row's templates spread over its lanes (gather_row_real_lanes).
+template <int METHOD, int NW, int L, class XT, class YT>
+__global__ void __launch_bounds__(kRowBlock)
+    matvec_real_lanes_kernel(OpArgs o, BondArgs bd, RatioArgs rt, GenericOrbitArgs ga,
+                             RingOrbitArgs ra, IndexArgs ix, const uint32_t *__restrict__ stab,
+                             uint64_t D, const XT *__restrict__ x, YT *__restrict__ y,
+                             const uint16_t *__restrict__ sched) {
+  __shared__ double so[64 * NW];
+  for (int k = threadIdx.x; k < bd.nbonds; k += kRowBlock) so[k] = bd.o[k];
+  __syncthreads();
+  const uint64_t lb = xcd_block(blockIdx.x, gridDim.x);
+  // a row's L lanes share the exit: the group's shuffles stay within it
+  const uint64_t slot = lb * (kRowBlock / L) + threadIdx.x / L;
+  if (slot >= D) return;
+  const int sub = static_cast<int>(threadIdx.x % L);
+  const uint64_t i = scheduled_row(sched, slot);
+  const double v =
+      gather_row_real_lanes<METHOD, NW, true, L>(o, bd, rt, ga, ra, ix, stab, i, x, so, sub);
+  if (sub == 0) st_re(y, i, v);
+}
+
 // ---- gather cache -----------------------------------------------------------
 constexpr uint32_t kCacheIndex = 0x3FFFFFFFu; // index bits; all ones: no term
 constexpr uint32_t kCacheNeg = 1u << 30;      // negative coefficient
@@ -496,10 +518,24 @@ struct Ctx {
   uint64_t D;
   bool bonds;
   const uint16_t *sched;
+  int lanes; // lanes per row of the exchange-bond matrix-free kernel (1, 2, 4)
 };
 
 template <int METHOD, int NW, bool BONDS, class XT, class YT>
 void launch_real_nw(const Ctx &c, const XT *x, YT *y, hipStream_t s) {
+  if constexpr (BONDS) {
+    if (c.lanes == 2 || c.lanes == 4) {
+      const uint64_t rows = kRowBlock / c.lanes;
+      const unsigned g = static_cast<unsigned>((c.D + rows - 1) / rows);
+      if (c.lanes == 2)
+        matvec_real_lanes_kernel<METHOD, NW, 2, XT, YT><<<g, kRowBlock, 0, s>>>(
+            c.o, c.bd, c.rt, c.ga, c.ra, c.ix, c.stab, c.D, x, y, c.sched);
+      else
+        matvec_real_lanes_kernel<METHOD, NW, 4, XT, YT><<<g, kRowBlock, 0, s>>>(
+            c.o, c.bd, c.rt, c.ga, c.ra, c.ix, c.stab, c.D, x, y, c.sched);
+      return;
+    }
+  }
   const unsigned g = static_cast<unsigned>((c.D + kRowBlock - 1) / kRowBlock);
   matvec_real_kernel<METHOD, NW, BONDS, XT, YT><<<g, kRowBlock, 0, s>>>(
       c.o, c.bd, c.rt, c.ga, c.ra, c.ix, c.stab, c.D, x, y, c.sched);
@@ -767,6 +803,7 @@ Ctx make_ctx(const Operator &op, const double *pc, const double *ps) {
   c.stab = b.stab.ptr;
   c.D = b.D;
   c.sched = op.schedule_built ? op.sched.ptr : nullptr;
+  c.lanes = op.lanes_per_row;
   return c;
 }
 

@@ -315,6 +315,88 @@ __device__ inline double gather_row_real(const OpArgs &o, const BondArgs &bd, co
 }
 
 
+// The row of gather_row_real on L lanes (L = 2 or 4, lane `sub` of the row's
+// group of L consecutive lanes): every lane evaluates the diagonal and the
+// active template list; in each round lane l takes the l-th of the next L
+// templates (orbit, index search, x gather, coefficient), and every lane then
+// accumulates the round's terms in template order from the group's
+// shuffles.  Same terms, same fma chain: bit-identical to gather_row_real.
+template <int METHOD, int NW, bool BONDS, int L, class XT>
+__device__ inline double gather_row_real_lanes(const OpArgs &o, const BondArgs &bd,
+                                               const RatioArgs &rt, const GenericOrbitArgs &ga,
+                                               const RingOrbitArgs &ra, const IndexArgs &ix_in,
+                                               const uint32_t *__restrict__ stab, uint64_t i,
+                                               const XT *__restrict__ x, const double *bond_o,
+                                               int sub) {
+  const IndexArgs ix = index_for_method<METHOD>(ix_in);
+  if (!bond_o) bond_o = bd.o;
+  const uint64_t r = ix.reps[i];
+  uint64_t act[NW];
+  const double diag = row_pass_a<NW, BONDS>(o, bd, r, act);
+  const RowOrbit<METHOD, BONDS> rob(ra, bd, r);
+  double acc = diag * ld_re(x, i);
+  const uint32_t si = stab[i];
+  const double sr = static_cast<double>(si);
+  const int row_di = rt.nd ? static_cast<int>(rt.div_idx[si]) : 0;
+  auto coef_of = [&](int j, const OrbitResult &orb) {
+    const double f = orb.count == static_cast<int32_t>(si)
+                         ? 1.0
+                         : (rt.nd ? rt.ratio[rt.div_idx[orb.count] * rt.nd + row_di]
+                                  : sqrt(static_cast<double>(orb.count) / sr));
+    double c = (BONDS ? bond_o[j] : o.coef_re[j]) * f;
+    return orb.phase != 0 ? -c : c;
+  };
+  while (any_set<NW>(act)) {
+    int j = -1;
+#pragma unroll
+    for (int l = 0; l < L; ++l) {
+      const int t = pop_first<NW>(act);
+      if (l == sub) j = t;
+    }
+    double c = 0.0, xv = 0.0;
+    int have = 0;
+    if (j >= 0) {
+      int64_t k = -1;
+      OrbitResult orb{0, 0, 1, true};
+      bool by_rank = false;
+      if constexpr (METHOD == 0 && BONDS) {
+        // symmetry-free weight sector: the move's target from the row index
+        if (ix.rank_mode == 1) {
+          by_rank = rank_hop(ix, ga, bd, r, i, j, k);
+          if (!by_rank) {
+            orb = rob.orbit(o, bd, ga, ra, r, j);
+            int64_t k2 = -1;
+            rank_of2(ix, orb.rep, orb.valid, 0, false, k, k2);
+          }
+          by_rank = true;
+        }
+      }
+      if (!by_rank) {
+        orb = rob.orbit(o, bd, ga, ra, r, j);
+        const OrbitResult none{0, 0, 0, false};
+        int64_t kb;
+        double xb;
+        lookup_gather2(ix, orb, none,
+                       [&](int64_t q) { return ld_re(x, static_cast<uint64_t>(q)); }, k, kb,
+                       xv, xb);
+      } else if (k >= 0) {
+        xv = ld_re(x, static_cast<uint64_t>(k));
+      }
+      if (k >= 0) {
+        c = coef_of(j, orb);
+        have = 1;
+      }
+    }
+#pragma unroll
+    for (int l = 0; l < L; ++l) {
+      const double cl = __shfl(c, l, L);
+      const double xl = __shfl(xv, l, L);
+      if (__shfl(have, l, L)) acc = fma(cl, xl, acc);
+    }
+  }
+  return acc;
+}
+
 // K values of one row of a row-interleaved block, loaded as one aligned
 // record (16-byte loads).
 template <class XT, int K> struct alignas(K * sizeof(XT) < 16 ? K * sizeof(XT) : 16) RowK {

@@ -34,6 +34,7 @@ SPINED_SYMBOLS = (
     "spined_operator_expectation_device", "spined_operator_expectation_device_noalloc",
     "spined_operator_count_offdiag", "spined_operator_orbit_method",
     "spined_operator_set_orbit_method", "spined_operator_set_schedule",
+    "spined_operator_set_rows_per_wave", "spined_operator_get_rows_per_wave",
     "spined_operator_get_schedule", "spined_basis_set_bucket_bits",
     "spined_basis_get_bucket_bits", "spined_basis_set_index_kind", "spined_basis_get_index_kind",
     "spined_basis_is_complete", "spined_operator_set_gather_cache",
@@ -60,6 +61,8 @@ class SpinedAutotuneReport(C.Structure):
         ("ms_per_matvec_first", C.c_double),
         ("ms_per_vector_block", C.c_double * 4),
         ("best_block", C.c_int32),
+        ("rows_per_wave", C.c_int32),
+        ("ms_per_rows_per_wave", C.c_double * 3),
     ]
 
 
@@ -177,6 +180,8 @@ def load():
         "spined_operator_set_schedule": (I, [P, I]),
         "spined_operator_autotune": (I, [P, C.c_int32, C.POINTER(SpinedAutotuneReport)]),
         "spined_operator_get_schedule": (I, [P]),
+        "spined_operator_set_rows_per_wave": (I, [P, I]),
+        "spined_operator_get_rows_per_wave": (I, [P]),
         "spined_operator_set_gather_cache": (I, [P, I]),
         "spined_operator_get_gather_cache": (I, [P]),
         "spined_basis_set_bucket_bits": (I, [P, I]),

@@ -372,7 +372,19 @@ def autotune(op: Operator_, repeats: int = 2) -> dict:
     check_status(_lib.load().spined_operator_autotune(op.ptr, repeats, C.byref(rep)))
     out = {f: getattr(rep, f) for f, _ in _lib.SpinedAutotuneReport._fields_}
     out["ms_per_vector_block"] = {k: v for k, v in zip((1, 2, 4, 8), rep.ms_per_vector_block) if v}
+    out["ms_per_rows_per_wave"] = {k: v for k, v in zip((64, 32, 16), rep.ms_per_rows_per_wave)
+                                   if v}
     return out
+
+
+def set_rows_per_wave(op: Operator_, rows: int) -> None:
+    """Rows per wave of the matrix-free real matvec of exchange-bond
+    operators: 64, 32 or 16 (-1 automatic: 64).  Sums are identical."""
+    check_status(_lib.load().spined_operator_set_rows_per_wave(op.ptr, rows))
+
+
+def rows_per_wave(op: Operator_) -> int:
+    return _lib.load().spined_operator_get_rows_per_wave(op.ptr)
 
 
 def set_schedule(op: Operator_, mode: int) -> None:

@@ -221,25 +221,22 @@ def test_hop_orbit_matches_full_scan(n, w, z, gens):
     assert len(states) > 0
 
 
-def test_eigsh_options_layout_matches_header():
-    """The ctypes mirror of spined_eigsh_options has the C layout (size and
-    the offsets of the fields a host sets through pokeByteOff), compiled
-    from include/spined.h with the system C compiler."""
-    import ctypes as C
+def _c_layout(ctype_name, struct):
+    """sizeof and the field offsets of a struct of include/spined.h, from
+    the system C compiler; None without one."""
     import shutil
     import subprocess
     import tempfile
     from pathlib import Path
-    from spined import _lib
     cc = shutil.which("gcc") or shutil.which("cc")
     if cc is None:
-        pytest.skip("no C compiler")
+        return None
     inc = Path(__file__).resolve().parent.parent / "include"
-    fields = [f for f, _ in _lib.SpinedEigshOptions._fields_]
+    fields = [f for f, _ in struct._fields_]
     src = "#include <stddef.h>\n#include <stdio.h>\n#include \"spined.h\"\nint main(void) {\n"
-    src += '  printf("%zu\\n", sizeof(spined_eigsh_options));\n'
+    src += f'  printf("%zu\\n", sizeof({ctype_name}));\n'
     for f in fields:
-        src += f'  printf("%zu\\n", offsetof(spined_eigsh_options, {f}));\n'
+        src += f'  printf("%zu\\n", offsetof({ctype_name}, {f}));\n'
     src += "  return 0;\n}\n"
     with tempfile.TemporaryDirectory() as d:
         c = Path(d) / "layout.c"
@@ -248,6 +245,22 @@ def test_eigsh_options_layout_matches_header():
         subprocess.run([cc, f"-I{inc}", str(c), "-o", str(exe)], check=True)
         out = [int(v) for v in subprocess.run([str(exe)], check=True, capture_output=True,
                                                text=True).stdout.split()]
-    assert out[0] == C.sizeof(_lib.SpinedEigshOptions)
-    for f, off in zip(fields, out[1:]):
-        assert getattr(_lib.SpinedEigshOptions, f).offset == off, f
+    return out[0], dict(zip(fields, out[1:]))
+
+
+@pytest.mark.parametrize("ctype_name,attr", [("spined_eigsh_options", "SpinedEigshOptions"),
+                                             ("spined_autotune_report", "SpinedAutotuneReport")])
+def test_struct_layouts_match_header(ctype_name, attr):
+    """The ctypes mirrors of the option and report structs have the C layout
+    (size and every field offset a host reads or sets through peekByteOff /
+    pokeByteOff), compiled from include/spined.h."""
+    import ctypes as C
+    from spined import _lib
+    struct = getattr(_lib, attr)
+    got = _c_layout(ctype_name, struct)
+    if got is None:
+        pytest.skip("no C compiler")
+    size, offs = got
+    assert size == C.sizeof(struct)
+    for f, off in offs.items():
+        assert getattr(struct, f).offset == off, f

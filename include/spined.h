@@ -276,7 +276,7 @@ typedef struct spined_batch_stats {
 /* evals, rnorms: [count]; cycles[i]: restart cycles instance i ran (-1: not
  * converged); seconds[i]: time from the launch to the end of instance i;
  * matvecs[i]: its matvecs (cycles, seconds, matvecs may be NULL).
- * max_basis_size <= 24 (0: 20). */
+ * max_basis_size <= 24 (0: 16). */
 int spined_batch_eigsh(const ls_operator *const ops[], uint64_t count, double tol,
                        int32_t max_basis_size, uint64_t seed, int32_t max_cycles, double *evals,
                        double *rnorms, int32_t *cycles, double *seconds, int64_t *matvecs,

@@ -59,9 +59,10 @@ class SpinedBatchStats(C.Structure):
                 ("matvecs_per_instance_max", C.c_int64)]
 
 
-def batch_eigsh(ops: List[I.Operator_], tol: float = 1e-9, max_basis_size: int = 20,
+def batch_eigsh(ops: List[I.Operator_], tol: float = 1e-9, max_basis_size: int = 0,
                 seed: int = 0x5EED, max_cycles: int = 200) -> dict:
-    """One persistent launch for all instances (spined_batch_eigsh)."""
+    """One persistent launch for all instances (spined_batch_eigsh); max_basis_size
+    0 takes the library default (16)."""
     L = _lib.load()
     f = L.spined_batch_eigsh
     f.restype = C.c_int

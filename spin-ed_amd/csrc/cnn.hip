@@ -423,7 +423,7 @@ __global__ void __launch_bounds__(64)
   const int c0 = 8 * (h ^ fs);                             // this lane's read half
   const char *tabc = reinterpret_cast<const char *>(tab);
   char *ringc = reinterpret_cast<char *>(ring);
-  const char *headc = reinterpret_cast<const char *>(head);
+  char *headc = reinterpret_cast<char *>(head);
   const uint32_t hoff = 512u * h;                          // table half, bytes
   const uint32_t rl = static_cast<uint32_t>(s * CH + c0) * 2; // ring read base
   const uint32_t rlh = rl + hi * kSlotBytes;               // ... one slot up for d odd
@@ -468,7 +468,7 @@ __global__ void __launch_bounds__(64)
       for (int r = 0; r < 4; ++r) o[r] = static_cast<__bf16>(acc[r] * gelu_gate4(acc[r]));
       *reinterpret_cast<bf16x4 *>(ringc + wl + u * kSlotBytes) = o;
       if (u == 0) *reinterpret_cast<bf16x4 *>(ringc + wl + kRing * kSlotBytes) = o;
-      if (pos < 4) *reinterpret_cast<bf16x4 *>(const_cast<char *>(headc) + wl + pos * kSlotBytes) = o;
+      if (pos < 4) *reinterpret_cast<bf16x4 *>(headc + wl + pos * kSlotBytes) = o;
       if (pos >= 4) {
         // layer 3 at pos - 2 reads layer 2 at pos - 4 + d, slot (u - 4 + d)
         // mod 8: d = 2q + hi for q < 2 (slot 8 stands for slot 0), d = 4

@@ -422,6 +422,23 @@ uint64_t level2_size(const Basis &b, int bits, uint2 *dir) {
   return total;
 }
 
+// True when the permutations move site n - 1 to every site (the group acts
+// transitively on the sites, as translations of a periodic lattice do).
+bool site_transitive(const Group *g, int n) {
+  if (!g || n <= 0) return n == 1;
+  std::vector<char> seen(n, 0);
+  int count = 0;
+  for (const auto &p : g->perms) {
+    if (static_cast<int>(p.size()) != n) return false;
+    const int s = p[n - 1];
+    if (s < n && !seen[s]) {
+      seen[s] = 1;
+      ++count;
+    }
+  }
+  return count == n;
+}
+
 void build_basis(Basis &b) {
   const auto t0 = std::chrono::steady_clock::now();
   hipStream_t s = stream();
@@ -430,6 +447,11 @@ void build_basis(Basis &b) {
   if (b.w < 0) {
     if (n > 48) throw Error(LS_NOT_IMPLEMENTED, "no Hamming weight restriction above 48 spins");
     total = 1ull << n;
+  } else if (b.w < n && site_transitive(b.group.get(), n)) {
+    // every state with a zero has an image with site n - 1 empty, so its
+    // orbit minimum lies below 2^(n-1): the candidates are the weight-w
+    // states of sites 0 .. n - 2, the first C(n - 1, w) in colex order
+    total = binomial(n - 1, b.w);
   } else {
     total = binomial(n, b.w);
   }

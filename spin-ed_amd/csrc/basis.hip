@@ -32,7 +32,13 @@ struct EnumArgs {
   int n;
   int w;              // -1: all 2^n states
   uint64_t total;     // number of candidates
+  // candidate = (y << shift) | low for the enumerated weight-w states y of
+  // n bits (a monotone map: the candidates stay in ascending order)
+  int shift;
+  uint64_t low;
 };
+
+__device__ inline uint64_t candidate(const EnumArgs &e, uint64_t y) { return (y << e.shift) | e.low; }
 
 __device__ inline uint64_t unrank_colex(const EnumArgs &e, uint64_t rank) {
   if (e.w < 0) return rank;
@@ -89,10 +95,10 @@ __global__ void __launch_bounds__(kBlock)
     const uint64_t r0 = (word0 + t) * 64;
     const uint64_t rem = e.total - r0;
     const int cnt = rem < 64 ? static_cast<int>(rem) : 64;
-    uint64_t x = unrank_colex(e, r0);
+    uint64_t y = unrank_colex(e, r0);
     for (int i = 0; i < cnt; ++i) {
-      if (is_basis_state<METHOD>(ga, ra, x)) bits |= 1ull << i;
-      if (i + 1 < cnt) x = next_state(e, x);
+      if (is_basis_state<METHOD>(ga, ra, candidate(e, y))) bits |= 1ull << i;
+      if (i + 1 < cnt) y = next_state(e, y);
     }
     masks[t] = bits;
   }
@@ -120,10 +126,10 @@ __global__ void __launch_bounds__(kBlock)
   uint64_t pos = base + block_offsets[blockIdx.x] + lds[threadIdx.x] - c;
   const uint64_t r0 = (word0 + t) * 64;
   const int last = 63 - __clzll(bits);
-  uint64_t x = unrank_colex(e, r0);
+  uint64_t y = unrank_colex(e, r0);
   for (int i = 0; i <= last; ++i) {
-    if ((bits >> i) & 1) reps[pos++] = x;
-    if (i < last) x = next_state(e, x);
+    if ((bits >> i) & 1) reps[pos++] = candidate(e, y);
+    if (i < last) y = next_state(e, y);
   }
 }
 
@@ -444,14 +450,26 @@ void build_basis(Basis &b) {
   hipStream_t s = stream();
   const int n = b.n;
   uint64_t total;
+  int en = n, ew = b.w, shift = 0;
+  uint64_t low = 0;
   if (b.w < 0) {
     if (n > 48) throw Error(LS_NOT_IMPLEMENTED, "no Hamming weight restriction above 48 spins");
     total = 1ull << n;
+  } else if (b.w >= 2 && b.w < n && b.tables.ring_available) {
+    // all n rotations: an orbit minimum x has site n - 1 empty (some
+    // rotation empties it) and site 0 occupied (else x >> 1, a rotation of
+    // x, is smaller), so x = (y << 1) | 1 for y of weight w - 1 on n - 2 sites
+    en = n - 2;
+    ew = b.w - 1;
+    shift = 1;
+    low = 1;
+    total = binomial(en, ew);
   } else if (b.w < n && site_transitive(b.group.get(), n)) {
     // every state with a zero has an image with site n - 1 empty, so its
     // orbit minimum lies below 2^(n-1): the candidates are the weight-w
     // states of sites 0 .. n - 2, the first C(n - 1, w) in colex order
-    total = binomial(n - 1, b.w);
+    en = n - 1;
+    total = binomial(en, ew);
   } else {
     total = binomial(n, b.w);
   }
@@ -475,7 +493,7 @@ void build_basis(Basis &b) {
   DeviceBuffer<uint32_t> counts(seg_blocks);
   DeviceBuffer<uint64_t> offsets(seg_blocks + 1);
   DeviceBuffer<uint64_t> seg_total(1);
-  EnumArgs e{binom.ptr, n, b.w, total};
+  EnumArgs e{binom.ptr, en, ew, total, shift, low};
   const auto ga = b.tables.generic_args(true);
   const auto ra = b.tables.ring_args(true);
   uint64_t emitted = 0;

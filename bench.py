@@ -436,7 +436,12 @@ def roofline(D, e_off, n64, t64, n32, t32, cached):
             "per_kernel": per}
 
 
-def batch_roofline(sysb, res, m=20):
+# Krylov basis size of the batched solves (the library's default for
+# spined_batch_eigsh; the byte model below uses the same value)
+BATCH_BASIS = 16
+
+
+def batch_roofline(sysb, res, m=BATCH_BASIS):
     """Roofline of the persistent batched launch by a model of its
     algorithmic bytes: per matvec of instance i, D_i (8 + 4 + 8 + 8) +
     E_i 8 (representative, stabiliser, own x, y, one x per off-diagonal term;
@@ -459,10 +464,19 @@ def batch_roofline(sysb, res, m=20):
         tot += mv * per_mv + max(int(cy), 1) * (m + keep + 2) * 8 * D
     t = res["stats"]["seconds_cycles"]
     ach = tot / t if t > 0 else 0.0
-    return {"bound": "hbm", "achieved": ach / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
+    roof = {"bound": "hbm", "achieved": ach / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
             "frac": ach / PEAK_HBM, "traffic": None, "kernel": "batch_persistent_kernel",
             "algorithmic_bytes_per_launch": tot, "launch_s": t,
             "offdiag_terms_per_matvec_by_size": {str(k): v for k, v in sorted(e_off.items())}}
+    # HBM bytes of the launch from the committed rocprofv3 FETCH_SIZE /
+    # WRITE_SIZE passes over the same workload (copy-kernel calibration)
+    tj = ROOT / "profiles" / "batch_traffic_xxz512.json"
+    if tj.exists():
+        tdat = json.loads(tj.read_text())
+        if tdat.get("basis_size") == m:
+            roof["traffic"] = tdat["fetch_bytes"] + tdat["write_bytes"]
+            roof["traffic_source"] = str(tj.relative_to(ROOT))
+    return roof
 
 
 def run_batch(a, desc, steps=1, warmup=0, with_cpu=True):
@@ -477,12 +491,12 @@ def run_batch(a, desc, steps=1, warmup=0, with_cpu=True):
     dims = sysb.dimensions
     log(f"xxz512: {len(inst)} instances, sum of dimensions {int(dims.sum())}, setup {t_setup:.2f} s")
     for _ in range(warmup):
-        batch.batch_eigsh(sysb.ops, tol=1e-9)
+        batch.batch_eigsh(sysb.ops, tol=1e-9, max_basis_size=BATCH_BASIS)
     device_synchronize()
     t0 = time.perf_counter()
     res = None
     for _ in range(steps):
-        res = batch.batch_eigsh(sysb.ops, tol=1e-9)
+        res = batch.batch_eigsh(sysb.ops, tol=1e-9, max_basis_size=BATCH_BASIS)
         log(f"batch: {res['stats']}")
     device_synchronize()
     per = (time.perf_counter() - t0) / steps

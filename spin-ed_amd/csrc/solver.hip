@@ -447,6 +447,15 @@ __global__ void __launch_bounds__(kBlock)
     w[r] = fma(-theta, y[r], hy[r]);
 }
 
+// b[r] = B(a[r] * s); b may be a (in place)
+template <class B>
+__global__ void __launch_bounds__(kBlock) scale_convert_kernel(const double *a, double s, B *b,
+                                                               uint64_t n) {
+  for (uint64_t r = blockIdx.x * static_cast<uint64_t>(kBlock) + threadIdx.x; r < n;
+       r += static_cast<uint64_t>(gridDim.x) * kBlock)
+    b[r] = static_cast<B>(a[r] * s);
+}
+
 template <class A, class B>
 __global__ void __launch_bounds__(kBlock) convert_kernel(const A *__restrict__ a, B *__restrict__ b,
                                                          uint64_t n) {
@@ -1039,20 +1048,25 @@ bool final_phase(Operator &op, uint64_t n, int m, int keep, uint64_t k, double t
       est[i] = std::sqrt(std::max(rr, 0.0)) / ynorm;
       if (est[i] > tol_accept) ok = false;
       if (evecs) {
-        // normalise and copy out in the requested element type
+        // normalise (and convert) on the device, then one copy into the
+        // caller's column: y is recomputed for the next i, and hy (its
+        // residual norm already read) holds the FP32 values
         const double inv = 1.0 / ynorm;
-        std::vector<double> hv(ne);
-        SPINED_HIP(hipMemcpyAsync(hv.data(), y.ptr, ne * 8, hipMemcpyDeviceToHost, s));
-        SPINED_HIP(hipStreamSynchronize(s));
         // complex: ld counts complex elements, each two reals
         const uint64_t per = C ? 2 : 1;
         if (dtype == LS_FLOAT64 || dtype == LS_COMPLEX128) {
-          double *o = static_cast<double *>(evecs) + i * ld * per;
-          for (uint64_t q = 0; q < ne; ++q) o[q] = hv[q] * inv;
+          scale_convert_kernel<double><<<grid_for(ne), kBlock, 0, s>>>(y.ptr, inv, y.ptr, ne);
+          SPINED_CHECK_LAUNCH();
+          SPINED_HIP(hipMemcpyAsync(static_cast<double *>(evecs) + i * ld * per, y.ptr, ne * 8,
+                                    hipMemcpyDeviceToHost, s));
         } else {
-          float *o = static_cast<float *>(evecs) + i * ld * per;
-          for (uint64_t q = 0; q < ne; ++q) o[q] = static_cast<float>(hv[q] * inv);
+          float *tmp = reinterpret_cast<float *>(hy.ptr);
+          scale_convert_kernel<float><<<grid_for(ne), kBlock, 0, s>>>(y.ptr, inv, tmp, ne);
+          SPINED_CHECK_LAUNCH();
+          SPINED_HIP(hipMemcpyAsync(static_cast<float *>(evecs) + i * ld * per, tmp, ne * 4,
+                                    hipMemcpyDeviceToHost, s));
         }
+        SPINED_HIP(hipStreamSynchronize(s));
       }
     }
     p.harvest_events();

@@ -103,9 +103,38 @@ def prepare_output_file(cfg: UserConfig) -> None:
             f.create_group(g)
 
 
-def build_representatives(cfg: UserConfig) -> None:
+class _PendingWrite:
+    """A dataset written by a background thread while the GPU works (libhdf5
+    runs outside the GIL); wait() joins it and re-raises its error.  Only
+    one thread touches the file at a time: every other access of the output
+    file waits for it first."""
+
+    def __init__(self, path, dataset, data):
+        import threading
+        self._err = None
+
+        def run():
+            try:
+                with hdf5.File(path, "a") as f:
+                    f.write(dataset, data)
+            except BaseException as e:  # re-raised by wait()
+                self._err = e
+        self._t = threading.Thread(target=run, daemon=True)
+        self._t.start()
+
+    def wait(self) -> None:
+        if self._t is not None:
+            self._t.join()
+            self._t = None
+            if self._err is not None:
+                raise self._err
+
+
+def build_representatives(cfg: UserConfig, background: bool = False):
     """buildRepresentatives (SpinED.hs:315-337): reload /basis/representatives
-    when present (ls_build_unsafe), else build and store them."""
+    when present (ls_build_unsafe), else build and store them.  background:
+    return the representatives still to be stored (None when loaded) and
+    leave the store to the caller."""
     reps = None
     with hdf5.File(cfg.output, "a") as f:
         if f.exists(BASIS_PATH + "/representatives"):
@@ -116,9 +145,14 @@ def build_representatives(cfg: UserConfig) -> None:
     I.build_basis(cfg.basis, reps)
     dim = I.get_number_states(cfg.basis)
     log.info("Hilbert space dimension is %d", dim)
-    if reps is None:
-        with hdf5.File(cfg.output, "a") as f:
-            f.write(BASIS_PATH + "/representatives", I.basis_get_states(cfg.basis))
+    if reps is not None:
+        return None
+    states = I.basis_get_states(cfg.basis)
+    if background:
+        return states
+    with hdf5.File(cfg.output, "a") as f:
+        f.write(BASIS_PATH + "/representatives", states)
+    return None
 
 
 def with_datatype(is_real: bool, datatype: str):
@@ -129,21 +163,35 @@ def with_datatype(is_real: bool, datatype: str):
 
 
 def read_checkpoint(cfg: UserConfig, dim: int):
-    """The start vector of an earlier run's checkpoint, or None."""
+    """The start vector of an earlier run's checkpoint, or None: the
+    converged eigenvector of a finished run (marked in /_workspace/lanczos,
+    stored once, in /hamiltonian/eigenvectors), else the Ritz vector of an
+    interrupted one."""
     with hdf5.File(cfg.output, "a") as f:
-        if not f.exists(CHECKPOINT_PATH + "/start_vector"):
+        if (f.exists(CHECKPOINT_PATH + "/converged")
+                and f.exists(HAMILTONIAN_PATH + "/eigenvectors")):
+            v = hdf5.read_eigenvectors(f, dim, np.float64)
+            v = v[:, 0] if v.ndim == 2 and v.shape[0] == dim else np.zeros(0)
+        elif f.exists(CHECKPOINT_PATH + "/start_vector"):
+            v = f.read(CHECKPOINT_PATH + "/start_vector", np.float64)
+        else:
             return None
-        v = f.read(CHECKPOINT_PATH + "/start_vector", np.float64)
     return v if v.shape == (dim,) and np.all(np.isfinite(v)) and np.any(v) else None
 
 
+_UNREAD = object()
+
+
 def diagonalize(cfg: UserConfig, dtype, model=None,
-                checkpoint_every: int = DEFAULT_CHECKPOINT_EVERY):
+                checkpoint_every: int = DEFAULT_CHECKPOINT_EVERY, wait_io=lambda: None,
+                resume=_UNREAD):
     """diagonalize (SpinED.hs:370-411): lowest number_vectors eigenpairs,
     written to /hamiltonian/{eigenvalues, eigenvectors, residuals}; with
     checkpoint_every > 0 the solver's lowest Ritz vector goes to
     /_workspace/lanczos every checkpoint_every restarts, and a checkpoint
-    found there (same dimension, one eigenpair, real sector) is the start."""
+    found there (same dimension, one eigenpair, real sector) is the start
+    (resume: that vector or None, when the caller has read it already).
+    wait_io() is called before every access of the output file."""
     dtype = np.dtype(dtype)
     log.debug("Using max_primme_basis_size=%d, max_primme_block_size=%d, "
               "min_primme_restart_size=%d", cfg.max_basis_size, cfg.max_block_size,
@@ -158,16 +206,27 @@ def diagonalize(cfg: UserConfig, dtype, model=None,
     real = np.dtype(dtype).kind != "c"
     dim = I.get_number_states(cfg.basis)
     start_vector = None
-    resumed = read_checkpoint(cfg, dim) if real and cfg.num_evals == 1 else None
+    net, sign_mask = None, 0
+    resumed = None
+    if real and cfg.num_evals == 1:
+        if resume is _UNREAD:
+            wait_io()
+            resumed = read_checkpoint(cfg, dim)
+        else:
+            resumed = resume
     if resumed is not None:
         log.info("Resuming from the checkpoint in %s...", CHECKPOINT_PATH)
         start_vector = resumed
     elif cfg.spec.start_vector == "learned":
+        # evaluated on the device by the solver: the vector never leaves it
         from . import learned
-        start_vector = learned.start_vector_for(cfg, model)
+        net = learned.model_for(cfg, model)
+        sign_mask = learned.marshall_mask(cfg.spec.basis.number_spins)
 
     def write_checkpoint(v, info):
+        wait_io()
         with hdf5.File(cfg.output, "a") as f:
+            f.delete(CHECKPOINT_PATH + "/converged")
             f.write(CHECKPOINT_PATH + "/start_vector", v.astype(np.float64), warn=False)
             f.write(CHECKPOINT_PATH + "/ritz_values", np.asarray(info["ritz_values"]), warn=False)
             f.write(CHECKPOINT_PATH + "/matvecs", np.array([info["matvecs"]], np.int64),
@@ -175,7 +234,8 @@ def diagonalize(cfg: UserConfig, dtype, model=None,
 
     res = eigsh(cfg.hamiltonian.obj, k=cfg.num_evals, tol=cfg.eps if cfg.eps > 0 else 1e-10,
                 precision=cfg.spec.solver_precision, seed=cfg.spec.seed,
-                start_vector=start_vector, max_basis_size=cfg.max_basis_size,
+                start_vector=start_vector, model=net, sign_mask=sign_mask,
+                max_basis_size=cfg.max_basis_size,
                 min_restart_size=cfg.min_restart_size, max_block_size=cfg.max_block_size,
                 dtype=dtype, monitor=monitor,
                 checkpoint=write_checkpoint if real and checkpoint_every > 0 else None,
@@ -186,15 +246,17 @@ def diagonalize(cfg: UserConfig, dtype, model=None,
     real_t = np.float32 if dtype in (np.float32, np.complex64) else np.float64
     evals = res.evals.astype(real_t)
     rnorms = res.rnorms.astype(real_t)
+    wait_io()
     with hdf5.File(cfg.output, "a") as f:
         f.write(HAMILTONIAN_PATH + "/eigenvalues", evals)
         # KnownDataset (Block a): the column-major D x k block as a row-major (k, D) matrix
         f.write(HAMILTONIAN_PATH + "/eigenvectors", np.ascontiguousarray(res.evecs.T))
         f.write(HAMILTONIAN_PATH + "/residuals", rnorms)
         if real and checkpoint_every > 0 and res.status == 0:
-            # the converged vector: a later run on this file starts from it
-            f.write(CHECKPOINT_PATH + "/start_vector", res.evecs[:, 0].astype(np.float64),
-                    warn=False)
+            # converged: a later run on this file starts from the eigenvector
+            # just written (the interrupted-run vector is no longer needed)
+            f.delete(CHECKPOINT_PATH + "/start_vector")
+            f.write(CHECKPOINT_PATH + "/converged", np.array([1], np.int64), warn=False)
             f.write(CHECKPOINT_PATH + "/ritz_values", np.asarray(res.evals), warn=False)
             f.write(CHECKPOINT_PATH + "/matvecs", np.array([res.stats["matvecs"]], np.int64),
                     warn=False)
@@ -224,10 +286,22 @@ def run_app(path: str, debug: bool = False,
     t0 = time.perf_counter()
     cfg = to_config(read_config(path))
     prepare_output_file(cfg)
-    build_representatives(cfg)
+    to_store = build_representatives(cfg, background=True)
     is_real = I.is_operator_real(cfg.hamiltonian.obj)
     dtype = with_datatype(is_real, cfg.datatype)
-    evals, evecs, rnorms = diagonalize(cfg, dtype, checkpoint_every=checkpoint_every)
+    # a checkpoint is read before the representatives' store starts: the
+    # store then overlaps the solve, and nothing else touches the file
+    # until it is done
+    resume = (read_checkpoint(cfg, I.get_number_states(cfg.basis))
+              if is_real and cfg.num_evals == 1 else None)
+    pending = (_PendingWrite(cfg.output, BASIS_PATH + "/representatives", to_store)
+               if to_store is not None else None)
+    wait_io = pending.wait if pending is not None else (lambda: None)
+    try:
+        evals, evecs, rnorms = diagonalize(cfg, dtype, checkpoint_every=checkpoint_every,
+                                           wait_io=wait_io, resume=resume)
+    finally:
+        wait_io()
     compute_expectations(cfg, evecs)
     return dict(evals=evals, rnorms=rnorms, seconds=time.perf_counter() - t0,
                 dimension=I.get_number_states(cfg.basis), output=cfg.output,

@@ -177,6 +177,11 @@ class File:
                                             H5P_DEFAULT), "create group")
                 lib().H5Gclose(g)
 
+    def delete(self, path: str) -> None:
+        """Unlink a dataset or group when present."""
+        if self.exists(path):
+            lib().H5Ldelete(self.fid, path.encode(), H5P_DEFAULT)
+
     def write(self, path: str, data: np.ndarray, warn: bool = True) -> None:
         """writeDataset (SpinED.hs:354-360): overwrite with a warning (warn=False:
         silently, for the solver's periodic checkpoints)."""

@@ -231,13 +231,19 @@ def load(path: str) -> CNN:
     return CNN(params=np.load(path, allow_pickle=False))
 
 
-def start_vector_for(cfg, model=None):
-    """Host start vector for the YAML front end (`start_vector: learned`,
-    `model: params.npy`); the ring's Marshall sublattice is the even sites."""
+def model_for(cfg, model=None) -> CNN:
+    """The network of the YAML front end (`start_vector: learned`,
+    `model: params.npy`), or the one given."""
     if model is None:
         if not cfg.spec.model:
             raise ValueError("start_vector: learned needs `model:` (a .npy parameter file)")
         model = load(cfg.spec.model)
+    return model
+
+
+def start_vector_for(cfg, model=None):
+    """Host copy of the learned start vector of the YAML front end; the
+    ring's Marshall sublattice is the even sites."""
     n = cfg.spec.basis.number_spins
-    v = model.start_vector_device(cfg.basis, n)
+    v = model_for(cfg, model).start_vector_device(cfg.basis, n)
     return v.numpy()

@@ -125,6 +125,11 @@ def test_hdf5_round_trip(tmp_path):
         # nested groups made level by level; int64; a silent overwrite
         f.write("/_workspace/lanczos/matvecs", np.array([7], np.int64))
         f.write("/_workspace/lanczos/matvecs", np.array([-3, 2 ** 40], np.int64), warn=False)
+        # unlink a dataset; a missing one is no error
+        f.write("/_workspace/lanczos/start_vector", vec[0])
+        f.delete("/_workspace/lanczos/start_vector")
+        f.delete("/_workspace/lanczos/absent")
+        assert not f.exists("/_workspace/lanczos/start_vector")
     with hdf5.File(p, "r") as f:
         assert f.exists("/basis/representatives") and not f.exists("/basis/missing")
         np.testing.assert_array_equal(f.read("/basis/representatives", np.uint64), reps)

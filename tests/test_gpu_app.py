@@ -172,9 +172,11 @@ def test_solver_checkpoints_and_resume():
 
 @pytest.mark.skipif(not hdf5.available(), reason="libhdf5 not found")
 def test_app_workspace_checkpoint_resume(tmp_path):
-    """The application writes its Lanczos checkpoint to /_workspace/lanczos
-    (the converged vector at the end); a second run on the same file starts
-    from it and needs only a few matvecs for the same E0."""
+    """The application writes its Lanczos checkpoint to /_workspace/lanczos;
+    at the end it marks the run converged there instead of storing the
+    eigenvector a second time, and a second run on the same file starts from
+    /hamiltonian/eigenvectors and needs only a few matvecs for the same E0.
+    A run with a learned start evaluates the network on the device."""
     out = tmp_path / "ring24.h5"
     deck = tmp_path / "ring24.yaml"
     perm = [(i + 1) % 24 for i in range(24)]
@@ -203,9 +205,11 @@ output: {out}
     assert not r["resumed"]
     D = r["dimension"]
     with hdf5.File(out, "r") as f:
-        v = f.read("/_workspace/lanczos/start_vector", np.float64)
+        assert not f.exists("/_workspace/lanczos/start_vector")
+        assert f.read("/_workspace/lanczos/converged", np.int64)[0] == 1
         rv = f.read("/_workspace/lanczos/ritz_values", np.float64)
         mv = f.read("/_workspace/lanczos/matvecs", np.int64)
+        v = hdf5.read_eigenvectors(f, D)[:, 0]
     assert v.shape == (D,) and abs(np.linalg.norm(v) - 1.0) <= 1e-8
     np.testing.assert_allclose(rv[0], r["evals"][0], rtol=1e-12)
     assert mv[0] == r["matvecs"]
@@ -214,3 +218,45 @@ output: {out}
     assert r2["matvecs"] < r["matvecs"] // 2
     np.testing.assert_allclose(r2["evals"], r["evals"], rtol=1e-10)
     assert abs(r["evals"][0] - (-42.680058066148944)) <= 1e-10 * 42.68   # Bethe ansatz, N = 24
+
+
+@pytest.mark.skipif(not hdf5.available(), reason="libhdf5 not found")
+def test_app_learned_start_on_device(tmp_path):
+    """`start_vector: learned` with `model:` (a saved parameter file): the
+    solver evaluates the network on the device and the mixed solve reaches
+    the Bethe-ansatz energy (any network gives a valid start)."""
+    from spined import learned
+    from test_oracle_pins import bethe_ground_energy
+    mp = tmp_path / "cnn.npy"
+    learned.save(learned.CNN(seed=3), str(mp))
+    out = tmp_path / "ring20.h5"
+    deck = tmp_path / "ring20.yaml"
+    n = 20
+    deck.write_text(f"""
+basis:
+  number_spins: {n}
+  hamming_weight: {n // 2}
+  spin_inversion: 1
+  symmetries:
+    - permutation: {[(i + 1) % n for i in range(n)]}
+      sector: 0
+    - permutation: {[n - 1 - i for i in range(n)]}
+      sector: 0
+hamiltonian:
+  name: H
+  terms:
+    - matrix: [[1, 0, 0, 0], [0, -1, 2, 0], [0, 2, -1, 0], [0, 0, 0, 1]]
+      sites: {[[i, (i + 1) % n] for i in range(n)]}
+observables: []
+number_vectors: 1
+precision: 1e-10
+output: {out}
+solver_precision: mixed
+start_vector: learned
+model: {mp}
+""")
+    r = app.run_app(str(deck), checkpoint_every=0)
+    e0 = bethe_ground_energy(n)
+    assert abs(r["evals"][0] - e0) <= 1e-10 * abs(e0)
+    assert r["rnorms"][0] <= 1e-10
+    assert not r["resumed"]

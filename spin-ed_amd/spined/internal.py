@@ -175,7 +175,26 @@ def get_number_states(basis: SpinBasis) -> int:
 
 
 def basis_get_states(basis: SpinBasis) -> np.ndarray:
-    """basisGetStates (Internal.hs:238-244): copy out of the borrowed buffer."""
+    """basisGetStates (Internal.hs:238-244): the representatives as an owned
+    array.  One device-to-host copy into the result; ls_get_states, whose
+    host buffer the basis keeps for the states object to borrow, serves the
+    C ABI (get_states_borrowed)."""
+    L = _lib.load()
+    reps = C.c_void_p()
+    stab = C.c_void_p()
+    size = C.c_uint64()
+    check_status(L.spined_basis_device_states(basis.ptr, C.byref(reps), C.byref(stab),
+                                              C.byref(size)))
+    n = int(size.value)
+    out = np.empty(n, np.uint64)
+    if n:
+        check_status(L.spined_memcpy(out.ctypes.data, reps, n * 8, 1))
+    return out
+
+
+def get_states_borrowed(basis: SpinBasis) -> np.ndarray:
+    """ls_get_states / ls_states_get_data / ls_destroy_states, as the
+    Haskell binding uses them (a copy of the borrowed buffer)."""
     L = _lib.load()
     raw = C.c_void_p()
     check_status(L.ls_get_states(C.byref(raw), basis.ptr))

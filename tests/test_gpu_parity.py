@@ -40,6 +40,8 @@ def test_basis_bit_exact(spec):
     oreps, ostab = o.basis()
     assert reps.dtype == np.uint64
     np.testing.assert_array_equal(reps, oreps)
+    # the borrowed buffer of ls_get_states (the Haskell binding's path)
+    np.testing.assert_array_equal(spined.get_states_borrowed(basis), oreps)
     assert spined.get_number_states(basis) == len(oreps)
     d, _ = o.sector_dimension()
     assert len(oreps) == round(d)

@@ -346,9 +346,9 @@ int ls_get_states(ls_states **out, const ls_spin_basis *basis) {
     std::lock_guard<std::mutex> lk(b.mu);
     require_built(b);
     if (!b.host_reps) {
-      auto h = std::make_shared<std::vector<uint64_t>>(b.D);
+      auto h = std::make_shared<HostStates>(b.D);
       if (b.D)
-        SPINED_HIP(hipMemcpy(h->data(), b.reps.ptr, b.D * 8, hipMemcpyDeviceToHost));
+        SPINED_HIP(hipMemcpy(h->data.get(), b.reps.ptr, b.D * 8, hipMemcpyDeviceToHost));
       b.host_reps = h;
     }
     *out = new ls_states{b.host_reps};
@@ -356,10 +356,10 @@ int ls_get_states(ls_states **out, const ls_spin_basis *basis) {
 }
 
 const uint64_t *ls_states_get_data(const ls_states *states) {
-  return states && states->data ? states->data->data() : nullptr;
+  return states && states->data ? states->data->data.get() : nullptr;
 }
 uint64_t ls_states_get_size(const ls_states *states) {
-  return states && states->data ? states->data->size() : 0;
+  return states && states->data ? states->data->size : 0;
 }
 void ls_destroy_states(ls_states *states) { delete states; }
 

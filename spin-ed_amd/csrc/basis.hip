@@ -13,6 +13,7 @@
 // the representatives come out globally sorted: no sort pass is needed.
 // Stabiliser sizes and the bucket index follow.
 #include <algorithm>
+#include <cstring>
 #include <chrono>
 #include <cmath>
 #include <vector>
@@ -559,7 +560,9 @@ void adopt_basis(Basis &b, uint64_t n, const uint64_t *reps) {
   const double dim = sector_dimension(b, &im);
   b.complete = minima && std::fabs(im) < 0.5 && std::llround(dim) == static_cast<long long>(n);
   build_index(b);
-  b.host_reps = std::make_shared<std::vector<uint64_t>>(reps, reps + n);
+  auto h = std::make_shared<HostStates>(n);
+  if (n) std::memcpy(h->data.get(), reps, n * 8);
+  b.host_reps = h;
   b.built = true;
 }
 

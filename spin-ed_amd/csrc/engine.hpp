@@ -83,6 +83,14 @@ void ensure_device(OrbitTables &t);
 // Largest level-2 size of the bucket index: about 2^36 buckets (256 GiB).
 constexpr int kMaxBucketBits = 36;
 
+// Host copy of the representatives (the buffer ls_get_states lends):
+// allocated without initialisation and filled by one copy.
+struct HostStates {
+  std::unique_ptr<uint64_t[]> data;
+  uint64_t size = 0;
+  explicit HostStates(uint64_t n) : data(new uint64_t[n > 0 ? n : 1]), size(n) {}
+};
+
 struct Basis {
   std::shared_ptr<const Group> group;
   int n = 0, w = -1, z = 0;
@@ -109,7 +117,7 @@ struct Basis {
   uint64_t hmask = 0;
   int rank_mode = 0;                  // IndexArgs::rank_mode
   DeviceBuffer<uint32_t> rank_tab;    // w x 64 binomials (IndexArgs)
-  std::shared_ptr<std::vector<uint64_t>> host_reps; // lazily downloaded
+  std::shared_ptr<HostStates> host_reps; // lazily downloaded
   double seconds_build = 0.0;
 
   // reps holds every valid orbit minimum of the sector (set by the

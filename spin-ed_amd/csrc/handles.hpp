@@ -14,7 +14,7 @@ struct ls_spin_basis {
   std::shared_ptr<spined::Basis> p;
 };
 struct ls_states {
-  std::shared_ptr<std::vector<uint64_t>> data;
+  std::shared_ptr<spined::HostStates> data;
 };
 struct ls_interaction {
   std::shared_ptr<const spined::Interaction> p;

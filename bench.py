@@ -61,6 +61,8 @@ def parse():
     p.add_argument("--basis-size", type=int, default=0)
     p.add_argument("--cpu-sample-seconds", type=float, default=15.0)
     p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--no-square", action="store_true",
+                   help="skip the 6x6 square-lattice block Lanczos line (block_square6x6)")
     p.add_argument("--no-batch", action="store_true",
                    help="skip the 512-instance batched solve reported beside the headline")
     p.add_argument("--traffic-json", default=None,
@@ -362,6 +364,13 @@ def main():
         spined.release_device_pool()
         batch_res = run_batch(a, CONFIGS["xxz512"][1], steps=1, warmup=0, with_cpu=not a.no_cpu)
         log(f"xxz512: {batch_res['value']:.1f} instances/s, launch {batch_res['result']['launch_s']:.2f} s")
+    square_res = None
+    if not a.no_square:
+        import gc
+        gc.collect()
+        spined.release_device_pool()
+        square_res = run_square6x6()
+        log(f"square6x6: {square_res['result']}")
 
     e_exact = {40: -70.986091153332861, 24: -42.680058066148944}.get(n)
     out = {
@@ -400,6 +409,7 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "batch_xxz512": batch_res,
+        "block_square6x6": square_res,
     }
     print(json.dumps(out))
     return 0
@@ -477,6 +487,49 @@ def batch_roofline(sysb, res, m=BATCH_BASIS):
             roof["traffic"] = tdat["fetch_bytes"] + tdat["write_bytes"]
             roof["traffic_source"] = str(tj.relative_to(ROOT))
     return roof
+
+
+def run_square6x6():
+    """Heisenberg 6x6 square lattice (72 bonds, w = 18, 36 translations x D4
+    x spin inversion, trivial irrep, 15,804,956 states): lowest eigenvalue to
+    residual 1e-10 by K = 8 block Lanczos with FP64 and with FP32 storage,
+    beside the one-vector FP64 solve; each solve builds its own gather cache
+    inside its time."""
+    import spined
+    from spined import lattice
+    from spined.device import device_synchronize
+    L = 6
+    t = time.perf_counter()
+    basis = spined.mk_basis(spined.mk_group([spined.mk_symmetry(p, s)
+                                             for p, s in lattice.square_symmetries(L)]),
+                            L * L, L * L // 2, 1)
+    op = spined.mk_operator(basis, [spined.mk_interaction(lattice.HEISENBERG,
+                                                          lattice.square_bonds(L))])
+    spined.build_basis(basis)
+    device_synchronize()
+    t_build = time.perf_counter() - t
+    res = {}
+    for name, prec, blk, mb in (("block8_fp64", "fp64", 8, 48), ("block8_mixed", "mixed", 8, 48),
+                                ("onevector_fp64", "fp64", 0, 0)):
+        spined.set_gather_cache(op, 0)
+        spined.set_gather_cache(op, -1)
+        device_synchronize()
+        t0 = time.perf_counter()
+        r = spined.eigsh(op, k=1, tol=1e-10, precision=prec, max_block_size=blk,
+                         max_basis_size=mb, want_vectors=False)
+        device_synchronize()
+        res[name] = {"s": round(time.perf_counter() - t0, 3), "E0": float(r.evals[0]),
+                     "residual": float(r.rnorms[0]), "status": int(r.status),
+                     "vector_matvecs": int(r.stats["matvecs"])}
+    spined.set_gather_cache(op, 0)
+    return {"metric": "s to E0 (residual<=1e-10), 6x6 square lattice, K=8 block Lanczos",
+            "unit": "s", "higher_is_better": False,
+            "config": {"workload": "Heisenberg 6x6 square lattice, 72 bonds, w=18, "
+                                   "36 translations x D4 x spin inversion, trivial irrep",
+                       "dimension": int(spined.get_number_states(basis)),
+                       "basis_build_s": round(t_build, 3),
+                       "E0_published": 4 * 36 * -0.678872},
+            "result": res}
 
 
 def run_batch(a, desc, steps=1, warmup=0, with_cpu=True):

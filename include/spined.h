@@ -313,7 +313,9 @@ int spined_cnn_start_vector(const spined_cnn *model, const ls_spin_basis *basis,
  * B[k][j] = (3k + 7j) % 5 - 2. */
 int spined_debug_mfma_probe(float *out_host);
 /* Runs the device orbit code (the same inline functions the kernels use) on
- * the host for `count` states: method 0 = generic programs, 1 = ring.  For
+ * the host for `count` states: method 0 = generic (the torus factorisation
+ * when the group has one), 1 = ring, 2 = generic with one program per
+ * element (the factorisation off).  For
  * CPU-side verification against the oracle; no compute path uses it. */
 int spined_debug_orbit(const ls_spin_basis *basis, int method, uint64_t count,
                        const uint64_t *states, uint64_t *reps, int32_t *phases, int32_t *counts,
@@ -327,6 +329,11 @@ int spined_debug_orbit_hop(const ls_spin_basis *basis, uint64_t count, const uin
                            uint8_t *valid);
 /* 1 if the basis' group is a ring (rotation/reflection) group. */
 int spined_debug_ring_available(const ls_spin_basis *basis);
+/* Shape l1 x l2 of the torus factorisation that the generic orbit uses (the
+ * group holds every translation of a row-major l1 x l2 torus, site = l1 y +
+ * x, and each element is a translation after one of a few point elements);
+ * 0 x 0 when the orbit runs one program per element. */
+int spined_debug_torus_shape(const ls_spin_basis *basis, int32_t *l1, int32_t *l2);
 /* Plain device copy of `bytes` (multiple of 16): the calibration kernel for
  * HBM byte counters in profiles. */
 int spined_debug_copy_device(void *dst, const void *src, uint64_t bytes);

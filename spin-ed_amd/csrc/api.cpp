@@ -973,7 +973,8 @@ int spined_debug_orbit(const ls_spin_basis *basis, int method, uint64_t count,
     const Basis &b = *basis->p;
     if (method == 1 && !b.tables.ring_available)
       throw Error(LS_INVALID_ARGUMENT, "ring method unavailable for this group");
-    const auto ga = b.tables.generic_args(false);
+    auto ga = b.tables.generic_args(false);
+    if (method == 2) ga.torus_l1 = 0; // one program per element, no torus factorisation
     const auto ra = b.tables.ring_args(false);
     for (uint64_t i = 0; i < count; ++i) {
       const OrbitResult r = method == 1 ? orbit_ring(ra, states[i]) : orbit_generic(ga, states[i]);
@@ -1036,6 +1037,14 @@ uint64_t spined_debug_gather_out_words(void) { return gather_calibration_out_wor
 
 int spined_debug_ring_available(const ls_spin_basis *basis) {
   return basis && basis->p->tables.ring_available ? 1 : 0;
+}
+
+int spined_debug_torus_shape(const ls_spin_basis *basis, int32_t *l1, int32_t *l2) {
+  return guarded([&] {
+    if (!basis || !l1 || !l2) throw Error(LS_INVALID_ARGUMENT, "NULL");
+    *l1 = basis->p->tables.torus_l1;
+    *l2 = basis->p->tables.torus_l2;
+  });
 }
 
 } // extern "C"

@@ -63,6 +63,14 @@ struct OrbitTables {
   std::vector<int32_t> h_phases, h_ring_phases;
   int ring_has_rev = 0;
   uint32_t total_terms = 0;
+  // torus factorisation of the generic orbit (GenericOrbitArgs); l1 = 0: off
+  int torus_l1 = 0, torus_l2 = 0;
+  std::vector<uint32_t> h_point_elem;
+  std::vector<int32_t> h_tp_phase;
+  std::vector<uint64_t> h_row_lo, h_row_hi;
+  DeviceBuffer<uint32_t> point_elem;
+  DeviceBuffer<int32_t> tp_phase;
+  DeviceBuffer<uint64_t> row_lo, row_hi;
   // sqrt(c / s) for stabiliser sizes c, s (divisors of |G|): ratio[i * nd + j]
   // with i = div_idx[c], j = div_idx[s]; nd = 0 when |G| is too large.
   int nd = 0;

@@ -120,6 +120,16 @@ GenericOrbitArgs OrbitTables::generic_args(bool device) const {
   a.has_flip = has_flip;
   a.flip_phase = flip_phase;
   a.L = L;
+  a.trivial = trivial_chars ? 1 : 0;
+  if (torus_l1) {
+    a.torus_l1 = torus_l1;
+    a.torus_l2 = torus_l2;
+    a.n_point = static_cast<int32_t>(h_point_elem.size());
+    a.point_elem = device ? point_elem.ptr : h_point_elem.data();
+    a.tp_phase = device ? tp_phase.ptr : h_tp_phase.data();
+    a.row_lo = device ? row_lo.ptr : h_row_lo.data();
+    a.row_hi = device ? row_hi.ptr : h_row_hi.data();
+  }
   return a;
 }
 
@@ -146,6 +156,78 @@ std::map<int, uint64_t> shift_terms(const std::vector<int> &src_of_dest) {
     terms[static_cast<int>(k) - q] |= 1ull << q;
   }
   return terms;
+}
+
+// Torus factorisation of the group (GenericOrbitArgs): the first l1 x l2
+// = n (l1, l2 >= 2) whose translations t_{dx,dy}, t[i] = site((x_i + dx)
+// mod l1, (y_i + dy) mod l2) with site = l1 y + x, all belong to the group,
+// and a transversal of their cosets: every element is p[t[i]] for exactly
+// one (p, t).  Taken when it needs fewer instructions than one program per
+// element (about 4 per translation against the programs' terms).
+void detect_torus(const Group &g, int n, OrbitTables &t) {
+  t.torus_l1 = t.torus_l2 = 0;
+  t.h_point_elem.clear();
+  t.h_tp_phase.clear();
+  t.h_row_lo.clear();
+  t.h_row_hi.clear();
+  if (g.n != n || n < 4 || t.ring_available) return;
+  std::map<std::vector<uint8_t>, int> index;
+  for (int e = 0; e < t.nperm; ++e) index.emplace(g.perms[e], e);
+  for (int l1 = 2; l1 <= n / 2; ++l1) {
+    if (n % l1) continue;
+    const int l2 = n / l1, nt = n;
+    std::vector<std::vector<uint8_t>> tr(nt, std::vector<uint8_t>(n));
+    bool all = true;
+    for (int dy = 0; dy < l2 && all; ++dy)
+      for (int dx = 0; dx < l1 && all; ++dx) {
+        auto &p = tr[dy * l1 + dx];
+        for (int i = 0; i < n; ++i)
+          p[i] = static_cast<uint8_t>(((i / l1 + dy) % l2) * l1 + (i % l1 + dx) % l1);
+        all = index.count(p) != 0;
+      }
+    if (!all || t.nperm % nt) continue;
+    std::vector<char> covered(t.nperm, 0);
+    std::vector<uint32_t> points;
+    std::vector<int32_t> phases;
+    bool ok = true;
+    for (int e = 0; e < t.nperm && ok; ++e) {
+      if (covered[e]) continue;
+      points.push_back(static_cast<uint32_t>(e));
+      std::vector<uint8_t> c(n);
+      for (int k = 0; k < nt && ok; ++k) {
+        for (int i = 0; i < n; ++i) c[i] = g.perms[e][tr[k][i]];
+        const auto it = index.find(c);
+        if (it == index.end() || covered[it->second]) {
+          ok = false;
+          break;
+        }
+        covered[it->second] = 1;
+        phases.push_back(static_cast<int32_t>(g.phases[it->second]));
+      }
+    }
+    if (!ok || static_cast<int>(points.size()) * nt != t.nperm) continue;
+    uint64_t point_terms = 0;
+    for (uint32_t e : points) point_terms += t.h_offsets[e + 1] - t.h_offsets[e];
+    const uint64_t direct = 2ull * t.total_terms + 2ull * t.nperm;
+    const uint64_t torus = 2ull * point_terms + points.size() * (2ull * l1 + 4ull * nt);
+    if (torus >= direct) return;
+    t.torus_l1 = l1;
+    t.torus_l2 = l2;
+    t.h_point_elem = points;
+    t.h_tp_phase = phases;
+    const uint64_t rows = [&] {
+      uint64_t r = 0;
+      for (int y = 0; y < l2; ++y) r |= 1ull << (y * l1);
+      return r;
+    }();
+    for (int dx = 0; dx < l1; ++dx) {
+      const uint64_t lo = (1ull << (l1 - dx)) - 1;  // row bits 0 .. l1-dx-1
+      const uint64_t hi = ((1ull << l1) - 1) & ~lo; // row bits l1-dx .. l1-1
+      t.h_row_lo.push_back(lo * rows);
+      t.h_row_hi.push_back(hi * rows);
+    }
+    return;
+  }
 }
 
 template <class T> void upload(DeviceBuffer<T> &d, const std::vector<T> &h) {
@@ -227,6 +309,8 @@ void build_orbit_tables(const Group &g, int n, int spin_inversion, OrbitTables &
     }
   }
 
+  detect_torus(g, n, t);
+
   // stabiliser sizes divide |G|: tabulate sqrt(c / s) over its divisors,
   // computed exactly as the row gather would (IEEE division and sqrt)
   const int64_t G = static_cast<int64_t>(t.nperm) * (t.has_flip ? 2 : 1);
@@ -257,6 +341,10 @@ void ensure_device(OrbitTables &t) {
   upload(t.flags, t.h_flags);
   upload(t.phases, t.h_phases);
   upload(t.ring_phases, t.h_ring_phases);
+  upload(t.point_elem, t.h_point_elem);
+  upload(t.tp_phase, t.h_tp_phase);
+  upload(t.row_lo, t.h_row_lo);
+  upload(t.row_hi, t.h_row_hi);
   upload(t.div_idx, t.h_div_idx);
   upload(t.ratio, t.h_ratio);
   t.on_device = true;

@@ -70,6 +70,22 @@ struct GenericOrbitArgs {
   int32_t has_flip;
   int32_t flip_phase;
   int32_t L;
+  // Torus factorisation (torus_l1 > 0): the group holds every translation
+  // t_{dx,dy} of an l1 x l2 torus in row-major site order (site = l1 y + x;
+  // (t.x) at (x, y) = x at ((x + dx) mod l1, (y + dy) mod l2)), and every
+  // element is t_{dx,dy} after one of the n_point elements point_elem[p]
+  // (a transversal of the translation cosets).  tp_phase[p * l1 l2 +
+  // dy l1 + dx] is that element's phase (not read when trivial);
+  // row_lo[dx] / row_hi[dx] select, in every row, the bits that the in-row
+  // rotation by dx takes from the right / wraps around from the left.
+  const uint32_t *__restrict__ point_elem;
+  const int32_t *__restrict__ tp_phase;
+  const uint64_t *__restrict__ row_lo;
+  const uint64_t *__restrict__ row_hi;
+  int32_t torus_l1;
+  int32_t torus_l2;
+  int32_t n_point;
+  int32_t trivial; // every phase is 0 (the flip's included)
 };
 
 __host__ __device__ inline uint64_t apply_program(const GenericOrbitArgs &a, int g, uint64_t x,
@@ -85,7 +101,40 @@ __host__ __device__ inline uint64_t apply_program(const GenericOrbitArgs &a, int
   return y;
 }
 
+// Orbit over the torus factorisation: the point element's program once,
+// then the l1 in-row rotations (two masked shifts each) and, for each, the
+// l2 row rotations (one rotation of the word by l1 sites each).  Same
+// (rep, count, valid) as the scan over every element; the phase is that of
+// an element reaching rep (all of them carry it when valid).
+__host__ __device__ inline OrbitResult orbit_torus(const GenericOrbitArgs &a, uint64_t x) {
+  const uint64_t rx = reverse_n(x, a.n_sites);
+  const int l1 = a.torus_l1, l2 = a.torus_l2, n = a.n_sites;
+  const uint64_t mask = a.site_mask;
+  uint64_t best = ~0ull;
+  int32_t ph = 0, cnt = 0;
+  bool ok = true;
+  for (int p = 0; p < a.n_point; ++p) {
+    const uint64_t y = apply_program(a, static_cast<int>(a.point_elem[p]), x, rx);
+    const int32_t *tph = a.tp_phase + p * l1 * l2;
+    for (int dx = 0; dx < l1; ++dx) {
+      uint64_t z = ((y >> dx) & a.row_lo[dx]) | ((y << ((l1 - dx) & 63)) & a.row_hi[dx]);
+      for (int dy = 0; dy < l2; ++dy) {
+        const int32_t q = a.trivial ? 0 : tph[dy * l1 + dx];
+        orbit_update(best, ph, cnt, ok, z, q);
+        if (a.has_flip) {
+          int32_t qf = q + a.flip_phase;
+          if (qf >= a.L) qf -= a.L;
+          orbit_update(best, ph, cnt, ok, z ^ mask, qf);
+        }
+        z = ((z >> l1) | (z << (n - l1))) & mask;
+      }
+    }
+  }
+  return OrbitResult{best, ph, cnt, ok};
+}
+
 __host__ __device__ inline OrbitResult orbit_generic(const GenericOrbitArgs &a, uint64_t x) {
+  if (a.torus_l1) return orbit_torus(a, x);
   const uint64_t rx = reverse_n(x, a.n_sites);
   uint64_t best = ~0ull;
   int32_t ph = 0, cnt = 0;
@@ -106,7 +155,43 @@ __host__ __device__ inline OrbitResult orbit_generic(const GenericOrbitArgs &a, 
 // Early-exit representative test used by the basis builder: returns the
 // stabiliser size if x is a representative whose symmetric state exists,
 // else 0.
+// rep_test_generic over the torus factorisation (same result).
+__host__ __device__ inline uint32_t rep_test_torus(const GenericOrbitArgs &a, uint64_t x) {
+  const uint64_t rx = reverse_n(x, a.n_sites);
+  const int l1 = a.torus_l1, l2 = a.torus_l2, n = a.n_sites;
+  const uint64_t mask = a.site_mask;
+  uint32_t cnt = 0;
+  for (int p = 0; p < a.n_point; ++p) {
+    const uint64_t y = apply_program(a, static_cast<int>(a.point_elem[p]), x, rx);
+    const int32_t *tph = a.tp_phase + p * l1 * l2;
+    for (int dx = 0; dx < l1; ++dx) {
+      uint64_t z = ((y >> dx) & a.row_lo[dx]) | ((y << ((l1 - dx) & 63)) & a.row_hi[dx]);
+      for (int dy = 0; dy < l2; ++dy) {
+        const int32_t q = a.trivial ? 0 : tph[dy * l1 + dx];
+        if (z < x) return 0;
+        if (z == x) {
+          if (q != 0) return 0;
+          ++cnt;
+        }
+        if (a.has_flip) {
+          const uint64_t zf = z ^ mask;
+          if (zf < x) return 0;
+          if (zf == x) {
+            int32_t qf = q + a.flip_phase;
+            if (qf >= a.L) qf -= a.L;
+            if (qf != 0) return 0;
+            ++cnt;
+          }
+        }
+        z = ((z >> l1) | (z << (n - l1))) & mask;
+      }
+    }
+  }
+  return cnt;
+}
+
 __host__ __device__ inline uint32_t rep_test_generic(const GenericOrbitArgs &a, uint64_t x) {
+  if (a.torus_l1) return rep_test_torus(a, x);
   const uint64_t rx = reverse_n(x, a.n_sites);
   uint32_t cnt = 0;
   for (int g = 0; g < a.n_perm; ++g) {

@@ -40,7 +40,7 @@ SPINED_SYMBOLS = (
     "spined_basis_is_complete", "spined_operator_set_gather_cache",
     "spined_operator_get_gather_cache",
     "spined_basis_sector_dimension", "spined_debug_orbit",
-    "spined_debug_ring_available", "spined_debug_orbit_hop", "spined_device_synchronize",
+    "spined_debug_ring_available", "spined_debug_torus_shape", "spined_debug_orbit_hop", "spined_device_synchronize",
     "spined_device_malloc", "spined_device_release_pool",
     "spined_device_free", "spined_memcpy", "spined_memset", "spined_cnn_num_params",
     "spined_cnn_create", "spined_cnn_destroy", "spined_cnn_get_params", "spined_cnn_set_params",
@@ -192,6 +192,7 @@ def load():
         "spined_basis_sector_dimension": (I, [P, C.POINTER(D)]),
         "spined_debug_orbit": (I, [P, I, U64, P, P, P, P, P]),
         "spined_debug_ring_available": (I, [P]),
+        "spined_debug_torus_shape": (I, [P, P, P]),
         "spined_debug_orbit_hop": (I, [P, U64, P, P, P, P, P, P]),
         "spined_device_synchronize": (I, []),
         "spined_device_malloc": (I, [PP, U64]),

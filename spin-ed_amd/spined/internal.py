@@ -481,3 +481,10 @@ def debug_orbit_hop(basis: SpinBasis, states, sites):
 
 def ring_available(basis: SpinBasis) -> bool:
     return bool(_lib.load().spined_debug_ring_available(basis.ptr))
+
+
+def torus_shape(basis: SpinBasis):
+    """(l1, l2) of the generic orbit's torus factorisation; (0, 0) when off."""
+    l1, l2 = C.c_int32(), C.c_int32()
+    check_status(_lib.load().spined_debug_torus_shape(basis.ptr, C.byref(l1), C.byref(l2)))
+    return l1.value, l2.value

@@ -162,6 +162,60 @@ def test_device_orbit_code_matches_oracle(n, w, z, gens):
         assert all(p == r[1] for p, r, v in zip(ph, ref, val) if v)
 
 
+def _torus_gens(l1, l2, kx, ky, reflect=False):
+    """Translations of a row-major l1 x l2 torus (site = l1 y + x) with
+    momenta (kx, ky), optionally the in-row reflection (sector 0)."""
+    idx = lambda x, y: (y % l2) * l1 + (x % l1)
+    gens = [([idx(x + 1, y) for y in range(l2) for x in range(l1)], kx),
+            ([idx(x, y + 1) for y in range(l2) for x in range(l1)], ky)]
+    if reflect:
+        gens.append(([idx(l1 - 1 - x, y) for y in range(l2) for x in range(l1)], 0))
+    return gens
+
+
+@pytest.mark.parametrize("n,w,z,gens,shape", [
+    (36, 18, 1, lattice.square_symmetries(6), (6, 6)),
+    (16, 8, -1, lattice.square_symmetries(4), (4, 4)),
+    (25, 12, None, lattice.square_symmetries(5), (5, 5)),
+    (9, None, 1, lattice.square_symmetries(3), (3, 3)),
+    (24, 12, 1, _torus_gens(4, 6, 0, 0, True), (4, 6)),
+    (24, None, None, _torus_gens(4, 6, 1, 2), (4, 6)),
+    (24, 12, -1, _torus_gens(6, 4, 3, 2), (6, 4)),
+    (20, 10, 1, _torus_gens(5, 4, 0, 2, True), (5, 4)),
+    (12, 6, 1, lattice.ring_symmetries(12, 0, 0), (0, 0)),
+    (12, 6, None, [(lattice.ring_translation(12), 0)] + [([(i + 2) % 12 for i in range(12)], 0)],
+     (0, 0)),
+])
+def test_torus_factorised_orbit(n, w, z, gens, shape):
+    """The generic orbit over the torus factorisation (a point element's
+    program, then in-row and row rotations) gives the scan over every
+    element's program exactly: representative, stabiliser count and
+    validity, and the phase wherever the orbit is valid; both equal the
+    oracle.  Cases: square tori with their point groups, rectangles with
+    non-zero momenta (complex phases) and spin inversion; ring groups keep
+    the scan (no torus)."""
+    b = spined.mk_basis(spined.mk_group([spined.mk_symmetry(p, s) for p, s in gens]), n, w, z)
+    assert spined.torus_shape(b) == shape
+    rng = np.random.default_rng(n + 7)
+    if w is None:
+        st = rng.integers(0, 1 << n, 1500, dtype=np.uint64)
+    else:
+        st = np.array([sum(1 << int(i) for i in rng.choice(n, w, replace=False))
+                       for _ in range(1500)], dtype=np.uint64)
+    fac = spined.debug_orbit(b, st, 0)
+    scan = spined.debug_orbit(b, st, 2)
+    for q in (0, 2, 3):
+        np.testing.assert_array_equal(fac[q], scan[q])
+    v = fac[3].astype(bool)
+    np.testing.assert_array_equal(fac[1][v], scan[1][v])
+    o = O.System(n, w, z, gens)
+    ref = [o.orbit(int(x)) for x in st]
+    assert fac[0].tolist() == [r[0] for r in ref]
+    assert fac[2].tolist() == [r[2] for r in ref]
+    assert fac[3].tolist() == [r[3] for r in ref]
+    assert all(p == r[1] for p, r, ok in zip(fac[1], ref, v) if ok)
+
+
 def test_build_unsafe_rejects_unsorted_input():
     g = spined.mk_group([])
     b = spined.mk_basis(g, 6, 3)

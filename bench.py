@@ -37,6 +37,8 @@ CONFIGS = {
     "ring32": (32, "Heisenberg ring N=32, w=16, k=0, p=+1, z=+1"),
     "ring28": (28, "Heisenberg ring N=28, w=14, k=0, p=+1, z=+1"),
     "ring24": (24, "Heisenberg ring N=24, w=12, k=0, p=+1, z=+1 (28,968 states)"),
+    "square6x6": (36, "Heisenberg 6x6 square lattice, w=18, 36 translations x D4 x spin "
+                      "inversion: K=8 block Lanczos (FP64, FP32 storage) and one-vector FP64"),
     "xxz512": (0, "512 XXZ rings, N uniform over {12..26}, Delta~U(0,2), J_b~U(0.8,1.2), "
                   "w=N/2, z=+1, residual 1e-9, one persistent batched solver launch"),
 }
@@ -110,6 +112,9 @@ def main():
     from spined.device import DeviceArray, device_synchronize
 
     n, desc = CONFIGS[a.config]
+    if a.config == "square6x6":
+        print(json.dumps(run_square6x6()), flush=True)
+        return 0
     if a.config == "xxz512":
         return bench_batch(a, desc)
     big = n >= 36

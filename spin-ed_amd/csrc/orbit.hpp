@@ -120,11 +120,16 @@ __host__ __device__ inline OrbitResult orbit_torus(const GenericOrbitArgs &a, ui
       uint64_t z = ((y >> dx) & a.row_lo[dx]) | ((y << ((l1 - dx) & 63)) & a.row_hi[dx]);
       for (int dy = 0; dy < l2; ++dy) {
         const int32_t q = a.trivial ? 0 : tph[dy * l1 + dx];
-        orbit_update(best, ph, cnt, ok, z, q);
         if (a.has_flip) {
+          // z and z ^ mask differ, so only the smaller can reach or beat
+          // best: one update with it counts exactly what two would
+          const uint64_t zf = z ^ mask;
           int32_t qf = q + a.flip_phase;
           if (qf >= a.L) qf -= a.L;
-          orbit_update(best, ph, cnt, ok, z ^ mask, qf);
+          const bool lo = zf < z;
+          orbit_update(best, ph, cnt, ok, lo ? zf : z, lo ? qf : q);
+        } else {
+          orbit_update(best, ph, cnt, ok, z, q);
         }
         z = ((z >> l1) | (z << (n - l1))) & mask;
       }
@@ -142,11 +147,15 @@ __host__ __device__ inline OrbitResult orbit_generic(const GenericOrbitArgs &a, 
   for (int g = 0; g < a.n_perm; ++g) {
     const uint64_t y = apply_program(a, g, x, rx);
     const int32_t p = a.phases[g];
-    orbit_update(best, ph, cnt, ok, y, p);
     if (a.has_flip) {
+      // y and y ^ mask differ: one update with the smaller (orbit_torus)
+      const uint64_t yf = y ^ a.site_mask;
       int32_t pf = p + a.flip_phase;
       if (pf >= a.L) pf -= a.L;
-      orbit_update(best, ph, cnt, ok, y ^ a.site_mask, pf);
+      const bool lo = yf < y;
+      orbit_update(best, ph, cnt, ok, lo ? yf : y, lo ? pf : p);
+    } else {
+      orbit_update(best, ph, cnt, ok, y, p);
     }
   }
   return OrbitResult{best, ph, cnt, ok};

@@ -13,7 +13,7 @@ import pyoracle
 import spined
 from spined import lattice
 from spined.internal import LatticeSymmetriesException
-from helpers import Spec, engine, oracle, ring
+from helpers import Spec, engine, oracle, ring, torus
 from test_oracle_pins import bethe_ground_energy
 
 pytestmark = pytest.mark.gpu
@@ -28,6 +28,13 @@ BASIS_CASES = [
     ring(14, k=None, p=None, z=1, name="xxz14_flip"),
     Spec(16, 8, 1, lattice.square_symmetries(4), [(lattice.HEISENBERG, lattice.square_bonds(4))], "square4x4"),
     Spec(12, None, None, [(lattice.ring_translation(12), 2)], [(lattice.HEISENBERG, lattice.ring_bonds(12))], "ring12_now_k2"),
+    # the generic orbit's torus factorisation: complex momenta, reflections, flips
+    torus(4, 3, kx=1, ky=2),
+    torus(3, 4, reflect=True, z=-1),
+    torus(4, 4, kx=2, ky=0, z=1),
+    torus(6, 3, kx=0, ky=1, reflect=True),
+    Spec(25, 12, None, lattice.square_symmetries(5), [(lattice.HEISENBERG, lattice.square_bonds(5))],
+         "square5x5"),
 ]
 
 
@@ -45,6 +52,8 @@ def test_basis_bit_exact(spec):
     assert spined.get_number_states(basis) == len(oreps)
     d, _ = o.sector_dimension()
     assert len(oreps) == round(d)
+    if spec.name.startswith(("torus", "square")):
+        assert spined.torus_shape(basis) != (0, 0)
 
 
 def _ulp_diff(a, b):

@@ -184,6 +184,30 @@ __global__ void svqb_kernel(const double *__restrict__ G, double *__restrict__ X
     }
 }
 
+// Gram-Schmidt test after one projection pass (C1 = V^T W over `rows`
+// basis vectors, G = W'^T W' of the projected block): column c keeps
+// |w'_c|^2 = G_cc of its |w_c|^2 = G_cc + sum_q C1[q][c]^2 (V orthonormal).
+// When every column keeps at least 1e-4 of it (1% of the norm: at most two
+// digits lost to cancellation, the criterion of the one-vector phase) one
+// pass suffices and *need = 0.  C2, the second pass's coefficients, is
+// zeroed either way (a second pass overwrites it).
+template <int K>
+__global__ void second_pass_test_kernel(const double *__restrict__ C1, int rows,
+                                        const double *__restrict__ G, double *__restrict__ C2,
+                                        int *__restrict__ need) {
+  for (int e = threadIdx.x; e < rows * K; e += blockDim.x) C2[e] = 0.0;
+  if (threadIdx.x == 0) {
+    int any = 0;
+    for (int c = 0; c < K; ++c) {
+      double cn = 0.0;
+      for (int q = 0; q < rows; ++q) cn = fma(C1[q * K + c], C1[q * K + c], cn);
+      const double wn = G[c * K + c];
+      if (!(wn >= 1e-4 * (wn + cn))) any = 1; // NaN: take the second pass
+    }
+    *need = any;
+  }
+}
+
 // T entries of block column j: rows q < (j+1)K from C1 + C2 (the block
 // diagonal symmetrised), rows of block j+1 from B; mirrored.
 template <int K>
@@ -313,6 +337,7 @@ template <class VT, int K> struct BlockPhase {
   DeviceBuffer<int> status;
   std::vector<hipEvent_t> ev;
   int64_t matvecs = 0; // vector applications (K per block matvec)
+  int64_t second_passes = 0; // steps whose Gram-Schmidt took a second pass
   double matvec_seconds = 0.0;
   unsigned Gd;
   int mlast = 0;
@@ -341,7 +366,7 @@ template <class VT, int K> struct BlockPhase {
     T.allocate(kMaxBasis * kMaxBasis);
     theta.allocate(kMaxBasis);
     S.allocate(kMaxBasis * kMaxBasis);
-    status.allocate(2);
+    status.allocate(3);
     reset_projected_kernel<<<1, 256, 0, s>>>(T.ptr, kMaxBasis, theta.ptr, 0);
     SPINED_CHECK_LAUNCH();
   }
@@ -387,19 +412,32 @@ template <class VT, int K> struct BlockPhase {
     SPINED_CHECK_LAUNCH();
   }
 
-  void reorth(int nblk, double *c1, double *c2) {
+  // Classical Gram-Schmidt of W against the first nblk blocks: one pass,
+  // and a second when second_pass_test_kernel asks for it.  Returns true
+  // when G already holds W^T W of the projected block (one pass: the test's
+  // Gram matrix is the first SVQB pass's).
+  bool reorth(int nblk, double *c1, double *c2) {
     dots<VT>(vptr.ptr, nblk, c1);
     block_sub_kernel<VT, K><<<Gd, kBlock, 0, s>>>(vptr.ptr, nblk, c1, W.ptr, n);
     SPINED_CHECK_LAUNCH();
+    dots<double>(wptr.ptr, 1, G.ptr);
+    second_pass_test_kernel<K><<<1, 256, 0, s>>>(c1, nblk * K, G.ptr, c2, status.ptr + 2);
+    SPINED_CHECK_LAUNCH();
+    int need = 1;
+    SPINED_HIP(hipMemcpyAsync(&need, status.ptr + 2, sizeof(int), hipMemcpyDeviceToHost, s));
+    SPINED_HIP(hipStreamSynchronize(s));
+    if (!need) return true;
+    ++second_passes;
     dots<VT>(vptr.ptr, nblk, c2);
     block_sub_kernel<VT, K><<<Gd, kBlock, 0, s>>>(vptr.ptr, nblk, c2, W.ptr, n);
     SPINED_CHECK_LAUNCH();
+    return false;
   }
 
   // W = Q B with Q orthonormal (two SVQB passes) stored into block `dst`;
-  // returns false when W is rank deficient.
-  bool orthonormalize_into(int dst) {
-    dots<double>(wptr.ptr, 1, G.ptr);
+  // returns false when W is rank deficient.  gram_ready: G holds W^T W.
+  bool orthonormalize_into(int dst, bool gram_ready = false) {
+    if (!gram_ready) dots<double>(wptr.ptr, 1, G.ptr);
     svqb_kernel<K><<<1, 64, 0, s>>>(G.ptr, X.ptr, nullptr, Bt.ptr, status.ptr);
     SPINED_CHECK_LAUNCH();
     block_transform_kernel<K, double><<<Gd, kBlock, 0, s>>>(W.ptr, X.ptr, W.ptr, n);
@@ -425,8 +463,8 @@ template <class VT, int K> struct BlockPhase {
   bool step(int j) {
     block_matvec(hptr[j], W.ptr);
     double *c1 = C.ptr, *c2 = C.ptr + static_cast<size_t>(mb + 1) * K * K;
-    reorth(j + 1, c1, c2);
-    const bool ok = orthonormalize_into(j + 1);
+    const bool gram = reorth(j + 1, c1, c2);
+    const bool ok = orthonormalize_into(j + 1, gram);
     store_block_column_kernel<K><<<1, 256, 0, s>>>(c1, c2, B.ptr, j, j + 1 == mb ? 1 : 0, T.ptr,
                                                    kMaxBasis);
     SPINED_CHECK_LAUNCH();
@@ -491,8 +529,10 @@ template <class VT, int K> struct BlockPhase {
       harvest_events();
       ++res.cycles;
       if (verbose)
-        log_line("block phase %d (K=%d) cycle %d matvecs %lld theta0 %.15g est %.3e", phase_id,
-                 K, res.cycles, static_cast<long long>(matvecs), th[0], worst);
+        log_line("block phase %d (K=%d) cycle %d matvecs %lld theta0 %.15g est %.3e "
+                 "second Gram-Schmidt passes %lld",
+                 phase_id, K, res.cycles, static_cast<long long>(matvecs), th[0], worst,
+                 static_cast<long long>(second_passes));
       int stop = 0;
       if (monitor) {
         spined_monitor_info info{matvecs, res.cycles, phase_id, nconv, k, res.theta.data(),

@@ -160,6 +160,28 @@ def test_block_eigsh_mixed_ring26():
     assert r.stats["matvecs_mixed"] > 0
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_block_eigsh_invariant_subspace(precision, capfd):
+    """One exchange bond on 18 sites (w = 9, D = 48,620): H = 1 - 4 P (P the
+    bond's singlet projector), so the second block of the Krylov space
+    already closes it and the Gram-Schmidt of the next block cancels to
+    rounding: the second pass runs there (the log counts it) and the block
+    turns rank deficient; the solve still ends at E0 = -3."""
+    basis = spined.mk_basis(spined.mk_group([]), 18, 9, None)
+    op = spined.mk_operator(basis, [spined.mk_interaction(lattice.HEISENBERG, [[0, 1]])])
+    spined.build_basis(basis)
+    r = spined.eigsh(op, k=1, tol=1e-10, max_block_size=8, precision=precision, verbose=True,
+                     max_matvecs=3000)
+    err = capfd.readouterr().err
+    print(r.evals, r.rnorms, r.stats, flush=True)
+    assert r.status == 0
+    assert abs(r.evals[0] + 3.0) <= 1e-12
+    assert r.rnorms[0] <= 1e-10
+    counts = [int(line.rsplit(" ", 1)[1]) for line in err.splitlines()
+              if "second Gram-Schmidt passes" in line]
+    assert counts and max(counts) >= 1
+
+
 # ---- row schedule (rows of equal work side by side) ----------------------------
 @pytest.mark.parametrize("spec", [ring(20, k=0, p=0, z=1), ring(24), ring(16, k=0, p=0, z=1),
                                   ring(12, k=0, p=0, z=None, w=None, matrix=SIGMA_XX,

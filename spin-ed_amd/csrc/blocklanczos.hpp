@@ -23,14 +23,18 @@ namespace {
 constexpr int kMaxKeep = 32;         // kept Ritz vectors per restart (registers)
 constexpr int kMaxBlockCycles = 400; // the one-vector phase takes over after this
 
-// partial[(b*K*K + a*K + c) * G + g] = sum over this workgroup's rows of
-// V_b[r][a] * W[r][c] (FP64), for blocks b = blockIdx.y.
+// partial[(b*K*K + a*K + c) * G + g] = sum over the rows of chunk g of
+// V_b[r][a] * W[r][c] (FP64), for blocks b = blockIdx.x and chunks g =
+// blockIdx.y (G = gridDim.y): the workgroups of one chunk are dispatched
+// side by side, so W's rows, read once per block b, come from the Infinity
+// Cache after the first read.
 template <class VT, int K>
 __global__ void __launch_bounds__(kBlock)
     block_dot_kernel(const VT *const *__restrict__ V, const double *__restrict__ W, uint64_t n,
                      double *__restrict__ partial) {
   __shared__ double lds[K * K][kBlock / 64];
-  const int b = blockIdx.y;
+  const int b = blockIdx.x;
+  const uint64_t g = blockIdx.y, ng = gridDim.y;
   const RowK<VT, K> *vb = reinterpret_cast<const RowK<VT, K> *>(V[b]);
   const RowK<double, K> *wr = reinterpret_cast<const RowK<double, K> *>(W);
   double acc[K][K];
@@ -38,8 +42,7 @@ __global__ void __launch_bounds__(kBlock)
   for (int a = 0; a < K; ++a)
 #pragma unroll
     for (int c = 0; c < K; ++c) acc[a][c] = 0.0;
-  for (uint64_t r = blockIdx.x * static_cast<uint64_t>(kBlock) + threadIdx.x; r < n;
-       r += static_cast<uint64_t>(gridDim.x) * kBlock) {
+  for (uint64_t r = g * kBlock + threadIdx.x; r < n; r += ng * kBlock) {
     const RowK<VT, K> v = vb[r];
     const RowK<double, K> w = wr[r];
 #pragma unroll
@@ -60,7 +63,7 @@ __global__ void __launch_bounds__(kBlock)
   if (threadIdx.x < K * K) {
     double s = 0.0;
     for (int q = 0; q < kBlock / 64; ++q) s += lds[threadIdx.x][q];
-    partial[(static_cast<uint64_t>(b) * K * K + threadIdx.x) * gridDim.x + blockIdx.x] = s;
+    partial[(static_cast<uint64_t>(b) * K * K + threadIdx.x) * ng + g] = s;
   }
 }
 
@@ -406,7 +409,7 @@ template <class VT, int K> struct BlockPhase {
 
   // out[(b*K + a)*K + c] = sum_r V_b[r][a] W[r][c] for b < nblk
   template <class PT> void dots(const PT *const *V, int nblk, double *out) {
-    block_dot_kernel<PT, K><<<dim3(Gd, nblk), kBlock, 0, s>>>(V, W.ptr, n, partial.ptr);
+    block_dot_kernel<PT, K><<<dim3(nblk, Gd), kBlock, 0, s>>>(V, W.ptr, n, partial.ptr);
     SPINED_CHECK_LAUNCH();
     reduce_partials_kernel<<<nblk * K * K, kBlock, 0, s>>>(partial.ptr, nblk * K * K, Gd, out, 0);
     SPINED_CHECK_LAUNCH();

@@ -315,7 +315,8 @@ int spined_debug_mfma_probe(float *out_host);
 /* Runs the device orbit code (the same inline functions the kernels use) on
  * the host for `count` states: method 0 = generic (the torus factorisation
  * when the group has one), 1 = ring, 2 = generic with one program per
- * element (the factorisation off).  For
+ * element (the factorisation off), 3 = the generic orbit minimum alone
+ * (reps only; phases, counts and valid are left untouched).  For
  * CPU-side verification against the oracle; no compute path uses it. */
 int spined_debug_orbit(const ls_spin_basis *basis, int method, uint64_t count,
                        const uint64_t *states, uint64_t *reps, int32_t *phases, int32_t *counts,

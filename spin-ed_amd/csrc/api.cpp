@@ -977,6 +977,10 @@ int spined_debug_orbit(const ls_spin_basis *basis, int method, uint64_t count,
     if (method == 2) ga.torus_l1 = 0; // one program per element, no torus factorisation
     const auto ra = b.tables.ring_args(false);
     for (uint64_t i = 0; i < count; ++i) {
+      if (method == 3) { // the minimum alone (orbit_min_generic); no phase, count or validity
+        reps[i] = orbit_min_generic(ga, states[i]);
+        continue;
+      }
       const OrbitResult r = method == 1 ? orbit_ring(ra, states[i]) : orbit_generic(ga, states[i]);
       reps[i] = r.rep;
       if (phases) phases[i] = r.phase;

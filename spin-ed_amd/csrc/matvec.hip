@@ -124,18 +124,27 @@ __global__ void __launch_bounds__(kRowBlock)
   const RowOrbit<METHOD, BONDS> rob(ra, bd, r);
   const int32_t si = static_cast<int32_t>(stab[i]);
   uint32_t *out = entries + woff[blockIdx.x] * 64 + threadIdx.x;
+  // generic groups with trivial characters: the orbit minimum alone (no
+  // phase can differ), |Stab| of the target from the basis' entry
+  const bool min_only = METHOD == 0 && ga.trivial;
   auto entry = [&](int64_t k, const OrbitResult &orb) {
     if (k < 0) return kCacheIndex;
+    const int32_t cnt = min_only ? static_cast<int32_t>(stab[k]) : orb.count;
     return static_cast<uint32_t>(k) | (orb.phase != 0 ? kCacheNeg : 0u) |
-           (orb.count != si ? kCacheRatio : 0u);
+           (cnt != si ? kCacheRatio : 0u);
+  };
+  auto orbit_of = [&](int j) {
+    if (min_only)
+      return OrbitResult{orbit_min_generic(ga, r ^ (BONDS ? bd.flip[j] : o.flip[j])), 0, 0, true};
+    return rob.orbit(o, bd, ga, ra, r, j);
   };
   int t = 0;
   while (any_set<NW>(act)) {
     const int ja = pop_first<NW>(act);
     const int jb = pop_first<NW>(act);
-    const OrbitResult oa = rob.orbit(o, bd, ga, ra, r, ja);
+    const OrbitResult oa = orbit_of(ja);
     OrbitResult ob{0, 0, 0, false};
-    if (jb >= 0) ob = rob.orbit(o, bd, ga, ra, r, jb);
+    if (jb >= 0) ob = orbit_of(jb);
     int64_t ka, kb;
     find_index2(ix, oa.rep, oa.valid, ob.rep, ob.valid, ka, kb);
     out[64 * static_cast<uint64_t>(t++)] = entry(ka, oa);

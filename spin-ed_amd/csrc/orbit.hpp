@@ -161,6 +161,45 @@ __host__ __device__ inline OrbitResult orbit_generic(const GenericOrbitArgs &a, 
   return OrbitResult{best, ph, cnt, ok};
 }
 
+// Orbit minimum alone, for sectors whose characters are all trivial (every
+// orbit is then valid with phase 0, and |Stab| is the basis' stabiliser
+// entry of the representative): the scans of orbit_torus / orbit_generic
+// without the count and phase bookkeeping.  Same rep as orbit_generic.
+__host__ __device__ inline uint64_t orbit_min_generic(const GenericOrbitArgs &a, uint64_t x) {
+  const uint64_t rx = reverse_n(x, a.n_sites);
+  const uint64_t mask = a.site_mask;
+  uint64_t best = ~0ull;
+  if (a.torus_l1) {
+    const int l1 = a.torus_l1, l2 = a.torus_l2, n = a.n_sites;
+    for (int p = 0; p < a.n_point; ++p) {
+      const uint64_t y = apply_program(a, static_cast<int>(a.point_elem[p]), x, rx);
+      for (int dx = 0; dx < l1; ++dx) {
+        uint64_t z = ((y >> dx) & a.row_lo[dx]) | ((y << ((l1 - dx) & 63)) & a.row_hi[dx]);
+        for (int dy = 0; dy < l2; ++dy) {
+          uint64_t m = z;
+          if (a.has_flip) {
+            const uint64_t zf = z ^ mask;
+            m = zf < z ? zf : z;
+          }
+          best = m < best ? m : best;
+          z = ((z >> l1) | (z << (n - l1))) & mask;
+        }
+      }
+    }
+    return best;
+  }
+  for (int g = 0; g < a.n_perm; ++g) {
+    const uint64_t y = apply_program(a, g, x, rx);
+    uint64_t m = y;
+    if (a.has_flip) {
+      const uint64_t yf = y ^ mask;
+      m = yf < y ? yf : y;
+    }
+    best = m < best ? m : best;
+  }
+  return best;
+}
+
 // Early-exit representative test used by the basis builder: returns the
 // stabiliser size if x is a representative whose symmetric state exists,
 // else 0.

@@ -191,7 +191,7 @@ def test_torus_factorised_orbit(n, w, z, gens, shape):
     program, then in-row and row rotations) gives the scan over every
     element's program exactly: representative, stabiliser count and
     validity, and the phase wherever the orbit is valid; both equal the
-    oracle.  Cases: square tori with their point groups, rectangles with
+    oracle, and the minimum-only scan finds the same representatives.  Cases: square tori with their point groups, rectangles with
     non-zero momenta (complex phases) and spin inversion; ring groups keep
     the scan (no torus)."""
     b = spined.mk_basis(spined.mk_group([spined.mk_symmetry(p, s) for p, s in gens]), n, w, z)
@@ -206,6 +206,8 @@ def test_torus_factorised_orbit(n, w, z, gens, shape):
     scan = spined.debug_orbit(b, st, 2)
     for q in (0, 2, 3):
         np.testing.assert_array_equal(fac[q], scan[q])
+    # the minimum-only scan (gather-cache fill of trivial-character sectors)
+    np.testing.assert_array_equal(spined.debug_orbit(b, st, 3)[0], scan[0])
     v = fac[3].astype(bool)
     np.testing.assert_array_equal(fac[1][v], scan[1][v])
     o = O.System(n, w, z, gens)

@@ -10,7 +10,7 @@ import pytest
 
 import spined
 from spined import lattice
-from helpers import Spec, engine, oracle, ring
+from helpers import Spec, engine, oracle, ring, torus
 
 pytestmark = pytest.mark.gpu
 
@@ -29,6 +29,12 @@ def _cases():
              "square4x4"),
         ring(14, k=None, p=None, z=1, name="xxz14_flip"),
         sx,
+        # generic orbits: torus factorisation with trivial characters (the
+        # fill's minimum-only scan) and with phases; a group that is no torus
+        torus(6, 4, reflect=True, z=1),
+        torus(4, 4, kx=2, ky=0, z=1),
+        Spec(16, 8, None, [([(i + 2) % 16 for i in range(16)], 0)],
+             [(lattice.HEISENBERG, lattice.ring_bonds(16))], "ring16_translation_by_2"),
     ]
 
 

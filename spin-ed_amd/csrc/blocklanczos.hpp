@@ -114,21 +114,26 @@ __global__ void __launch_bounds__(kBlock)
   }
 }
 
-// Scaled SVQB factors of the K x K Gram matrix G = W^T W (one thread):
+// Scaled SVQB factors of the K x K Gram matrix G = W^T W (one thread; every
+// loop over K is unrolled, so the matrices stay in registers):
 //   G~ = D^-1/2 G D^-1/2 = U L U^T,  X = D^-1/2 U L^-1/2,  F = L^1/2 U^T D^1/2
 // so that Q = W X is orthonormal and W = Q F.  Bout = F * Bin (Bin NULL: F).
 // status[0] = 1 when the block is (numerically) rank deficient.
 template <int K>
-__global__ void svqb_kernel(const double *__restrict__ G, double *__restrict__ X,
-                            const double *__restrict__ Bin, double *__restrict__ Bout,
-                            int *__restrict__ status) {
+__global__ void __launch_bounds__(64)
+    svqb_kernel(const double *__restrict__ G, double *__restrict__ X,
+                const double *__restrict__ Bin, double *__restrict__ Bout, int *__restrict__ status) {
   if (threadIdx.x != 0) return;
   double A[K][K], U[K][K], d[K];
-  for (int a = 0; a < K; ++a) d[a] = G[a * K + a] > 0.0 ? sqrt(G[a * K + a]) : 0.0;
   bool deficient = false;
-  for (int a = 0; a < K; ++a)
+#pragma unroll
+  for (int a = 0; a < K; ++a) {
+    d[a] = G[a * K + a] > 0.0 ? sqrt(G[a * K + a]) : 0.0;
     if (!(d[a] > 0.0)) deficient = true;
+  }
+#pragma unroll
   for (int a = 0; a < K; ++a)
+#pragma unroll
     for (int c = 0; c < K; ++c) {
       A[a][c] = deficient ? 0.0 : G[a * K + c] / (d[a] * d[c]);
       U[a][c] = a == c ? 1.0 : 0.0;
@@ -136,25 +141,32 @@ __global__ void svqb_kernel(const double *__restrict__ G, double *__restrict__ X
   // cyclic Jacobi on the K x K matrix
   for (int sweep = 0; sweep < 30 && !deficient; ++sweep) {
     double off = 0.0;
+#pragma unroll
     for (int p = 0; p < K; ++p)
+#pragma unroll
       for (int q = p + 1; q < K; ++q) off += A[p][q] * A[p][q];
     if (off < 1e-32) break;
+#pragma unroll
     for (int p = 0; p < K; ++p)
+#pragma unroll
       for (int q = p + 1; q < K; ++q) {
         if (A[p][q] == 0.0) continue;
         const double tau = (A[q][q] - A[p][p]) / (2.0 * A[p][q]);
         const double t = (tau >= 0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
         const double cs = 1.0 / sqrt(1.0 + t * t), sn = t * cs;
+#pragma unroll
         for (int r = 0; r < K; ++r) {
           const double ap = A[r][p], aq = A[r][q];
           A[r][p] = cs * ap - sn * aq;
           A[r][q] = sn * ap + cs * aq;
         }
+#pragma unroll
         for (int r = 0; r < K; ++r) {
           const double ap = A[p][r], aq = A[q][r];
           A[p][r] = cs * ap - sn * aq;
           A[q][r] = sn * ap + cs * aq;
         }
+#pragma unroll
         for (int r = 0; r < K; ++r) {
           const double up = U[r][p], uq = U[r][q];
           U[r][p] = cs * up - sn * uq;
@@ -163,22 +175,29 @@ __global__ void svqb_kernel(const double *__restrict__ G, double *__restrict__ X
       }
   }
   double lmax = 0.0;
+#pragma unroll
   for (int a = 0; a < K; ++a) lmax = fmax(lmax, A[a][a]);
+#pragma unroll
   for (int a = 0; a < K; ++a)
     if (!(A[a][a] > 1e-24 * lmax)) deficient = true;
   status[0] = deficient ? 1 : 0;
   double F[K][K];
+#pragma unroll
   for (int a = 0; a < K; ++a)
+#pragma unroll
     for (int c = 0; c < K; ++c) {
       const double l = deficient ? 1.0 : A[c][c];
       const double da = d[a] > 0.0 ? d[a] : 1.0;
       X[a * K + c] = U[a][c] / (da * sqrt(l));
       F[c][a] = sqrt(l) * U[a][c] * da; // F = L^1/2 U^T D^1/2
     }
+#pragma unroll
   for (int a = 0; a < K; ++a)
+#pragma unroll
     for (int c = 0; c < K; ++c) {
       double s = 0.0;
       if (Bin) {
+#pragma unroll
         for (int q = 0; q < K; ++q) s += F[a][q] * Bin[q * K + c];
       } else {
         s = F[a][c];

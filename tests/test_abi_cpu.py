@@ -185,6 +185,10 @@ def _torus_gens(l1, l2, kx, ky, reflect=False):
     (12, 6, 1, lattice.ring_symmetries(12, 0, 0), (0, 0)),
     (12, 6, None, [(lattice.ring_translation(12), 0)] + [([(i + 2) % 12 for i in range(12)], 0)],
      (0, 0)),
+    # translations along the rows only: no torus, the per-element scan
+    (16, 8, 1, _torus_gens(4, 4, 0, 0)[:1], (0, 0)),
+    # a 2-leg ladder (2 x 8 torus): rung swap and leg translations
+    (16, 8, -1, _torus_gens(2, 8, 0, 0), (2, 8)),
 ])
 def test_torus_factorised_orbit(n, w, z, gens, shape):
     """The generic orbit over the torus factorisation (a point element's
